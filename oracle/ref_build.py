@@ -1,0 +1,184 @@
+"""Build and run the reference (LAlbertoA/tlaloque, Fortran 90) from its own
+sources with flang, for one small problem, writing only under oracle/_ref/.
+
+TEST INFRASTRUCTURE ONLY (pins the C oracle; see oracle/__init__.py).
+
+The reference fixes its problem at compile time in two user files,
+``user/parameters.f90`` (the parameter module) and ``user/user.f90`` (initial
+conditions and boundary hooks); its own test harness regenerates both for
+every test (tests_utilities.jl:286-398 and :704-795) and compiles the rest of
+the tree unchanged.  This script does the same: it writes a parameters module
+and a user module for the problem, compiles every other file straight from
+``/root/reference/source`` (constants, globals, cooling, pointsource_gravity,
+sources, winds, HydroCore, init, boundaries, HLLC, mainHD -- the list of
+Makefile:15-30) with ``-O2 -cpp -DDOUBLEP -fdefault-real-8`` (Makefile:46-54),
+runs the program with BIN output (source/HydroCore.f90:277-283: UPrim with
+ghosts, little-endian) after every step, and reads the snapshots back.
+
+The initial state comes from a file (``ic.bin``) written here, so the
+reference and the C oracle start from the same bits.  Only this container
+runs this script: nothing on the GPU side reads /root/reference.
+"""
+from __future__ import annotations
+
+import glob
+import os
+import shutil
+import subprocess
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+REF = "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(HERE, "_ref")
+FLANG = shutil.which("flang") or shutil.which("amdflang") or "/opt/rocm/llvm/bin/flang"
+FFLAGS = ["-O2", "-cpp", "-DDOUBLEP", "-fdefault-real-8"]
+
+# Makefile:15-30, minus the two user files that a problem provides.
+REF_SOURCES_BEFORE_USER = ["source/constants.f90", None, "source/globals.f90", "source/cooling.f90",
+                           "source/pointsource_gravity.f90", "source/sources.f90",
+                           "source/winds.f90"]
+REF_SOURCES_AFTER_USER = ["source/HydroCore.f90", "source/init.f90", "source/boundaries.f90",
+                          "source/HLLC.f90", "source/mainHD.f90"]
+
+BC_NAME = {1: "BC_OUTFLOW", 2: "BC_REFLECTIVE", 3: "BC_PERIODIC", 4: "BC_INFLOW"}
+
+
+def available() -> bool:
+    return os.path.isdir(os.path.join(REF, "source")) and os.path.exists(FLANG)
+
+
+def _r(x: float) -> str:
+    """A real(8) literal that reads back to the same double."""
+    return repr(float(x)) + "_8"
+
+
+def _parameters_module(p, tfin: float, cooling_file: Optional[str]) -> str:
+    lb, rb, db, tb, fb, bb = (BC_NAME[int(b)] for b in p.bc)
+    lines = [
+        "module parameters",
+        "  use constants",
+        "  implicit none",
+        f"  integer, parameter :: LB = {lb}, RB = {rb}, TB = {tb}, DB = {db}, FB = {fb}, BB = {bb}",
+        f"  integer, parameter :: limtr = {int(p.limiter)}",
+        "  integer, parameter :: ndim = 3, neq = 5, size = 0",
+        f"  integer, parameter :: nxtot = {p.nx}, nytot = {p.ny}, nztot = {p.nz}",
+        "  integer, parameter :: choice = 2",
+        "  integer, parameter :: num_out = 100000",
+        "  integer, parameter :: outfile = BIN",
+        "  character(*), parameter :: outputpath = './DATA/'",
+    ]
+    if cooling_file:
+        lines.append(f"  character(*), parameter :: cooling_file = '{cooling_file}'")
+    lines += [
+        "  character(*), parameter :: posfile = 'none'",
+        "  integer, parameter :: nghost = 2",
+        f"  real, parameter :: xl = {_r(p.xl)}, xr = {_r(p.xr)}",
+        f"  real, parameter :: yl = {_r(p.yl)}, yr = {_r(p.yr)}",
+        f"  real, parameter :: zl = {_r(p.zl)}, zr = {_r(p.zr)}",
+        f"  real, parameter :: gamma = {_r(p.gamma)}",
+        f"  real, parameter :: mu0 = {_r(p.mu0)}",
+        "  real, parameter :: Gconst = GR",
+        f"  real, parameter :: cfl = {_r(p.cfl)}",
+        f"  real, parameter :: eta = {_r(p.eta)}",
+        f"  real, parameter :: tfin = {_r(tfin)}",
+        f"  real, parameter :: cooling_limit = {_r(p.cooling_limit)}",
+        "  integer, parameter :: mpix = 1, mpiy = 1, mpiz = 1",
+        "  integer, parameter :: nx = nxtot, ny = nytot, nz = nztot, np = 1",
+        "  integer, parameter :: mpi_real_kind = 0",
+        "  real, parameter :: dtout = tfin/num_out",
+        "  integer, parameter :: nxmax = nx+nghost, nymax = ny+nghost, nzmax = nz+nghost",
+        "  integer, parameter :: nxmin = 1-nghost, nymin = 1-nghost, nzmin = 1-nghost",
+        "  real, parameter :: dx = (xr-xl)/nxtot, dy = (yr-yl)/nytot, dz = (zr-zl)/nztot",
+        "  logical, parameter :: logged = .false.",
+        "  integer, parameter :: logu = 12",
+        "  character(*), parameter :: logfile = './log.dat'",
+        "  character(*), parameter :: paramFile = 'tlaloque.dat'",
+        "end module parameters",
+    ]
+    return "\n".join(lines) + "\n"
+
+
+def _user_module(p) -> str:
+    winds = list(p.winds)
+    lines = [
+        "module user",
+        "  use parameters",
+        "  use globals, only: posstars, Nstrs",
+        "  use winds",
+        "  use pgravity",
+        "  implicit none",
+        "  type(spherical_wind_type), dimension(:), allocatable :: spherical_wind",
+        "contains",
+        "  subroutine user_initconds(U)",
+        "    real, dimension(neq, nxmin:nxmax, nymin:nymax, nzmin:nzmax), intent(inout) :: U",
+        "    open(unit=77, file='ic.bin', access='stream', form='unformatted', status='old', &",
+        "         convert='LITTLE_ENDIAN')",
+        "    read(77) U",
+        "    close(77)",
+        f"    Nstrs = {len(winds)}",
+        "    allocate(spherical_wind(0:Nstrs-1))",
+        "    allocate(posstars(0:2,0:Nstrs-1))",
+    ]
+    for i, w in enumerate(winds):
+        for name, val in zip(("xc", "yc", "zc", "vx", "vy", "vz", "radius", "mdot", "vinf",
+                              "temp", "mu"), w.as_tuple()):
+            lines.append(f"    spherical_wind({i})%{name} = {_r(val)}")
+        lines.append(f"    posstars(:,{i}) = (/ {_r(w.xc)}, {_r(w.yc)}, {_r(w.zc)} /)")
+    lines += [
+        "  end subroutine user_initconds",
+        "  subroutine user_boundaries(U)",
+        "    real, dimension(neq, nxmin:nxmax, nymin:nymax, nzmin:nzmax), intent(inout) :: U",
+        "    integer :: i",
+        "    do i = 0, Nstrs-1",
+        "       call imposeSphericalWind(spherical_wind(i), U)",
+        "    enddo",
+        "  end subroutine user_boundaries",
+        "end module user",
+    ]
+    return "\n".join(lines) + "\n"
+
+
+def build_and_run(name: str, p, u0: np.ndarray, tfin: float, cooling: bool = False,
+                  timeout: float = 600.0) -> Dict[str, object]:
+    """Compile the reference for problem ``p`` and run it from ``u0`` (Fortran
+    layout, C shape (nz+4, ny+4, nx+4, 5)) until t >= tfin.
+
+    Returns {"prims": [UPrim after 0, 1, ... steps], "log": stdout}."""
+    if not available():
+        raise RuntimeError("reference sources or flang not available")
+    case = os.path.join(OUT, name)
+    src, obj, run = (os.path.join(case, d) for d in ("src", "obj", "run"))
+    if os.path.isdir(case):
+        shutil.rmtree(case)
+    for d in (src, obj, run, os.path.join(run, "DATA")):
+        os.makedirs(d)
+    cfile = os.path.join(REF, "Z1.0.dat") if cooling else None
+    with open(os.path.join(src, "parameters.f90"), "w") as fh:
+        fh.write(_parameters_module(p, tfin, cfile))
+    with open(os.path.join(src, "user.f90"), "w") as fh:
+        fh.write(_user_module(p))
+    flags = FFLAGS + (["-DCOOL"] if cooling else [])
+    objs: List[str] = []
+    order: Sequence[str] = REF_SOURCES_BEFORE_USER + ["user"] + REF_SOURCES_AFTER_USER
+    for s in order:
+        if s is None:
+            path = os.path.join(src, "parameters.f90")
+        elif s == "user":
+            path = os.path.join(src, "user.f90")
+        else:
+            path = os.path.join(REF, s)
+        o = os.path.join(obj, os.path.basename(path).replace(".f90", ".o"))
+        subprocess.run([FLANG, *flags, "-module-dir", obj, "-c", path, "-o", o], check=True,
+                       cwd=obj, capture_output=True)
+        objs.append(o)
+    exe = os.path.join(case, "tlaloque_ref")
+    subprocess.run([FLANG, *flags, *objs, "-o", exe], check=True, capture_output=True)
+    np.ascontiguousarray(u0, dtype="<f8").tofile(os.path.join(run, "ic.bin"))
+    res = subprocess.run([exe], cwd=run, check=True, capture_output=True, text=True,
+                         timeout=timeout)
+    files = sorted(glob.glob(os.path.join(run, "DATA", "points000N*.bin")))
+    shape = u0.shape
+    prims = [np.fromfile(f, dtype="<f8").reshape(shape) for f in files]
+    return {"prims": prims, "log": res.stdout}

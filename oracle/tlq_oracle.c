@@ -1,0 +1,537 @@
+/*
+ * tlq_oracle.c -- FP64 CPU restatement of the tlaloque HydroCore time step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the CPU baseline
+ * timed by bench.py.  Nothing in the product (tlaloque_amd, libtlq.so) calls
+ * it.  Each routine cites the reference routine it follows; arithmetic keeps
+ * the reference's operand order so that a build of the reference with the
+ * same compiler settings agrees to the last bits (libm aside).  Build with
+ * -ffp-contract=off (no fused multiply-add), as in oracle/Makefile.
+ *
+ * Layout: see tlq_oracle.h (Fortran (e,i,j,k), e fastest, two ghost layers).
+ */
+#include "tlq_oracle.h"
+
+#include <math.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Constants of source/constants.f90:6-15 (cgs). */
+#define ORC_AMU 1.660538782e-24
+#define ORC_KB 1.380650400e-16
+#define ORC_GR 6.67430e-8
+#define ORC_PI 3.14159265358979
+
+/* Fortran (e, i, j, k) with i, j, k in -1..n+2 (offset 0 at index -1) -> flat
+ * offset of component 0. */
+static inline size_t cell(const orc_params *p, int i, int j, int k) {
+    const size_t sx = (size_t)p->nx + 4, sy = (size_t)p->ny + 4;
+    return 5 * ((size_t)(i + 1) + sx * (size_t)(j + 1) + sx * sy * (size_t)(k + 1));
+}
+
+size_t orc_array_len(const orc_params *p) {
+    return (size_t)5 * (p->nx + 4) * (p->ny + 4) * (p->nz + 4);
+}
+
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+
+int orc_get_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* Cell-centre coordinates, source/HydroCore.f90:35-75 (single rank: coords = 0). */
+static inline double xcoord(const orc_params *p, int i) {
+    const double dx = (p->xr - p->xl) / p->nx;
+    return (p->xl + dx * (double)i) - dx / 2.0;
+}
+static inline double ycoord(const orc_params *p, int j) {
+    const double dy = (p->yr - p->yl) / p->ny;
+    return (p->yl + dy * (double)j) - dy / 2.0;
+}
+static inline double zcoord(const orc_params *p, int k) {
+    const double dz = (p->zr - p->zl) / p->nz;
+    return (p->zl + dz * (double)k) - dz / 2.0;
+}
+
+/* calc_prims, source/HydroCore.f90:1-33: all cells, ghosts included. */
+void orc_calc_prims(const orc_params *p, double *prim, const double *uu) {
+    const double g1 = p->gamma - 1.0;
+    const size_t ncell = orc_array_len(p) / 5;
+#pragma omp parallel for schedule(static)
+    for (size_t c = 0; c < ncell; ++c) {
+        const double *u = uu + 5 * c;
+        double *w = prim + 5 * c;
+        w[0] = u[0];
+        w[1] = u[1] / u[0];
+        w[2] = u[2] / u[0];
+        w[3] = u[3] / u[0];
+        double pr = g1 * (u[4] - 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) / u[0]);
+        w[4] = pr > 1e-20 ? pr : 1e-20; /* HydroCore.f90:21 */
+    }
+}
+
+/* csound, source/HydroCore.f90:157-168. */
+static inline double csound(double gamma, double pres, double den) {
+    return sqrt(gamma * pres / den);
+}
+
+/* time_step, source/HydroCore.f90:113-155 (serial branch, :148). */
+double orc_time_step(const orc_params *p, const double *prim) {
+    double mx = 0.0, my = 0.0, mz = 0.0;
+#pragma omp parallel for reduction(max : mx, my, mz) schedule(static)
+    for (int k = 1; k <= p->nz; ++k)
+        for (int j = 1; j <= p->ny; ++j)
+            for (int i = 1; i <= p->nx; ++i) {
+                const double *w = prim + cell(p, i, j, k);
+                const double cs = csound(p->gamma, w[4], w[0]);
+                const double ax = fabs(w[1]) + cs, ay = fabs(w[2]) + cs, az = fabs(w[3]) + cs;
+                if (ax > mx) mx = ax;
+                if (ay > my) my = ay;
+                if (az > mz) mz = az;
+            }
+    const double dx = (p->xr - p->xl) / p->nx, dy = (p->yr - p->yl) / p->ny,
+                 dz = (p->zr - p->zl) / p->nz;
+    double m = dx / mx;
+    if (dy / my < m) m = dy / my;
+    if (dz / mz < m) m = dz / mz;
+    return p->cfl * m / sqrt(3.0); /* HydroCore.f90:144 */
+}
+
+/* prim2f, source/HydroCore.f90:213-251.  dim = 0, 1, 2 for x, y, z. */
+static inline void prim2f(double gamma, const double *w, int dim, double *f) {
+    const int n = dim + 1;
+    const double vn = w[n];
+    f[0] = w[0] * vn;
+    for (int c = 1; c <= 3; ++c) {
+        const int a = c < n ? c : n, b = c < n ? n : c;
+        f[c] = w[0] * w[a] * w[b];
+        if (c == n) f[c] = f[c] + w[4];
+    }
+    f[4] = vn * (0.5 * w[0] * (w[1] * w[1] + w[2] * w[2] + w[3] * w[3]) +
+                 (gamma / (gamma - 1.0)) * w[4]);
+}
+
+/* average(), source/HLLC.f90:586-636 (identical in HLL.f90:345-395). */
+static inline double lim_avg(int opt, double a, double b) {
+    double s, c, d, r;
+    switch (opt) {
+    case ORC_LIM_NONE:
+        return 0.5 * (a + b);
+    case ORC_LIM_VANLEER:
+        if (a * b <= 0.0) return 0.0;
+        return a * b * (a + b) / (a * a + b * b);
+    case ORC_LIM_MINMOD:
+        s = copysign(1.0, a);
+        r = fabs(a) < s * b ? fabs(a) : s * b;
+        return s * (r > 0.0 ? r : 0.0);
+    case ORC_LIM_UMIST:
+        s = copysign(1.0, a);
+        c = 0.25 * a + 0.75 * b;
+        d = 0.75 * a + 0.25 * b;
+        r = 2.0 * fabs(a);
+        if (2.0 * s * b < r) r = 2.0 * s * b;
+        if (s * c < r) r = s * c;
+        if (s * d < r) r = s * d;
+        return s * (r > 0.0 ? r : 0.0);
+    case ORC_LIM_WOODWARD:
+        s = copysign(1.0, a);
+        c = 0.5 * (a + b);
+        r = 2.0 * fabs(a);
+        if (2.0 * s * b < r) r = 2.0 * s * b;
+        if (s * c < r) r = s * c;
+        return s * (r > 0.0 ? r : 0.0);
+    case ORC_LIM_SUPERBEE:
+        s = copysign(1.0, b);
+        c = 2.0 * fabs(b) < s * a ? 2.0 * fabs(b) : s * a;
+        d = fabs(b) < 2.0 * s * a ? fabs(b) : 2.0 * s * a;
+        r = 0.0;
+        if (c > r) r = c;
+        if (d > r) r = d;
+        return s * r;
+    default:
+        return NAN;
+    }
+}
+
+void orc_limiter_avg(int lim, const double *a, const double *b, double *out, int n) {
+    for (int t = 0; t < n; ++t) out[t] = lim_avg(lim, a[t], b[t]);
+}
+
+/* limiter, source/HLLC.f90:560-582: reconstructs pl (in place) from
+ * (pll, pl, pr) and pr from (pl, pr, prr). */
+static inline void limiter(int lim, const double *pll, double *pl, double *pr, const double *prr) {
+    for (int e = 0; e < 5; ++e) {
+        const double dl = pl[e] - pll[e], dm = pr[e] - pl[e], dr = prr[e] - pr[e];
+        const double al = lim_avg(lim, dl, dm), ar = lim_avg(lim, dm, dr);
+        pl[e] = pl[e] + 0.5 * al;
+        pr[e] = pr[e] - 0.5 * ar;
+    }
+}
+
+/* wavespeed, source/HLLC.f90:498-518 (Davis estimates + contact speed). */
+static inline void wavespeed(double gamma, const double *wl, const double *wr, int dim, double *sl,
+                             double *sr, double *sst) {
+    const int n = dim + 1;
+    const double csl = csound(gamma, wl[4], wl[0]), csr = csound(gamma, wr[4], wr[0]);
+    const double a = wl[n] - csl, b = wr[n] - csr;
+    const double c = wl[n] + csl, d = wr[n] + csr;
+    *sl = a < b ? a : b;
+    *sr = c > d ? c : d;
+    *sst = (wr[4] - wl[4] + wl[0] * wl[n] * (*sl - wl[n]) - wr[0] * wr[n] * (*sr - wr[n])) /
+           (wl[0] * (*sl - wl[n]) - wr[0] * (*sr - wr[n]));
+}
+
+/* One HLLC face: source/HLLC.f90:99-135 (choice 1) and 275-314 (choice 2).
+ * pl/pr are the (reconstructed) primitive states, ul/ur the cell-centred
+ * conserved states of the two cells that share the face. */
+void orc_hllc_face(const orc_params *p, const double *pl, const double *pr, const double *ul,
+                   const double *ur, int dim, double *f) {
+    const int n = dim + 1;
+    double sl, sr, sst, fs[5], us[5];
+    wavespeed(p->gamma, pl, pr, dim, &sl, &sr, &sst);
+    if (sl >= 0) {
+        prim2f(p->gamma, pl, dim, f);
+    } else if (sr <= 0) {
+        prim2f(p->gamma, pr, dim, f);
+    } else if (sst >= 0) {
+        prim2f(p->gamma, pl, dim, fs);
+        const double rs = pl[0] * (sl - pl[n]) / (sl - sst);
+        us[0] = rs;
+        for (int c = 1; c <= 3; ++c) us[c] = c == n ? rs * sst : rs * pl[c];
+        us[4] = rs * (ul[4] / pl[0] + (sst - pl[n]) * (sst + pl[4] / (pl[0] * (sl - pl[n]))));
+        for (int e = 0; e < 5; ++e) f[e] = fs[e] + sl * (us[e] - ul[e]);
+    } else if (sst <= 0) {
+        prim2f(p->gamma, pr, dim, fs);
+        const double rs = pr[0] * (sr - pr[n]) / (sr - sst);
+        us[0] = rs;
+        for (int c = 1; c <= 3; ++c) us[c] = c == n ? rs * sst : rs * pr[c];
+        us[4] = rs * (ur[4] / pr[0] + (sst - pr[n]) * (sst + pr[4] / (pr[0] * (sr - pr[n]))));
+        for (int e = 0; e < 5; ++e) f[e] = fs[e] + sr * (us[e] - ur[e]);
+    }
+    /* A NaN speed matches no branch and leaves f unassigned, as in the reference. */
+}
+
+/* One HLL face: source/HLL.f90:106-128 and 197-223 (wavespeed 299-317). */
+void orc_hll_face(const orc_params *p, const double *pl, const double *pr, const double *ul,
+                  const double *ur, int dim, double *f) {
+    double sl, sr, sst, fl[5], fr[5];
+    wavespeed(p->gamma, pl, pr, dim, &sl, &sr, &sst);
+    (void)sst;
+    if (sl >= 0) {
+        prim2f(p->gamma, pl, dim, f);
+    } else if (sr <= 0) {
+        prim2f(p->gamma, pr, dim, f);
+    } else {
+        prim2f(p->gamma, pl, dim, fl);
+        prim2f(p->gamma, pr, dim, fr);
+        for (int e = 0; e < 5; ++e)
+            f[e] = (sr * fl[e] - sl * fr[e] + sl * sr * (ur[e] - ul[e])) / (sr - sl);
+    }
+}
+
+/* HLLCFluxes / HLLFluxes, source/HLLC.f90:64-496, source/HLL.f90:71-297.
+ * Faces i = 0..nx, j = 0..ny, k = 0..nz in all three directions (HLLC.f90:94-96). */
+void orc_fluxes(const orc_params *p, const double *uu, const double *prim, double *ff,
+                double *gg, double *hh, int choice) {
+    double *out[3] = {ff, gg, hh};
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k <= p->nz; ++k)
+        for (int j = 0; j <= p->ny; ++j)
+            for (int i = 0; i <= p->nx; ++i) {
+                for (int dim = 0; dim < 3; ++dim) {
+                    const int di = dim == 0, dj = dim == 1, dk = dim == 2;
+                    const size_t c0 = cell(p, i, j, k), c1 = cell(p, i + di, j + dj, k + dk);
+                    double wl[5], wr[5];
+                    memcpy(wl, prim + c0, sizeof wl);
+                    memcpy(wr, prim + c1, sizeof wr);
+                    if (choice == 2) {
+                        const double *wll = prim + cell(p, i - di, j - dj, k - dk);
+                        const double *wrr = prim + cell(p, i + 2 * di, j + 2 * dj, k + 2 * dk);
+                        limiter(p->limiter, wll, wl, wr, wrr);
+                    }
+                    double *f = out[dim] + c0;
+                    if (p->solver == ORC_SOLVER_HLL)
+                        orc_hll_face(p, wl, wr, uu + c0, uu + c1, dim, f);
+                    else
+                        orc_hllc_face(p, wl, wr, uu + c0, uu + c1, dim, f);
+                }
+            }
+}
+
+/* Godunov, source/HLLC.f90:1-62: interior update; src (if non-NULL) is the
+ * S_Solver source added as dt*S (HLLC.f90:39-41). */
+void orc_godunov(const orc_params *p, double *uout, const double *uu, const double *ff,
+                 const double *gg, const double *hh, const double *src, double dt) {
+    const double dx = (p->xr - p->xl) / p->nx, dy = (p->yr - p->yl) / p->ny,
+                 dz = (p->zr - p->zl) / p->nz;
+    const double dtdx = dt / dx, dtdy = dt / dy, dtdz = dt / dz;
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= p->nz; ++k)
+        for (int j = 1; j <= p->ny; ++j)
+            for (int i = 1; i <= p->nx; ++i) {
+                const size_t c = cell(p, i, j, k);
+                const size_t cx = cell(p, i - 1, j, k), cy = cell(p, i, j - 1, k),
+                             cz = cell(p, i, j, k - 1);
+                for (int e = 0; e < 5; ++e) {
+                    uout[c + e] = uu[c + e] - dtdx * (ff[c + e] - ff[cx + e]) -
+                                  dtdy * (gg[c + e] - gg[cy + e]) - dtdz * (hh[c + e] - hh[cz + e]);
+                    if (src) uout[c + e] = uout[c + e] + dt * src[c + e];
+                }
+            }
+}
+
+/* Copy one full ghost plane: dst layer <- src layer along axis, with the
+ * normal momentum component (1 + axis) negated when sign < 0. */
+static void copy_plane(const orc_params *p, double *u, int axis, int dst, int src, int negate) {
+    const int lo = -1;
+    const int hi0 = p->nx + 2, hi1 = p->ny + 2, hi2 = p->nz + 2;
+    if (axis == 0) {
+        for (int k = lo; k <= hi2; ++k)
+            for (int j = lo; j <= hi1; ++j) {
+                double *d = u + cell(p, dst, j, k);
+                const double *s = u + cell(p, src, j, k);
+                for (int e = 0; e < 5; ++e) d[e] = (negate && e == 1) ? -s[e] : s[e];
+            }
+    } else if (axis == 1) {
+        for (int k = lo; k <= hi2; ++k)
+            for (int i = lo; i <= hi0; ++i) {
+                double *d = u + cell(p, i, dst, k);
+                const double *s = u + cell(p, i, src, k);
+                for (int e = 0; e < 5; ++e) d[e] = (negate && e == 2) ? -s[e] : s[e];
+            }
+    } else {
+        for (int j = lo; j <= hi1; ++j)
+            for (int i = lo; i <= hi0; ++i) {
+                double *d = u + cell(p, i, j, dst);
+                const double *s = u + cell(p, i, j, src);
+                for (int e = 0; e < 5; ++e) d[e] = (negate && e == 3) ? -s[e] : s[e];
+            }
+    }
+}
+
+/* boundariesI serial branch, source/boundaries.f90:179-259: one ghost layer,
+ * order left, right (x), top, down (y), back, front (z). */
+void orc_boundaries1(const orc_params *p, double *u) {
+    const int n[3] = {p->nx, p->ny, p->nz};
+    /* (axis, side) in the reference's order; side 0 = low face, 1 = high face. */
+    const int order[6][2] = {{0, 0}, {0, 1}, {1, 1}, {1, 0}, {2, 1}, {2, 0}};
+    for (int s = 0; s < 6; ++s) {
+        const int axis = order[s][0], side = order[s][1];
+        const int bc = p->bc[2 * axis + side];
+        const int g = side ? n[axis] + 1 : 0;          /* ghost layer */
+        const int in = side ? n[axis] : 1;             /* adjacent interior layer */
+        const int wrap = side ? 1 : n[axis];           /* periodic partner */
+        if (bc == ORC_BC_OUTFLOW) copy_plane(p, u, axis, g, in, 0);
+        else if (bc == ORC_BC_REFLECTIVE) copy_plane(p, u, axis, g, in, 1);
+        else if (bc == ORC_BC_PERIODIC) copy_plane(p, u, axis, g, wrap, 0);
+        /* BC_INFLOW is a no-op in the reference (boundaries.f90:191-193). */
+    }
+}
+
+/* boundariesII serial branch, source/boundaries.f90:432-533: two ghost
+ * layers, order left, right, down, top, front, back. */
+void orc_boundaries2(const orc_params *p, double *u) {
+    const int n[3] = {p->nx, p->ny, p->nz};
+    for (int axis = 0; axis < 3; ++axis)
+        for (int side = 0; side < 2; ++side) {
+            const int bc = p->bc[2 * axis + side];
+            const int N = n[axis];
+            if (side == 0) {
+                if (bc == ORC_BC_PERIODIC) {
+                    copy_plane(p, u, axis, -1, N - 1, 0);
+                    copy_plane(p, u, axis, 0, N, 0);
+                } else if (bc == ORC_BC_OUTFLOW) {
+                    copy_plane(p, u, axis, 0, 1, 0);
+                    copy_plane(p, u, axis, -1, 1, 0);
+                } else if (bc == ORC_BC_REFLECTIVE) {
+                    copy_plane(p, u, axis, 0, 1, 1);
+                    copy_plane(p, u, axis, -1, 2, 1);
+                }
+            } else {
+                if (bc == ORC_BC_PERIODIC) {
+                    copy_plane(p, u, axis, N + 1, 1, 0);
+                    copy_plane(p, u, axis, N + 2, 2, 0);
+                } else if (bc == ORC_BC_OUTFLOW) {
+                    copy_plane(p, u, axis, N + 1, N, 0);
+                    copy_plane(p, u, axis, N + 2, N, 0);
+                } else if (bc == ORC_BC_REFLECTIVE) {
+                    copy_plane(p, u, axis, N + 1, N, 1);
+                    copy_plane(p, u, axis, N + 2, N - 1, 1);
+                }
+            }
+        }
+}
+
+/* imposeSphericalWind for every wind, source/winds.f90:32-90 called from
+ * user_boundaries, user/user.f90:57-70: all cells, ghosts included. */
+void orc_winds(const orc_params *p, double *u) {
+    for (int w = 0; w < p->n_winds; ++w) {
+        const orc_wind *wd = &p->winds[w];
+#pragma omp parallel for schedule(static)
+        for (int k = -1; k <= p->nz + 2; ++k)
+            for (int j = -1; j <= p->ny + 2; ++j)
+                for (int i = -1; i <= p->nx + 2; ++i) {
+                    const double x = xcoord(p, i), y = ycoord(p, j), z = zcoord(p, k);
+                    const double dist = sqrt((x - wd->xc) * (x - wd->xc) + (y - wd->yc) * (y - wd->yc) +
+                                             (z - wd->zc) * (z - wd->zc));
+                    if (dist < wd->radius) {
+                        const double dens = wd->mdot / (wd->vinf * dist * dist * 4.0 * ORC_PI);
+                        const double vx = wd->vinf * (x - wd->xc) / dist + wd->vx;
+                        const double vy = wd->vinf * (y - wd->yc) / dist + wd->vy;
+                        const double vz = wd->vinf * (z - wd->zc) / dist + wd->vz;
+                        const double pres = dens * ORC_KB * wd->temp / (wd->mu * ORC_AMU);
+                        double *c = u + cell(p, i, j, k);
+                        c[0] = dens;
+                        c[1] = dens * vx;
+                        c[2] = dens * vy;
+                        c[3] = dens * vz;
+                        c[4] = 0.5 * dens * (vx * vx + vy * vy + vz * vz) + pres / (p->gamma - 1.0);
+                    }
+                }
+    }
+}
+
+/* step, source/HydroCore.f90:170-211: UPP = U; interior U = UP + eta*Lap(UPP);
+ * every ghost cell of U takes UP's value.  Time/iteration bookkeeping
+ * (:208-209) is the caller's. */
+void orc_step(const orc_params *p, double *u, double *up, double *upp, double dt) {
+    (void)dt;
+    const size_t len = orc_array_len(p);
+    memcpy(upp, u, len * sizeof(double));
+    const double eta = p->eta;
+#pragma omp parallel for schedule(static)
+    for (int k = -1; k <= p->nz + 2; ++k)
+        for (int j = -1; j <= p->ny + 2; ++j)
+            for (int i = -1; i <= p->nx + 2; ++i) {
+                const size_t c = cell(p, i, j, k);
+                const int interior = i >= 1 && i <= p->nx && j >= 1 && j <= p->ny && k >= 1 &&
+                                     k <= p->nz;
+                if (!interior) {
+                    for (int e = 0; e < 5; ++e) u[c + e] = up[c + e];
+                    continue;
+                }
+                const size_t xp = cell(p, i + 1, j, k), xm = cell(p, i - 1, j, k);
+                const size_t yp = cell(p, i, j + 1, k), ym = cell(p, i, j - 1, k);
+                const size_t zp = cell(p, i, j, k + 1), zm = cell(p, i, j, k - 1);
+                for (int e = 0; e < 5; ++e)
+                    u[c + e] = up[c + e] + eta * (upp[xp + e] + upp[xm + e] + upp[yp + e] +
+                                                  upp[ym + e] + upp[zp + e] + upp[zm + e] -
+                                                  6.0 * upp[c + e]);
+            }
+}
+
+/* find_coef, source/cooling.f90:121-144: first row with T_i > temp, linear
+ * interpolation of log10 Lambda in log10 T.  The reference leaves the result
+ * unassigned for temp >= T_max (the loop falls through); here that case
+ * returns the last row's Lambda. */
+double orc_find_coef(const orc_params *p, double temp) {
+    for (int i = 1; i < p->n_table; ++i) {
+        if (p->table_T[i] > temp) {
+            const double T0 = log10(p->table_T[i - 1]), C0 = p->table_L[i - 1];
+            const double T1 = log10(p->table_T[i]), C1 = p->table_L[i];
+            return pow(10.0, (C1 - C0) / (T1 - T0) * (log10(temp) - T0) + C0);
+        }
+    }
+    return pow(10.0, p->table_L[p->n_table - 1]);
+}
+
+/* apply_cooling, source/cooling.f90:47-119 with calcTemp (:146-159). */
+void orc_cooling(const orc_params *p, double *prim, double *u, double dt) {
+    const double Tmin = p->table_T[0]; /* cooling.f90:42 */
+    const double g1 = p->gamma - 1.0;
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= p->nz; ++k)
+        for (int j = 1; j <= p->ny; ++j)
+            for (int i = 1; i <= p->nx; ++i) {
+                const size_t c = cell(p, i, j, k);
+                double *w = prim + c;
+                const double temp = w[4] * p->mu0 * ORC_AMU / (w[0] * ORC_KB);
+                if (!(temp >= Tmin)) continue;
+                double eth = w[4] / g1;
+                const double vel2 = w[1] * w[1] + w[2] * w[2] + w[3] * w[3];
+                const double ek = 0.5 * w[0] * vel2;
+                const double aloss = orc_find_coef(p, temp);
+                const double numdens = w[0] / (p->mu0 * ORC_AMU);
+                const double ce = aloss * (numdens * numdens) / eth;
+                double f = exp(-ce * dt);
+                if (f < 1.0 - p->cooling_limit) f = 1.0 - p->cooling_limit;
+                const double new_temp = temp * f;
+                if (new_temp < 10.0) f = 10.0 / temp;
+                w[4] = w[4] * f;
+                eth = w[4] / g1;
+                u[c + 4] = ek + eth;
+            }
+}
+
+/* Point-mass potential at a cell centre: pointsource_gravity.f90:12-39
+ * (Phi -= G*M/r per mass, summed in order) with optional Plummer softening. */
+static inline double potential(const orc_params *p, int i, int j, int k) {
+    const double x = xcoord(p, i), y = ycoord(p, j), z = zcoord(p, k);
+    const double eps2 = p->softening * p->softening;
+    double phi = 0.0;
+    for (int s = 0; s < p->n_masses; ++s) {
+        const double *m = p->masses + 4 * s;
+        const double r = sqrt((m[0] - x) * (m[0] - x) + (m[1] - y) * (m[1] - y) +
+                              (m[2] - z) * (m[2] - z) + eps2);
+        phi = phi - ORC_GR * m[3] / r;
+    }
+    return phi;
+}
+
+/* S_Solver, source/sources.f90:872-897, with Phi the point-mass potential. */
+void orc_gravity_source(const orc_params *p, const double *prim, double *src) {
+    const double dx = (p->xr - p->xl) / p->nx, dy = (p->yr - p->yl) / p->ny,
+                 dz = (p->zr - p->zl) / p->nz;
+    memset(src, 0, orc_array_len(p) * sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= p->nz; ++k)
+        for (int j = 1; j <= p->ny; ++j)
+            for (int i = 1; i <= p->nx; ++i) {
+                const double g0 = (potential(p, i + 1, j, k) - potential(p, i - 1, j, k)) / (2.0 * dx);
+                const double g1 = (potential(p, i, j + 1, k) - potential(p, i, j - 1, k)) / (2.0 * dy);
+                const double g2 = (potential(p, i, j, k + 1) - potential(p, i, j, k - 1)) / (2.0 * dz);
+                const size_t c = cell(p, i, j, k);
+                const double *w = prim + c;
+                double *s = src + c;
+                s[0] = 0.0;
+                s[1] = -w[0] * g0;
+                s[2] = -w[0] * g1;
+                s[3] = -w[0] * g2;
+                s[4] = -(w[1] * w[0] * g0) - w[2] * w[0] * g1 - w[3] * w[0] * g2;
+            }
+}
+
+/* One iteration of the main loop, source/mainHD.f90:74-99. */
+double orc_advance(const orc_params *p, double *u, double *up, double *upp, double *prim,
+                   double *ff, double *gg, double *hh, double *src) {
+    const int grav = p->n_masses > 0 && src != NULL;
+    const double dt = orc_time_step(p, prim);            /* :74 */
+    orc_fluxes(p, u, prim, ff, gg, hh, 1);               /* :76 */
+    const double dth = dt / 2.0;                         /* :78 */
+    if (grav) orc_gravity_source(p, prim, src);
+    orc_godunov(p, upp, u, ff, gg, hh, grav ? src : NULL, dth); /* :80 */
+    orc_boundaries2(p, upp);                             /* :82 */
+    orc_winds(p, upp);
+    orc_calc_prims(p, prim, upp);                        /* :84 */
+    orc_fluxes(p, upp, prim, ff, gg, hh, 2);             /* :86 */
+    if (grav) orc_gravity_source(p, prim, src);
+    orc_godunov(p, up, u, ff, gg, hh, grav ? src : NULL, dt); /* :88 */
+    orc_boundaries1(p, up);                              /* :90 */
+    orc_winds(p, up);
+    orc_step(p, u, up, upp, dt);                         /* :92 */
+    orc_calc_prims(p, prim, u);                          /* :94 */
+    if (p->cooling) orc_cooling(p, prim, u, dt);         /* :96-98 */
+    return dt;
+}
