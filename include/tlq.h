@@ -1,0 +1,160 @@
+/*
+ * tlq.h -- C-ABI of the MI355X (gfx950) HydroCore time step.
+ *
+ * Drop-in for the hot path of tlaloque's main loop (source/mainHD.f90:61-100).
+ * The reference calls implicit-interface Fortran subroutines on module arrays
+ * (source/globals.f90:5-11); a Fortran caller binds these functions with
+ * ISO_C_BINDING (tlaloque_amd/fortran/tlq_bindings.f90).  Every function
+ * returns 0 on success or a negative TLQ_ERR_* code; the last error message
+ * is available from tlq_last_error().
+ *
+ * State lives on the GPU as structure-of-arrays (one contiguous array per
+ * conserved variable, x fastest, two ghost layers).  Host arrays crossing
+ * this boundary use the reference's Fortran layout: real(8) U(5, -1:nx+2,
+ * -1:ny+2, -1:nz+2), component fastest (init.f90:82-88).
+ *
+ * Threading: one host thread per context at a time.  All work of a context
+ * is ordered on one HIP stream; functions that return data synchronise it.
+ */
+#ifndef TLQ_H
+#define TLQ_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define TLQ_OK 0
+#define TLQ_ERR_ARG -1          /* invalid argument (bad id, size, pointer) */
+#define TLQ_ERR_HIP -2          /* HIP runtime failure */
+#define TLQ_ERR_NOMEM -3        /* device allocation failed */
+#define TLQ_ERR_UNSUPPORTED -4  /* valid in the reference but not supported here */
+#define TLQ_ERR_STATE -5        /* call out of order (e.g. advance before upload) */
+#define TLQ_ERR_NONFINITE -6    /* non-finite or non-positive density found */
+
+/* Boundary ids, source/constants.f90:20-23. */
+#define TLQ_BC_OUTFLOW 1
+#define TLQ_BC_REFLECTIVE 2
+#define TLQ_BC_PERIODIC 3
+#define TLQ_BC_INFLOW 4   /* a no-op stub in the reference; rejected here */
+
+/* Limiter ids, source/constants.f90:26-31. */
+#define TLQ_LIM_NONE 1
+#define TLQ_LIM_VANLEER 2
+#define TLQ_LIM_MINMOD 3
+#define TLQ_LIM_UMIST 4
+#define TLQ_LIM_WOODWARD 5
+#define TLQ_LIM_SUPERBEE 6
+
+/* Riemann solver: source/HLLC.f90 (default) or source/HLL.f90. */
+#define TLQ_SOLVER_HLLC 0
+#define TLQ_SOLVER_HLL 1
+
+/* Precision: all-FP64, or FP32 storage and sweeps with the pressure recovery
+ * E - rho v^2/2 and the conserved update accumulated in FP64. */
+#define TLQ_FP64 0
+#define TLQ_FP32 1
+
+/* Cooling: off, exact (cooling.f90:47-119), or the learned surrogate with a
+ * per-cell fallback to the exact integrator. */
+#define TLQ_COOL_OFF 0
+#define TLQ_COOL_EXACT 1
+#define TLQ_COOL_SURROGATE 2
+
+/* Device arrays of the reference's globals (source/globals.f90:5-11). */
+#define TLQ_BUF_U 0
+#define TLQ_BUF_UP 1
+#define TLQ_BUF_UPP 2
+#define TLQ_BUF_UPRIM 3
+#define TLQ_BUF_F 4
+#define TLQ_BUF_G 5
+#define TLQ_BUF_H 6
+#define TLQ_BUF_S 7
+
+/* Problem parameters: user/parameters.f90:10-49 as run-time values. */
+typedef struct tlq_params {
+    int nx, ny, nz;                 /* interior cells (parameters.f90:63-65) */
+    double xl, xr, yl, yr, zl, zr;  /* domain (parameters.f90:37-42) */
+    double gamma;                   /* parameters.f90:43 */
+    double mu0;                     /* parameters.f90:44 */
+    double cfl;                     /* parameters.f90:46 */
+    double eta;                     /* artificial viscosity, parameters.f90:47 */
+    double cooling_limit;           /* parameters.f90:49 */
+    int limiter;                    /* TLQ_LIM_*, parameters.f90:18 */
+    int bc[6];                      /* LB, RB, DB, TB, FB, BB (x-, x+, y-, y+, z-, z+) */
+    int solver;                     /* TLQ_SOLVER_* */
+    int precision;                  /* TLQ_FP64 / TLQ_FP32 */
+    int cooling;                    /* TLQ_COOL_* */
+    double softening;               /* Plummer length of the point masses (0: 1/r) */
+} tlq_params;
+
+/* spherical_wind_type, source/winds.f90:5-13. */
+typedef struct tlq_wind {
+    double xc, yc, zc, vx, vy, vz, radius, mdot, vinf, temp, mu;
+} tlq_wind;
+
+typedef struct tlq_ctx tlq_ctx;
+
+/* Lifecycle: replaces initmain's allocation (source/init.f90:82-88). */
+int tlq_create(const tlq_params *params, tlq_ctx **out);
+int tlq_destroy(tlq_ctx *ctx);
+const char *tlq_last_error(void);
+
+/* Sources.  Winds: the spherical_wind array driven by user_boundaries
+ * (user/user.f90:57-70, winds.f90:32-90).  Point masses: x, y, z, mass per
+ * mass (pointsource_gravity.f90:12-39; source term sources.f90:872-897).
+ * Cooling table: loaddata's table (cooling.f90:21-45), T and log10 Lambda. */
+int tlq_set_winds(tlq_ctx *ctx, const tlq_wind *winds, int n);
+int tlq_set_point_masses(tlq_ctx *ctx, const double *xyzm, int n);
+int tlq_set_cooling_table(tlq_ctx *ctx, const double *temp, const double *log_lambda, int n);
+
+/* Host <-> device copies of one global array in the reference's Fortran
+ * layout (5 x (nx+4) x (ny+4) x (nz+4) doubles). */
+int tlq_upload(tlq_ctx *ctx, int buf, const double *host_aos);
+int tlq_download(tlq_ctx *ctx, int buf, double *host_aos);
+
+/* Per-routine drop-ins, each acting on the device globals like its
+ * reference counterpart.  dst/src name TLQ_BUF_* arrays. */
+int tlq_calc_prims(tlq_ctx *ctx, int src);                        /* HydroCore.f90:1-33 */
+int tlq_time_step(tlq_ctx *ctx, double *dt);                      /* HydroCore.f90:113-155 */
+int tlq_fluxes(tlq_ctx *ctx, int src, int choice);               /* HLLC.f90:64-496 / HLL.f90:71-297 */
+int tlq_godunov(tlq_ctx *ctx, int dst, double dt);                /* HLLC.f90:1-62 (+ S_Solver) */
+int tlq_borders(tlq_ctx *ctx, int buf, int order);                /* boundaries.f90:1-26 */
+int tlq_step(tlq_ctx *ctx, double *t, int *it, double dt);        /* HydroCore.f90:170-211 */
+int tlq_cooling(tlq_ctx *ctx, double dt);                         /* cooling.f90:9-19 */
+
+/* Fused time step: one iteration of mainHD.f90:74-99 in two kernels.
+ * tlq_init_state performs mainHD.f90:38-44 (borders(U,1), calc_prims) on the
+ * uploaded TLQ_BUF_U.  tlq_advance runs one step and, when dt/t/it are not
+ * NULL, synchronises and returns them; tlq_advance_n enqueues n steps
+ * without synchronising. */
+int tlq_init_state(tlq_ctx *ctx);
+int tlq_advance(tlq_ctx *ctx, double *dt, double *t, int *it);
+int tlq_advance_n(tlq_ctx *ctx, int n);
+int tlq_get_time(tlq_ctx *ctx, double *dt, double *t, int *it);
+int tlq_sync(tlq_ctx *ctx);
+
+/* Diagnostics: device-side check that every interior density is finite and
+ * positive and every energy finite; *bad receives the count of bad cells. */
+int tlq_check_state(tlq_ctx *ctx, long long *bad);
+/* Fraction of interior cells that took the exact-cooling fallback in the
+ * last surrogate-cooling step. */
+int tlq_fallback_fraction(tlq_ctx *ctx, double *frac);
+
+/* Timing: HIP events on the context's stream.  tlq_timer_* brackets any
+ * enqueued work; with kernel timing on, every kernel launch of the fused step
+ * is bracketed and its durations accumulate per kernel (names as in
+ * tlq_kernel_stats: "stage1", "stage2", "cooling", ...). */
+int tlq_timer_start(tlq_ctx *ctx);
+int tlq_timer_stop(tlq_ctx *ctx, double *ms);
+int tlq_kernel_timing(tlq_ctx *ctx, int enable);
+int tlq_kernel_stats(tlq_ctx *ctx, int index, char *name, int name_len, double *total_ms,
+                     long long *launches);
+
+/* Sizes: bytes of one device field and the number of cells per field. */
+int tlq_layout(tlq_ctx *ctx, long long *cells_per_field, int *pitch_x, int *origin_x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TLQ_H */

@@ -1,0 +1,708 @@
+// tlq_api.hip -- host side of the C-ABI declared in include/tlq.h.
+//
+// One context owns the device copies of the reference's global arrays
+// (source/globals.f90:5-11) in structure-of-arrays layout, the problem
+// description, and one HIP stream on which every call is ordered.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tlq.h"
+#include "tlq_kernels.hpp"
+#include "tlq_launch.hpp"
+
+using namespace tlq;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(TLQ_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_));    \
+    } while (0)
+
+struct PendingTiming {
+    int kernel;
+    hipEvent_t a, b;
+};
+
+const char *const kKernelNames[] = {"stage1", "stage2", "init", "cooling"};
+constexpr int kNumKernels = 4;
+
+}  // namespace
+
+struct tlq_ctx {
+    tlq_params prm{};
+    Problem host{};            // host copy of the problem description
+    Problem *dprob = nullptr;  // device copy
+    Grid g{};
+    hipStream_t stream = nullptr;
+    int prec = TLQ_FP64;
+    size_t esize = 8;  // bytes per element
+    // Fused state: U ping-pong and U^{1/2}; per-routine arrays on demand.
+    void *u[2] = {nullptr, nullptr};
+    void *uh = nullptr;
+    void *bufs[8] = {};  // TLQ_BUF_*; U aliases u[cur]
+    int cur = 0;          // which u[] holds U^n
+    StepState *st = nullptr;
+    unsigned long long *scratch_max = nullptr;  // per-routine time_step maxima
+    unsigned long long *scratch_cnt = nullptr;
+    bool fused_ready = false;
+    Tiling tiling{};
+    // timing
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool ktiming = false;
+    std::vector<PendingTiming> pending;
+    double ktotal[kNumKernels] = {};
+    long long kcount[kNumKernels] = {};
+    double fallback = 0.0;
+};
+
+namespace {
+
+bool valid_bc(int b) { return b == TLQ_BC_OUTFLOW || b == TLQ_BC_REFLECTIVE || b == TLQ_BC_PERIODIC; }
+
+long long ncell_box(const Grid &g) { return (long long)(g.nx + 4) * (g.ny + 4) * (g.nz + 4); }
+long long ninterior(const Grid &g) { return (long long)g.nx * g.ny * g.nz; }
+
+void *&buf_ref(tlq_ctx *c, int b) {
+    if (b == TLQ_BUF_U) return c->u[c->cur];
+    return c->bufs[b];
+}
+
+int ensure_buf(tlq_ctx *c, int b) {
+    void *&p = buf_ref(c, b);
+    if (p) return TLQ_OK;
+    if (b == TLQ_BUF_UPP && c->uh) {  // U^{1/2} of the fused step is the reference's UPP
+        p = c->uh;
+        return TLQ_OK;
+    }
+    const size_t bytes = (size_t)5 * c->g.nc * c->esize;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        p = nullptr;
+        return fail(TLQ_ERR_NOMEM, "hipMalloc failed for a state array");
+    }
+    if (hipMemsetAsync(p, 0, bytes, c->stream) != hipSuccess)
+        return fail(TLQ_ERR_HIP, "hipMemsetAsync failed");
+    return TLQ_OK;
+}
+
+int push_problem(tlq_ctx *c) {
+    HIPCHK(hipMemcpyAsync(c->dprob, &c->host, sizeof(Problem), hipMemcpyHostToDevice, c->stream));
+    return TLQ_OK;
+}
+
+template <class R> const R *cp(void *p) { return static_cast<const R *>(p); }
+template <class R> R *mp(void *p) { return static_cast<R *>(p); }
+
+// Time one launch when kernel timing is on.
+struct KTimer {
+    tlq_ctx *c;
+    int k;
+    hipEvent_t a = nullptr, b = nullptr;
+    KTimer(tlq_ctx *ctx, int kernel) : c(ctx), k(kernel) {
+        if (!c->ktiming) return;
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        hipEventRecord(a, c->stream);
+    }
+    ~KTimer() {
+        if (!c->ktiming) return;
+        hipEventRecord(b, c->stream);
+        c->pending.push_back({k, a, b});
+    }
+};
+
+void resolve_timings(tlq_ctx *c) {
+    if (c->pending.empty()) return;
+    hipStreamSynchronize(c->stream);
+    for (auto &p : c->pending) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, p.a, p.b);
+        c->ktotal[p.kernel] += ms;
+        c->kcount[p.kernel] += 1;
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    c->pending.clear();
+}
+
+// ---- precision-dispatched operations --------------------------------------
+
+template <class R> int do_borders(tlq_ctx *c, void *buf, int order) {
+    const Grid &g = c->g;
+    const int n[3] = {g.nx, g.ny, g.nz};
+    const long long plane[3] = {(long long)(g.ny + 4) * (g.nz + 4), (long long)(g.nx + 4) * (g.nz + 4),
+                                (long long)(g.nx + 4) * (g.ny + 4)};
+    auto copy = [&](int axis, int dst, int src, int neg) -> int {
+        HIPCHK(Launch<R>::bc_plane(c->stream, c->dprob, mp<R>(buf), axis, dst, src, neg,
+                                   plane[axis]));
+        return TLQ_OK;
+    };
+    int rc;
+    if (order == 1) {
+        // boundariesI, boundaries.f90:179-259: x-, x+, y+, y-, z+, z-
+        const int seq[6][2] = {{0, 0}, {0, 1}, {1, 1}, {1, 0}, {2, 1}, {2, 0}};
+        for (auto &s : seq) {
+            const int axis = s[0], side = s[1], bc = c->prm.bc[2 * axis + side];
+            const int gl = side ? n[axis] + 1 : 0, in = side ? n[axis] : 1, wrap = side ? 1 : n[axis];
+            if (bc == TLQ_BC_OUTFLOW) rc = copy(axis, gl, in, 0);
+            else if (bc == TLQ_BC_REFLECTIVE) rc = copy(axis, gl, in, 1);
+            else rc = copy(axis, gl, wrap, 0);
+            if (rc) return rc;
+        }
+    } else {
+        // boundariesII, boundaries.f90:432-533: x-, x+, y-, y+, z-, z+
+        for (int axis = 0; axis < 3; ++axis)
+            for (int side = 0; side < 2; ++side) {
+                const int bc = c->prm.bc[2 * axis + side], N = n[axis];
+                int d0, s0, d1, s1, neg = 0;
+                if (side == 0) {
+                    if (bc == TLQ_BC_PERIODIC) { d0 = -1; s0 = N - 1; d1 = 0; s1 = N; }
+                    else if (bc == TLQ_BC_OUTFLOW) { d0 = 0; s0 = 1; d1 = -1; s1 = 1; }
+                    else { d0 = 0; s0 = 1; d1 = -1; s1 = 2; neg = 1; }
+                } else {
+                    if (bc == TLQ_BC_PERIODIC) { d0 = N + 1; s0 = 1; d1 = N + 2; s1 = 2; }
+                    else if (bc == TLQ_BC_OUTFLOW) { d0 = N + 1; s0 = N; d1 = N + 2; s1 = N; }
+                    else { d0 = N + 1; s0 = N; d1 = N + 2; s1 = N - 1; neg = 1; }
+                }
+                if ((rc = copy(axis, d0, s0, neg))) return rc;
+                if ((rc = copy(axis, d1, s1, neg))) return rc;
+            }
+    }
+    if (c->host.nwinds > 0) {
+        HIPCHK(Launch<R>::winds(c->stream, c->dprob, mp<R>(buf), -1, -1, -1, g.nx + 4, g.ny + 4,
+                                g.nz + 4));
+    }
+    return TLQ_OK;
+}
+
+template <class R> int do_init_maxima(tlq_ctx *c) {
+    KTimer kt(c, 2);
+    const int par = c->cur;  // parity of the step about to run
+    HIPCHK(hipMemsetAsync(&c->st->maxima[par][0], 0, 4 * sizeof(unsigned long long), c->stream));
+    HIPCHK(Launch<R>::maxima(c->stream, c->dprob, cp<R>(c->u[c->cur]), 0, &c->st->maxima[par][0],
+                             ninterior(c->g)));
+    c->fused_ready = true;
+    return TLQ_OK;
+}
+
+template <class R> int do_advance(tlq_ctx *c, int nsteps) {
+    if (!c->fused_ready) {
+        int rc = do_init_maxima<R>(c);
+        if (rc) return rc;
+    }
+    for (int s = 0; s < nsteps; ++s) {
+        const int par = c->cur;
+        {
+            KTimer kt(c, 0);
+            HIPCHK(Launch<R>::stage1(c->stream, c->dprob, c->tiling, cp<R>(c->u[par]), mp<R>(c->uh),
+                                     c->st, par));
+        }
+        {
+            KTimer kt(c, 1);
+            HIPCHK(Launch<R>::stage2(c->stream, c->dprob, c->tiling, cp<R>(c->u[par]),
+                                     cp<R>(c->uh), mp<R>(c->u[par ^ 1]), c->st, par));
+        }
+        c->cur ^= 1;
+    }
+    return TLQ_OK;
+}
+
+template <class R> int do_upload(tlq_ctx *c, void *dst, const double *host) {
+    const Grid &g = c->g;
+    const long long plane = (long long)(g.nx + 4) * (g.ny + 4);
+    const int nzt = g.nz + 4;
+    int chunk = (int)std::max<long long>(1, (64ll << 20) / (plane * 5 * 8));
+    if (chunk > nzt) chunk = nzt;
+    double *stage = nullptr;
+    HIPCHK(hipMalloc(&stage, (size_t)plane * chunk * 5 * 8));
+    for (int k0 = 0; k0 < nzt; k0 += chunk) {
+        const int nk = std::min(chunk, nzt - k0);
+        HIPCHK(hipMemcpyAsync(stage, host + (size_t)5 * plane * k0, (size_t)5 * plane * nk * 8,
+                              hipMemcpyHostToDevice, c->stream));
+        HIPCHK(Launch<R>::aos_to_soa(c->stream, c->dprob, stage, mp<R>(dst), k0 - 1, nk,
+                                     plane * nk));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(stage));
+    return TLQ_OK;
+}
+
+template <class R> int do_download(tlq_ctx *c, const void *src, double *host) {
+    const Grid &g = c->g;
+    const long long plane = (long long)(g.nx + 4) * (g.ny + 4);
+    const int nzt = g.nz + 4;
+    int chunk = (int)std::max<long long>(1, (64ll << 20) / (plane * 5 * 8));
+    if (chunk > nzt) chunk = nzt;
+    double *stage = nullptr;
+    HIPCHK(hipMalloc(&stage, (size_t)plane * chunk * 5 * 8));
+    for (int k0 = 0; k0 < nzt; k0 += chunk) {
+        const int nk = std::min(chunk, nzt - k0);
+        HIPCHK(Launch<R>::soa_to_aos(c->stream, c->dprob, cp<R>(const_cast<void *>(src)), stage,
+                                     k0 - 1, nk, plane * nk));
+        HIPCHK(hipMemcpyAsync(host + (size_t)5 * plane * k0, stage, (size_t)5 * plane * nk * 8,
+                              hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    HIPCHK(hipFree(stage));
+    return TLQ_OK;
+}
+
+#define DISPATCH(ctx, fn, ...) \
+    ((ctx)->prec == TLQ_FP32 ? fn<float>(ctx, ##__VA_ARGS__) : fn<double>(ctx, ##__VA_ARGS__))
+
+template <class R> int do_calc_prims(tlq_ctx *c, int src) {
+    int rc = ensure_buf(c, TLQ_BUF_UPRIM);
+    if (rc) return rc;
+    HIPCHK(Launch<R>::calc_prims(c->stream, c->dprob, cp<R>(buf_ref(c, src)),
+                                 mp<R>(c->bufs[TLQ_BUF_UPRIM]), ncell_box(c->g)));
+    return TLQ_OK;
+}
+
+template <class R> int do_time_step(tlq_ctx *c, double *dt) {
+    int rc = ensure_buf(c, TLQ_BUF_UPRIM);
+    if (rc) return rc;
+    HIPCHK(hipMemsetAsync(c->scratch_max, 0, 4 * sizeof(unsigned long long), c->stream));
+    HIPCHK(Launch<R>::maxima(c->stream, c->dprob, cp<R>(c->bufs[TLQ_BUF_UPRIM]), 1, c->scratch_max,
+                             ninterior(c->g)));
+    unsigned long long m[4];
+    HIPCHK(hipMemcpyAsync(m, c->scratch_max, sizeof m, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double a[3];
+    for (int q = 0; q < 3; ++q) std::memcpy(&a[q], &m[q], 8);
+    const Problem &P = c->host;
+    double mm = P.dx / a[0];
+    if (P.dy / a[1] < mm) mm = P.dy / a[1];
+    if (P.dz / a[2] < mm) mm = P.dz / a[2];
+    *dt = P.cfl * mm / std::sqrt(3.0);
+    return TLQ_OK;
+}
+
+template <class R> int do_fluxes(tlq_ctx *c, int src, int choice) {
+    for (int b : {TLQ_BUF_UPRIM, TLQ_BUF_F, TLQ_BUF_G, TLQ_BUF_H}) {
+        int rc = ensure_buf(c, b);
+        if (rc) return rc;
+    }
+    const Grid &g = c->g;
+    HIPCHK(Launch<R>::fluxes(c->stream, c->dprob, cp<R>(buf_ref(c, src)),
+                             cp<R>(c->bufs[TLQ_BUF_UPRIM]), mp<R>(c->bufs[TLQ_BUF_F]),
+                             mp<R>(c->bufs[TLQ_BUF_G]), mp<R>(c->bufs[TLQ_BUF_H]), choice,
+                             (long long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1)));
+    return TLQ_OK;
+}
+
+template <class R> int do_godunov(tlq_ctx *c, int dst, double dt) {
+    for (int b : {dst, TLQ_BUF_F, TLQ_BUF_G, TLQ_BUF_H}) {
+        int rc = ensure_buf(c, b);
+        if (rc) return rc;
+    }
+    const R *S = nullptr;
+    if (c->host.nmasses > 0) {
+        int rc = ensure_buf(c, TLQ_BUF_S);
+        if (rc) return rc;
+        if ((rc = ensure_buf(c, TLQ_BUF_UPRIM))) return rc;
+        HIPCHK(Launch<R>::gravity(c->stream, c->dprob, cp<R>(c->bufs[TLQ_BUF_UPRIM]),
+                                  mp<R>(c->bufs[TLQ_BUF_S]), ninterior(c->g)));
+        S = cp<R>(c->bufs[TLQ_BUF_S]);
+    }
+    HIPCHK(Launch<R>::godunov(c->stream, c->dprob, mp<R>(buf_ref(c, dst)), cp<R>(c->u[c->cur]),
+                              cp<R>(c->bufs[TLQ_BUF_F]), cp<R>(c->bufs[TLQ_BUF_G]),
+                              cp<R>(c->bufs[TLQ_BUF_H]), S, dt, ninterior(c->g)));
+    return TLQ_OK;
+}
+
+template <class R> int do_step(tlq_ctx *c) {
+    for (int b : {TLQ_BUF_UP, TLQ_BUF_UPP}) {
+        int rc = ensure_buf(c, b);
+        if (rc) return rc;
+    }
+    void *U = c->u[c->cur];
+    const long long n = 5 * c->g.nc;
+    HIPCHK(Launch<R>::copy(c->stream, mp<R>(c->bufs[TLQ_BUF_UPP]), cp<R>(U), n));
+    HIPCHK(Launch<R>::step(c->stream, c->dprob, mp<R>(U), cp<R>(c->bufs[TLQ_BUF_UP]),
+                           cp<R>(c->bufs[TLQ_BUF_UPP]), ncell_box(c->g)));
+    return TLQ_OK;
+}
+
+template <class R> int do_cooling(tlq_ctx *c, double dt) {
+    int rc = ensure_buf(c, TLQ_BUF_UPRIM);
+    if (rc) return rc;
+    HIPCHK(Launch<R>::cooling(c->stream, c->dprob, mp<R>(c->bufs[TLQ_BUF_UPRIM]),
+                              mp<R>(c->u[c->cur]), dt, ninterior(c->g)));
+    return TLQ_OK;
+}
+
+template <class R> int do_check(tlq_ctx *c, long long *bad) {
+    HIPCHK(hipMemsetAsync(c->scratch_cnt, 0, sizeof(unsigned long long), c->stream));
+    HIPCHK(Launch<R>::check(c->stream, c->dprob, cp<R>(c->u[c->cur]), c->scratch_cnt,
+                            ninterior(c->g)));
+    unsigned long long v = 0;
+    HIPCHK(hipMemcpyAsync(&v, c->scratch_cnt, sizeof v, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *bad = (long long)v;
+    return TLQ_OK;
+}
+
+int check_ctx(tlq_ctx *c) { return c ? TLQ_OK : fail(TLQ_ERR_ARG, "null context"); }
+int check_buf(int b) {
+    return (b >= TLQ_BUF_U && b <= TLQ_BUF_S) ? TLQ_OK : fail(TLQ_ERR_ARG, "bad buffer id");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *tlq_last_error(void) { return g_err.c_str(); }
+
+int tlq_create(const tlq_params *p, tlq_ctx **out) {
+    if (!p || !out) return fail(TLQ_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (p->nx < 2 || p->ny < 2 || p->nz < 2) return fail(TLQ_ERR_ARG, "nx, ny, nz must be >= 2");
+    for (int q = 0; q < 6; ++q) {
+        if (p->bc[q] == TLQ_BC_INFLOW)
+            return fail(TLQ_ERR_UNSUPPORTED,
+                        "BC_INFLOW is an unimplemented stub in the reference (boundaries.f90:191)");
+        if (!valid_bc(p->bc[q])) return fail(TLQ_ERR_ARG, "bad boundary id");
+    }
+    if (p->limiter < TLQ_LIM_NONE || p->limiter > TLQ_LIM_SUPERBEE)
+        return fail(TLQ_ERR_ARG, "bad limiter id (the reference stops: HLLC.f90:630-632)");
+    if (p->solver != TLQ_SOLVER_HLLC && p->solver != TLQ_SOLVER_HLL)
+        return fail(TLQ_ERR_ARG, "bad solver id");
+    if (p->precision != TLQ_FP64 && p->precision != TLQ_FP32)
+        return fail(TLQ_ERR_ARG, "bad precision id");
+    if (p->cooling < TLQ_COOL_OFF || p->cooling > TLQ_COOL_SURROGATE)
+        return fail(TLQ_ERR_ARG, "bad cooling id");
+    if (!(p->gamma > 1.0) || !(p->cfl > 0.0) || !(p->xr > p->xl) || !(p->yr > p->yl) ||
+        !(p->zr > p->zl) || !(p->mu0 > 0.0))
+        return fail(TLQ_ERR_ARG, "bad physical parameters");
+
+    tlq_ctx *c = new tlq_ctx();
+    c->prm = *p;
+    c->prec = p->precision;
+    c->esize = p->precision == TLQ_FP32 ? 4 : 8;
+    Grid &g = c->g;
+    g.nx = p->nx;
+    g.ny = p->ny;
+    g.nz = p->nz;
+    // i = 1 at offset 32 of every row; rows padded to a multiple of 32.
+    const int ox = 31;  // offset of i = 0
+    g.sx = ((ox + p->nx + 3 + 31) / 32) * 32;
+    g.sxy = (long long)g.sx * (p->ny + 4);
+    g.nc = g.sxy * (p->nz + 4);
+    g.base = ox + (long long)g.sx * 1 + g.sxy * 1;  // cell (0,0,0): j = k = -1 at row/plane 0
+    Problem &P = c->host;
+    std::memset(&P, 0, sizeof P);
+    P.g = g;
+    P.xl = p->xl;
+    P.yl = p->yl;
+    P.zl = p->zl;
+    P.dx = (p->xr - p->xl) / p->nx;
+    P.dy = (p->yr - p->yl) / p->ny;
+    P.dz = (p->zr - p->zl) / p->nz;
+    P.gamma = p->gamma;
+    P.mu0 = p->mu0;
+    P.cfl = p->cfl;
+    P.eta = p->eta;
+    P.cooling_limit = p->cooling_limit;
+    P.softening = p->softening;
+    P.limiter = p->limiter;
+    P.solver = p->solver;
+    for (int q = 0; q < 6; ++q) P.bc[q] = p->bc[q];
+    P.cooling = p->cooling;
+
+    auto bail = [&](int code, const char *m) {
+        tlq_destroy(c);
+        return fail(code, m);
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(TLQ_ERR_HIP, "hipStreamCreate failed");
+    if (hipMalloc(&c->dprob, sizeof(Problem)) != hipSuccess ||
+        hipMalloc(&c->st, sizeof(StepState)) != hipSuccess ||
+        hipMalloc(&c->scratch_max, 8 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&c->scratch_cnt, sizeof(unsigned long long)) != hipSuccess)
+        return bail(TLQ_ERR_NOMEM, "hipMalloc failed");
+    const size_t bytes = (size_t)5 * g.nc * c->esize;
+    for (int q = 0; q < 2; ++q) {
+        if (hipMalloc(&c->u[q], bytes) != hipSuccess) return bail(TLQ_ERR_NOMEM, "hipMalloc failed (U)");
+        hipMemsetAsync(c->u[q], 0, bytes, c->stream);
+    }
+    if (hipMalloc(&c->uh, bytes) != hipSuccess) return bail(TLQ_ERR_NOMEM, "hipMalloc failed (U1/2)");
+    hipMemsetAsync(c->uh, 0, bytes, c->stream);
+    c->bufs[TLQ_BUF_UPP] = c->uh;
+    StepState s0{};
+    hipMemcpyAsync(c->st, &s0, sizeof s0, hipMemcpyHostToDevice, c->stream);
+    hipEventCreate(&c->ev0);
+    hipEventCreate(&c->ev1);
+
+    int tx, ty, zc;
+    if (c->prec == TLQ_FP32) {
+        tx = TileShape<float>::TX;
+        ty = TileShape<float>::TY;
+        zc = TileShape<float>::ZC;
+    } else {
+        tx = TileShape<double>::TX;
+        ty = TileShape<double>::TY;
+        zc = TileShape<double>::ZC;
+    }
+    Tiling &T = c->tiling;
+    T.nbx = (p->nx + tx - 1) / tx;
+    T.nby = (p->ny + ty - 1) / ty;
+    T.nbz = (p->nz + zc - 1) / zc;
+    T.ntiles = T.nbx * T.nby * T.nbz;
+    T.per_xcd = (T.ntiles + 7) / 8;
+    int rc = push_problem(c);
+    if (rc) {
+        tlq_destroy(c);
+        return rc;
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(TLQ_ERR_HIP, "init sync failed");
+    *out = c;
+    return TLQ_OK;
+}
+
+int tlq_destroy(tlq_ctx *c) {
+    if (!c) return TLQ_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto &p : c->pending) {
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    for (int q = 0; q < 8; ++q)
+        if (c->bufs[q] && c->bufs[q] != c->uh) hipFree(c->bufs[q]);
+    for (int q = 0; q < 2; ++q)
+        if (c->u[q]) hipFree(c->u[q]);
+    if (c->uh) hipFree(c->uh);
+    if (c->dprob) hipFree(c->dprob);
+    if (c->st) hipFree(c->st);
+    if (c->scratch_max) hipFree(c->scratch_max);
+    if (c->scratch_cnt) hipFree(c->scratch_cnt);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return TLQ_OK;
+}
+
+int tlq_set_winds(tlq_ctx *c, const tlq_wind *w, int n) {
+    if (int rc = check_ctx(c)) return rc;
+    if (n < 0 || n > MAX_WINDS || (n > 0 && !w)) return fail(TLQ_ERR_ARG, "bad wind list");
+    for (int q = 0; q < n; ++q) {
+        const tlq_wind &s = w[q];
+        c->host.winds[q] = WindDev{s.xc, s.yc, s.zc, s.vx, s.vy, s.vz, s.radius, s.mdot, s.vinf,
+                                   s.temp, s.mu};
+    }
+    c->host.nwinds = n;
+    return push_problem(c);
+}
+
+int tlq_set_point_masses(tlq_ctx *c, const double *xyzm, int n) {
+    if (int rc = check_ctx(c)) return rc;
+    if (n < 0 || n > MAX_MASSES || (n > 0 && !xyzm)) return fail(TLQ_ERR_ARG, "bad mass list");
+    for (int q = 0; q < 4 * n; ++q) c->host.masses[q] = xyzm[q];
+    c->host.nmasses = n;
+    return push_problem(c);
+}
+
+int tlq_set_cooling_table(tlq_ctx *c, const double *T, const double *L, int n) {
+    if (int rc = check_ctx(c)) return rc;
+    if (n < 2 || n > MAX_TABLE || !T || !L) return fail(TLQ_ERR_ARG, "bad cooling table");
+    for (int q = 0; q < n; ++q) {
+        if (!(T[q] > 0.0) || (q > 0 && !(T[q] > T[q - 1])))
+            return fail(TLQ_ERR_ARG, "cooling table temperatures must increase");
+        c->host.tab_T[q] = T[q];
+        c->host.tab_logT[q] = std::log10(T[q]);
+        c->host.tab_L[q] = L[q];
+    }
+    c->host.ntable = n;
+    c->host.tmin = T[0];
+    return push_problem(c);
+}
+
+int tlq_upload(tlq_ctx *c, int buf, const double *host) {
+    if (int rc = check_ctx(c)) return rc;
+    if (int rc = check_buf(buf)) return rc;
+    if (!host) return fail(TLQ_ERR_ARG, "null host array");
+    if (int rc = ensure_buf(c, buf)) return rc;
+    if (buf == TLQ_BUF_U) c->fused_ready = false;
+    return DISPATCH(c, do_upload, buf_ref(c, buf), host);
+}
+
+int tlq_download(tlq_ctx *c, int buf, double *host) {
+    if (int rc = check_ctx(c)) return rc;
+    if (int rc = check_buf(buf)) return rc;
+    if (!host) return fail(TLQ_ERR_ARG, "null host array");
+    if (!buf_ref(c, buf)) return fail(TLQ_ERR_STATE, "buffer not allocated yet");
+    return DISPATCH(c, do_download, buf_ref(c, buf), host);
+}
+
+int tlq_calc_prims(tlq_ctx *c, int src) {
+    if (int rc = check_ctx(c)) return rc;
+    if (src != TLQ_BUF_U && src != TLQ_BUF_UP && src != TLQ_BUF_UPP)
+        return fail(TLQ_ERR_ARG, "calc_prims source must be U, UP or UPP");
+    if (!buf_ref(c, src)) return fail(TLQ_ERR_STATE, "source array not set");
+    return DISPATCH(c, do_calc_prims, src);
+}
+
+int tlq_time_step(tlq_ctx *c, double *dt) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!dt) return fail(TLQ_ERR_ARG, "null dt");
+    return DISPATCH(c, do_time_step, dt);
+}
+
+int tlq_fluxes(tlq_ctx *c, int src, int choice) {
+    if (int rc = check_ctx(c)) return rc;
+    if (choice != 1 && choice != 2) return fail(TLQ_ERR_ARG, "choice must be 1 or 2");
+    if (src != TLQ_BUF_U && src != TLQ_BUF_UPP) return fail(TLQ_ERR_ARG, "flux source must be U or UPP");
+    return DISPATCH(c, do_fluxes, src, choice);
+}
+
+int tlq_godunov(tlq_ctx *c, int dst, double dt) {
+    if (int rc = check_ctx(c)) return rc;
+    if (dst != TLQ_BUF_UP && dst != TLQ_BUF_UPP) return fail(TLQ_ERR_ARG, "Godunov target must be UP or UPP");
+    return DISPATCH(c, do_godunov, dst, dt);
+}
+
+int tlq_borders(tlq_ctx *c, int buf, int order) {
+    if (int rc = check_ctx(c)) return rc;
+    if (int rc = check_buf(buf)) return rc;
+    if (order != 1 && order != 2) return fail(TLQ_ERR_ARG, "order must be 1 or 2");
+    if (int rc = ensure_buf(c, buf)) return rc;
+    if (buf == TLQ_BUF_U) c->fused_ready = false;
+    return DISPATCH(c, do_borders, buf_ref(c, buf), order);
+}
+
+int tlq_step(tlq_ctx *c, double *t, int *it, double dt) {
+    if (int rc = check_ctx(c)) return rc;
+    int rc = DISPATCH(c, do_step);
+    if (rc) return rc;
+    c->fused_ready = false;
+    if (t) *t = *t + dt;  // HydroCore.f90:208-209
+    if (it) *it = *it + 1;
+    return TLQ_OK;
+}
+
+int tlq_cooling(tlq_ctx *c, double dt) {
+    if (int rc = check_ctx(c)) return rc;
+    if (c->host.ntable < 2) return fail(TLQ_ERR_STATE, "cooling table not set");
+    c->fused_ready = false;
+    return DISPATCH(c, do_cooling, dt);
+}
+
+int tlq_init_state(tlq_ctx *c) {
+    if (int rc = check_ctx(c)) return rc;
+    int rc = DISPATCH(c, do_borders, c->u[c->cur], 1);  // mainHD.f90:38
+    if (rc) return rc;
+    // t = 0, it = 0 (mainHD.f90:22-27)
+    StepState s0{};
+    HIPCHK(hipMemcpyAsync(c->st, &s0, sizeof s0, hipMemcpyHostToDevice, c->stream));
+    return DISPATCH(c, do_init_maxima);                   // calc_prims + first time_step
+}
+
+int tlq_advance_n(tlq_ctx *c, int n) {
+    if (int rc = check_ctx(c)) return rc;
+    if (n < 0) return fail(TLQ_ERR_ARG, "negative step count");
+    if (c->host.cooling != TLQ_COOL_OFF && c->host.ntable < 2)
+        return fail(TLQ_ERR_STATE, "cooling enabled but no table set");
+    return DISPATCH(c, do_advance, n);
+}
+
+int tlq_get_time(tlq_ctx *c, double *dt, double *t, int *it) {
+    if (int rc = check_ctx(c)) return rc;
+    StepState s;
+    HIPCHK(hipMemcpyAsync(&s, c->st, sizeof s, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (dt) *dt = s.dt;
+    if (t) *t = s.time[c->cur];
+    if (it) *it = (int)s.it;
+    return TLQ_OK;
+}
+
+int tlq_advance(tlq_ctx *c, double *dt, double *t, int *it) {
+    int rc = tlq_advance_n(c, 1);
+    if (rc) return rc;
+    if (dt || t || it) return tlq_get_time(c, dt, t, it);
+    return TLQ_OK;
+}
+
+int tlq_sync(tlq_ctx *c) {
+    if (int rc = check_ctx(c)) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return TLQ_OK;
+}
+
+int tlq_check_state(tlq_ctx *c, long long *bad) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!bad) return fail(TLQ_ERR_ARG, "null output");
+    return DISPATCH(c, do_check, bad);
+}
+
+int tlq_fallback_fraction(tlq_ctx *c, double *frac) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!frac) return fail(TLQ_ERR_ARG, "null output");
+    *frac = c->fallback;
+    return TLQ_OK;
+}
+
+int tlq_timer_start(tlq_ctx *c) {
+    if (int rc = check_ctx(c)) return rc;
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    return TLQ_OK;
+}
+
+int tlq_timer_stop(tlq_ctx *c, double *ms) {
+    if (int rc = check_ctx(c)) return rc;
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, c->ev0, c->ev1));
+    if (ms) *ms = f;
+    return TLQ_OK;
+}
+
+int tlq_kernel_timing(tlq_ctx *c, int enable) {
+    if (int rc = check_ctx(c)) return rc;
+    resolve_timings(c);
+    c->ktiming = enable != 0;
+    for (int q = 0; q < kNumKernels; ++q) {
+        c->ktotal[q] = 0.0;
+        c->kcount[q] = 0;
+    }
+    return TLQ_OK;
+}
+
+int tlq_kernel_stats(tlq_ctx *c, int index, char *name, int name_len, double *total_ms,
+                     long long *launches) {
+    if (int rc = check_ctx(c)) return rc;
+    if (index < 0 || index >= kNumKernels) return fail(TLQ_ERR_ARG, "bad kernel index");
+    resolve_timings(c);
+    if (name && name_len > 0) {
+        std::strncpy(name, kKernelNames[index], name_len - 1);
+        name[name_len - 1] = '\0';
+    }
+    if (total_ms) *total_ms = c->ktotal[index];
+    if (launches) *launches = c->kcount[index];
+    return TLQ_OK;
+}
+
+int tlq_layout(tlq_ctx *c, long long *cells, int *pitch_x, int *origin_x) {
+    if (int rc = check_ctx(c)) return rc;
+    if (cells) *cells = c->g.nc;
+    if (pitch_x) *pitch_x = c->g.sx;
+    if (origin_x) *origin_x = (int)(c->g.base - c->g.sx - c->g.sxy - 1);
+    return TLQ_OK;
+}
+
+}  // extern "C"

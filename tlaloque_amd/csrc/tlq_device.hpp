@@ -1,0 +1,354 @@
+// tlq_device.hpp -- device math of the HydroCore step for gfx950.
+//
+// Every routine is templated on the sweep type R: double (FP64 mode) or float
+// (FP32 mode).  In FP32 mode the pressure recovery E - |m|^2/(2 rho), the
+// conserved update, viscosity accumulation, cooling, winds and gravity run in
+// double; only the reconstruction and Riemann fluxes run in float.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tlq {
+
+// source/constants.f90:6-15 (cgs).
+constexpr double AMU = 1.660538782e-24;
+constexpr double KB = 1.380650400e-16;
+constexpr double GR = 6.67430e-8;
+constexpr double PI_REF = 3.14159265358979;
+
+enum { BC_OUTFLOW = 1, BC_REFLECTIVE = 2, BC_PERIODIC = 3, BC_INFLOW = 4 };
+enum { LIM_NONE = 1, LIM_VANLEER = 2, LIM_MINMOD = 3, LIM_UMIST = 4, LIM_WOODWARD = 5,
+       LIM_SUPERBEE = 6 };
+enum { SOLVER_HLLC = 0, SOLVER_HLL = 1 };
+
+// Structure-of-arrays geometry.  Cell (i, j, k), i in -1..nx+2, sits at
+// base + i + sx*j + sxy*k inside each field; field e starts at e*nc.  The
+// x pitch sx and the base are chosen so that i = 1 of every row is aligned
+// to 32 elements (128 B in FP32).
+struct Grid {
+    int nx, ny, nz;
+    int sx;          // elements per x row (padded)
+    long long sxy;   // elements per z plane
+    long long nc;    // elements per field
+    long long base;  // offset of cell (0, 0, 0)
+    __host__ __device__ long long idx(int i, int j, int k) const {
+        return base + (long long)i + (long long)sx * j + sxy * k;
+    }
+};
+
+struct WindDev {  // spherical_wind_type, source/winds.f90:5-13
+    double xc, yc, zc, vx, vy, vz, radius, mdot, vinf, temp, mu;
+};
+
+constexpr int MAX_WINDS = 64;
+constexpr int MAX_MASSES = 64;
+constexpr int MAX_TABLE = 1024;
+
+// Problem description shared by all kernels (lives in device memory).
+struct Problem {
+    Grid g;
+    double xl, yl, zl, dx, dy, dz;
+    double gamma, mu0, cfl, eta, cooling_limit, softening;
+    int limiter, solver;
+    int bc[6];  // x-, x+, y-, y+, z-, z+
+    int nwinds, nmasses, ntable;
+    int cooling;  // 0 off, 1 exact, 2 surrogate
+    double tmin;  // first table temperature (cooling.f90:42)
+    WindDev winds[MAX_WINDS];
+    double masses[4 * MAX_MASSES];
+    double tab_logT[MAX_TABLE];  // log10 T_i
+    double tab_T[MAX_TABLE];     // T_i
+    double tab_L[MAX_TABLE];     // log10 Lambda_i
+};
+
+// ---------------------------------------------------------------------------
+// Arithmetic helpers.  FP64 keeps IEEE division and square root (the oracle
+// comparison); FP32 uses the hardware reciprocal and square root (1 ulp).
+__device__ __forceinline__ double rdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float rdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double rsqrt_(double a) { return sqrt(a); }
+__device__ __forceinline__ float rsqrt_(float a) { return __builtin_amdgcn_sqrtf(a); }
+
+template <class R> __device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
+template <class R> __device__ __forceinline__ R rmax(R a, R b) { return a > b ? a : b; }
+
+// Cell-centre coordinates, source/HydroCore.f90:35-75 (single rank).
+__device__ __forceinline__ double xcoord(const Problem &P, int i) {
+    return (P.xl + P.dx * (double)i) - P.dx / 2.0;
+}
+__device__ __forceinline__ double ycoord(const Problem &P, int j) {
+    return (P.yl + P.dy * (double)j) - P.dy / 2.0;
+}
+__device__ __forceinline__ double zcoord(const Problem &P, int k) {
+    return (P.zl + P.dz * (double)k) - P.dz / 2.0;
+}
+
+// Pressure recovery of calc_prims (HydroCore.f90:18-21), always in FP64:
+// p = (gamma-1) (E - |m|^2/(2 rho)), floored at 1e-20.
+__device__ __forceinline__ double pressure64(double gamma, double r, double mx, double my,
+                                             double mz, double e) {
+    double p = (gamma - 1.0) * (e - 0.5 * (mx * mx + my * my + mz * mz) / r);
+    return p > 1e-20 ? p : 1e-20;
+}
+
+// calc_prims for one cell (HydroCore.f90:14-21): w = (rho, u, v, w, p).
+template <class R>
+__device__ __forceinline__ void cons2prim(double gamma, const R u[5], R w[5]) {
+    w[0] = u[0];
+    const R inv = rdiv(R(1), u[0]);
+    if constexpr (sizeof(R) == 8) {
+        w[1] = u[1] / u[0];
+        w[2] = u[2] / u[0];
+        w[3] = u[3] / u[0];
+    } else {
+        w[1] = u[1] * inv;
+        w[2] = u[2] * inv;
+        w[3] = u[3] * inv;
+    }
+    w[4] = (R)pressure64(gamma, (double)u[0], (double)u[1], (double)u[2], (double)u[3],
+                         (double)u[4]);
+}
+
+// prim2f, HydroCore.f90:213-251, for direction D (0 x, 1 y, 2 z).
+template <int D, class R>
+__device__ __forceinline__ void prim2f(R gamma_ratio, const R w[5], R f[5]) {
+    constexpr int n = D + 1;
+    const R vn = w[n];
+    f[0] = w[0] * vn;
+#pragma unroll
+    for (int c = 1; c <= 3; ++c) {
+        const int a = c < n ? c : n, b = c < n ? n : c;
+        f[c] = w[0] * w[a] * w[b];
+        if (c == n) f[c] = f[c] + w[4];
+    }
+    f[4] = vn * (R(0.5) * w[0] * (w[1] * w[1] + w[2] * w[2] + w[3] * w[3]) + gamma_ratio * w[4]);
+}
+
+// average(), HLLC.f90:586-636: the slope of a cell from its two one-sided
+// differences a (left) and b (right).
+template <class R> __device__ __forceinline__ R lim_avg(int opt, R a, R b) {
+    R s, c, d, r;
+    switch (opt) {
+    case LIM_NONE:
+        return R(0.5) * (a + b);
+    case LIM_VANLEER:
+        if constexpr (sizeof(R) == 8) {
+            if (a * b <= 0.0) return 0.0;
+            return a * b * (a + b) / (a * a + b * b);
+        } else {
+            // a*b (a+b)/(a^2+b^2) written with the ratio t = a/b so that
+            // differences of cgs densities (~1e-25) do not underflow in FP32.
+            if (!((a > 0.f && b > 0.f) || (a < 0.f && b < 0.f))) return 0.f;
+            const R t = rdiv(a, b);
+            return rdiv((a + b) * t, __builtin_fmaf(t, t, 1.f));
+        }
+    case LIM_MINMOD:
+        s = __builtin_copysign(R(1), a);
+        r = rmin(R(__builtin_fabs(a)), s * b);
+        return s * rmax(r, R(0));
+    case LIM_UMIST:
+        s = __builtin_copysign(R(1), a);
+        c = R(0.25) * a + R(0.75) * b;
+        d = R(0.75) * a + R(0.25) * b;
+        r = R(2) * R(__builtin_fabs(a));
+        r = rmin(r, R(2) * s * b);
+        r = rmin(r, s * c);
+        r = rmin(r, s * d);
+        return s * rmax(r, R(0));
+    case LIM_WOODWARD:
+        s = __builtin_copysign(R(1), a);
+        c = R(0.5) * (a + b);
+        r = R(2) * R(__builtin_fabs(a));
+        r = rmin(r, R(2) * s * b);
+        r = rmin(r, s * c);
+        return s * rmax(r, R(0));
+    case LIM_SUPERBEE:
+        s = __builtin_copysign(R(1), b);
+        c = rmin(R(2) * R(__builtin_fabs(b)), s * a);
+        d = rmin(R(__builtin_fabs(b)), R(2) * s * a);
+        return s * rmax(rmax(R(0), c), d);
+    default:
+        return R(0);
+    }
+}
+
+// wavespeed, HLLC.f90:498-518 (Davis estimates and the contact speed).
+template <int D, class R>
+__device__ __forceinline__ void wavespeed(R gamma, const R wl[5], const R wr[5], R &sl, R &sr,
+                                          R &sst) {
+    constexpr int n = D + 1;
+    const R csl = rsqrt_(rdiv(gamma * wl[4], wl[0]));
+    const R csr = rsqrt_(rdiv(gamma * wr[4], wr[0]));
+    sl = rmin(wl[n] - csl, wr[n] - csr);
+    sr = rmax(wl[n] + csl, wr[n] + csr);
+    sst = rdiv(wr[4] - wl[4] + wl[0] * wl[n] * (sl - wl[n]) - wr[0] * wr[n] * (sr - wr[n]),
+               wl[0] * (sl - wl[n]) - wr[0] * (sr - wr[n]));
+}
+
+// One HLLC face (HLLC.f90:99-135, 275-314): wl/wr reconstructed primitive
+// states, ul/ur the cell-centred conserved states of the two cells.
+template <int D, class R>
+__device__ __forceinline__ void hllc_face(R gamma, R gratio, const R wl[5], const R wr[5],
+                                          const R ul[5], const R ur[5], R f[5]) {
+    constexpr int n = D + 1;
+    R sl, sr, sst;
+    wavespeed<D>(gamma, wl, wr, sl, sr, sst);
+    if (sl >= R(0)) {
+        prim2f<D>(gratio, wl, f);
+    } else if (sr <= R(0)) {
+        prim2f<D>(gratio, wr, f);
+    } else {
+        // Star region: the side is chosen by the sign of the contact speed.
+        const bool left = sst >= R(0);
+        const R *w = left ? wl : wr;
+        const R *u = left ? ul : ur;
+        const R s = left ? sl : sr;
+        R fs[5];
+        if (left) prim2f<D>(gratio, wl, fs);
+        else prim2f<D>(gratio, wr, fs);
+        const R sw = s - w[n];
+        const R rs = rdiv(w[0] * sw, s - sst);
+        R us[5];
+        us[0] = rs;
+#pragma unroll
+        for (int c = 1; c <= 3; ++c) us[c] = c == n ? rs * sst : rs * w[c];
+        us[4] = rs * (rdiv(u[4], w[0]) + (sst - w[n]) * (sst + rdiv(w[4], w[0] * sw)));
+#pragma unroll
+        for (int e = 0; e < 5; ++e) f[e] = fs[e] + s * (us[e] - u[e]);
+    }
+}
+
+// One HLL face (HLL.f90:106-128, 197-223).
+template <int D, class R>
+__device__ __forceinline__ void hll_face(R gamma, R gratio, const R wl[5], const R wr[5],
+                                         const R ul[5], const R ur[5], R f[5]) {
+    R sl, sr, sst;
+    wavespeed<D>(gamma, wl, wr, sl, sr, sst);
+    if (sl >= R(0)) {
+        prim2f<D>(gratio, wl, f);
+    } else if (sr <= R(0)) {
+        prim2f<D>(gratio, wr, f);
+    } else {
+        R fl[5], fr[5];
+        prim2f<D>(gratio, wl, fl);
+        prim2f<D>(gratio, wr, fr);
+        const R inv = rdiv(R(1), sr - sl);
+#pragma unroll
+        for (int e = 0; e < 5; ++e) {
+            const R num = sr * fl[e] - sl * fr[e] + sl * sr * (ur[e] - ul[e]);
+            if constexpr (sizeof(R) == 8) f[e] = num / (sr - sl);
+            else f[e] = num * inv;
+        }
+    }
+}
+
+template <int D, class R>
+__device__ __forceinline__ void riemann(int solver, R gamma, R gratio, const R wl[5],
+                                        const R wr[5], const R ul[5], const R ur[5], R f[5]) {
+    if (solver == SOLVER_HLL) hll_face<D>(gamma, gratio, wl, wr, ul, ur, f);
+    else hllc_face<D>(gamma, gratio, wl, wr, ul, ur, f);
+}
+
+// Spherical wind state at a point (winds.f90:69-83); returns false outside
+// the wind's radius.  Computed in FP64.
+__device__ __forceinline__ bool wind_state(const Problem &P, const WindDev &wd, double x, double y,
+                                           double z, double out[5]) {
+    const double dxw = x - wd.xc, dyw = y - wd.yc, dzw = z - wd.zc;
+    const double dist = sqrt(dxw * dxw + dyw * dyw + dzw * dzw);
+    if (!(dist < wd.radius)) return false;
+    const double dens = wd.mdot / (wd.vinf * dist * dist * 4.0 * PI_REF);
+    const double vx = wd.vinf * dxw / dist + wd.vx;
+    const double vy = wd.vinf * dyw / dist + wd.vy;
+    const double vz = wd.vinf * dzw / dist + wd.vz;
+    const double pres = dens * KB * wd.temp / (wd.mu * AMU);
+    out[0] = dens;
+    out[1] = dens * vx;
+    out[2] = dens * vy;
+    out[3] = dens * vz;
+    out[4] = 0.5 * dens * (vx * vx + vy * vy + vz * vz) + pres / (P.gamma - 1.0);
+    return true;
+}
+
+// user_boundaries applies the winds in order (user.f90:64-68), so the last
+// wind that covers a cell sets it.  Returns true if any wind covers (i,j,k).
+__device__ __forceinline__ bool winds_at(const Problem &P, int i, int j, int k, double out[5]) {
+    if (P.nwinds == 0) return false;
+    const double x = xcoord(P, i), y = ycoord(P, j), z = zcoord(P, k);
+    for (int w = P.nwinds - 1; w >= 0; --w)
+        if (wind_state(P, P.winds[w], x, y, z, out)) return true;
+    return false;
+}
+
+// Point-mass potential at a cell centre (pointsource_gravity.f90:30-35 with
+// optional Plummer softening).
+__device__ __forceinline__ double potential(const Problem &P, int i, int j, int k) {
+    const double x = xcoord(P, i), y = ycoord(P, j), z = zcoord(P, k);
+    const double e2 = P.softening * P.softening;
+    double phi = 0.0;
+    for (int s = 0; s < P.nmasses; ++s) {
+        const double *m = P.masses + 4 * s;
+        const double r = sqrt((m[0] - x) * (m[0] - x) + (m[1] - y) * (m[1] - y) +
+                              (m[2] - z) * (m[2] - z) + e2);
+        phi = phi - GR * m[3] / r;
+    }
+    return phi;
+}
+
+// S_Solver (sources.f90:872-897) for one interior cell from its primitives.
+__device__ __forceinline__ void gravity_source(const Problem &P, int i, int j, int k,
+                                               double rho, double u, double v, double w,
+                                               double s[5]) {
+    const double g0 = (potential(P, i + 1, j, k) - potential(P, i - 1, j, k)) / (2.0 * P.dx);
+    const double g1 = (potential(P, i, j + 1, k) - potential(P, i, j - 1, k)) / (2.0 * P.dy);
+    const double g2 = (potential(P, i, j, k + 1) - potential(P, i, j, k - 1)) / (2.0 * P.dz);
+    s[0] = 0.0;
+    s[1] = -rho * g0;
+    s[2] = -rho * g1;
+    s[3] = -rho * g2;
+    s[4] = -(u * rho * g0) - v * rho * g1 - w * rho * g2;
+}
+
+// find_coef, cooling.f90:121-144: first row with T_i > T, then linear
+// interpolation of log10 Lambda in log10 T.  Rows are located by bisection
+// (same row as the reference's linear scan).  T >= T_max returns the last row.
+__device__ __forceinline__ double find_coef(const Problem &P, double temp) {
+    int lo = 0, hi = P.ntable - 1;  // invariant: T[lo] <= temp < T[hi] when found
+    if (!(P.tab_T[hi] > temp)) return pow(10.0, P.tab_L[hi]);
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (P.tab_T[mid] > temp) hi = mid;
+        else lo = mid;
+    }
+    // hi is the first index with T_hi > temp (T_lo <= temp, given temp >= T_0).
+    const double T0 = P.tab_logT[hi - 1], C0 = P.tab_L[hi - 1];
+    const double T1 = P.tab_logT[hi], C1 = P.tab_L[hi];
+    return pow(10.0, (C1 - C0) / (T1 - T0) * (log10(temp) - T0) + C0);
+}
+
+// apply_cooling for one cell (cooling.f90:67-112) on FP64 primitives.
+// Returns the cooled pressure; *cooled tells whether the T >= T_min gate passed.
+__device__ __forceinline__ double cool_exact(const Problem &P, double rho, double p, double dt,
+                                             bool *cooled) {
+    const double temp = p * P.mu0 * AMU / (rho * KB);
+    if (!(temp >= P.tmin)) {
+        *cooled = false;
+        return p;
+    }
+    *cooled = true;
+    const double eth = p / (P.gamma - 1.0);
+    const double aloss = find_coef(P, temp);
+    const double numdens = rho / (P.mu0 * AMU);
+    const double ce = aloss * (numdens * numdens) / eth;
+    double f = exp(-ce * dt);
+    if (f < 1.0 - P.cooling_limit) f = 1.0 - P.cooling_limit;
+    if (temp * f < 10.0) f = 10.0 / temp;
+    return p * f;
+}
+
+// Order-preserving map of a non-negative double onto uint64 for atomicMax.
+__device__ __forceinline__ unsigned long long ord64(double x) {
+    return (unsigned long long)__double_as_longlong(x);
+}
+
+}  // namespace tlq

@@ -1,0 +1,54 @@
+// tlq_launch.hpp -- kernel launchers, explicitly instantiated per precision in
+// tlq_fp64.hip (built without FMA contraction, for bitwise agreement with the
+// reference's arithmetic) and tlq_fp32.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "tlq_device.hpp"
+
+namespace tlq {
+
+struct StepState;
+struct Tiling;
+
+// Tile shapes of the fused kernels per precision.
+template <class R> struct TileShape;
+template <> struct TileShape<float> {
+    static constexpr int TX = 64, TY = 8, ZC = 64;
+};
+template <> struct TileShape<double> {
+    static constexpr int TX = 64, TY = 4, ZC = 64;
+};
+
+template <class R> struct Launch {
+    static hipError_t calc_prims(hipStream_t s, const Problem *P, const R *U, R *W, long long ncell);
+    static hipError_t maxima(hipStream_t s, const Problem *P, const R *A, int prims,
+                             unsigned long long *out, long long ninterior);
+    static hipError_t fluxes(hipStream_t s, const Problem *P, const R *U, const R *W, R *F, R *G,
+                             R *H, int choice, long long nfaces);
+    static hipError_t godunov(hipStream_t s, const Problem *P, R *out, const R *U, const R *F,
+                              const R *G, const R *H, const R *S, double dt, long long ninterior);
+    static hipError_t bc_plane(hipStream_t s, const Problem *P, R *U, int axis, int dst, int src,
+                               int negate, long long nplane);
+    static hipError_t winds(hipStream_t s, const Problem *P, R *U, int i0, int j0, int k0, int ni,
+                            int nj, int nk);
+    static hipError_t copy(hipStream_t s, R *dst, const R *src, long long n);
+    static hipError_t step(hipStream_t s, const Problem *P, R *U, const R *UP, const R *UPP,
+                           long long ncell);
+    static hipError_t cooling(hipStream_t s, const Problem *P, R *W, R *U, double dt,
+                              long long ninterior);
+    static hipError_t gravity(hipStream_t s, const Problem *P, const R *W, R *S, long long ninterior);
+    static hipError_t aos_to_soa(hipStream_t s, const Problem *P, const double *aos, R *soa, int k0,
+                                 int nk, long long ncell);
+    static hipError_t soa_to_aos(hipStream_t s, const Problem *P, const R *soa, double *aos, int k0,
+                                 int nk, long long ncell);
+    static hipError_t check(hipStream_t s, const Problem *P, const R *U, unsigned long long *bad,
+                            long long ninterior);
+    static hipError_t stage1(hipStream_t s, const Problem *P, const Tiling &T, const R *U, R *Uh,
+                             StepState *st, int parity);
+    static hipError_t stage2(hipStream_t s, const Problem *P, const Tiling &T, const R *U,
+                             const R *Uh, R *Unew, StepState *st, int parity);
+};
+
+}  // namespace tlq

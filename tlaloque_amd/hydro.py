@@ -1,0 +1,291 @@
+"""Python host mirror of the reference's HydroCore interface over the C-ABI.
+
+``HydroCore`` owns one ``tlq_ctx`` (include/tlq.h) and exposes the routines
+the reference's main loop calls (source/mainHD.f90:61-100) under their
+reference names:
+
+    calc_prims(src)          source/HydroCore.f90:1-33
+    time_step()              source/HydroCore.f90:113-155
+    fluxes(src, choice)      source/HLLC.f90:64-496 (HLL.f90:71-297 with solver=HLL)
+    godunov(dst, dt)         source/HLLC.f90:1-62
+    borders(buf, order)      source/boundaries.f90:1-26 (+ user_boundaries winds)
+    step(t, it, dt)          source/HydroCore.f90:170-211
+    cooling(dt)              source/cooling.f90:9-19
+
+plus ``advance`` (the fused two-kernel step) and host <-> device copies in
+the reference's Fortran layout (numpy C shape (nz+4, ny+4, nx+4, 5)).
+
+Everything runs on the GPU through ``libtlq.so``; there is no CPU fallback.
+A missing or unbuilt library raises ``TlqError``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import params as P
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libtlq.so")
+
+BUF_U, BUF_UP, BUF_UPP, BUF_UPRIM, BUF_F, BUF_G, BUF_H, BUF_S = range(8)
+BUFFERS = {"U": BUF_U, "UP": BUF_UP, "UPP": BUF_UPP, "UPrim": BUF_UPRIM, "F": BUF_F, "G": BUF_G,
+           "H": BUF_H, "S": BUF_S}
+ERR_NAMES = {-1: "TLQ_ERR_ARG", -2: "TLQ_ERR_HIP", -3: "TLQ_ERR_NOMEM",
+             -4: "TLQ_ERR_UNSUPPORTED", -5: "TLQ_ERR_STATE", -6: "TLQ_ERR_NONFINITE"}
+
+# Every entry point include/tlq.h declares (checked by the CPU tests).
+EXPORTS = (
+    "tlq_create", "tlq_destroy", "tlq_last_error", "tlq_set_winds", "tlq_set_point_masses",
+    "tlq_set_cooling_table", "tlq_upload", "tlq_download", "tlq_calc_prims", "tlq_time_step",
+    "tlq_fluxes", "tlq_godunov", "tlq_borders", "tlq_step", "tlq_cooling", "tlq_init_state",
+    "tlq_advance", "tlq_advance_n", "tlq_get_time", "tlq_sync", "tlq_check_state",
+    "tlq_fallback_fraction", "tlq_timer_start", "tlq_timer_stop", "tlq_kernel_timing",
+    "tlq_kernel_stats", "tlq_layout",
+)
+
+
+class TlqError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class TlqParams(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("xl", ctypes.c_double), ("xr", ctypes.c_double), ("yl", ctypes.c_double),
+        ("yr", ctypes.c_double), ("zl", ctypes.c_double), ("zr", ctypes.c_double),
+        ("gamma", ctypes.c_double), ("mu0", ctypes.c_double), ("cfl", ctypes.c_double),
+        ("eta", ctypes.c_double), ("cooling_limit", ctypes.c_double),
+        ("limiter", ctypes.c_int), ("bc", ctypes.c_int * 6), ("solver", ctypes.c_int),
+        ("precision", ctypes.c_int), ("cooling", ctypes.c_int), ("softening", ctypes.c_double),
+    ]
+
+
+class TlqWind(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("xc", "yc", "zc", "vx", "vy", "vz", "radius", "mdot", "vinf", "temp", "mu")]
+
+
+_lib = None
+
+
+def load_library(path: Optional[str] = None):
+    """Load libtlq.so (built by tlaloque_amd.build); raise if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise TlqError(-5, f"{path} is not built: run python -m tlaloque_amd.build")
+    L = ctypes.CDLL(path)
+    vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+    sig = {
+        "tlq_create": [ctypes.POINTER(TlqParams), ctypes.POINTER(vp)],
+        "tlq_destroy": [vp],
+        "tlq_set_winds": [vp, ctypes.POINTER(TlqWind), ip],
+        "tlq_set_point_masses": [vp, dp, ip],
+        "tlq_set_cooling_table": [vp, dp, dp, ip],
+        "tlq_upload": [vp, ip, dp],
+        "tlq_download": [vp, ip, dp],
+        "tlq_calc_prims": [vp, ip],
+        "tlq_time_step": [vp, dp],
+        "tlq_fluxes": [vp, ip, ip],
+        "tlq_godunov": [vp, ip, ctypes.c_double],
+        "tlq_borders": [vp, ip, ip],
+        "tlq_step": [vp, dp, ctypes.POINTER(ip), ctypes.c_double],
+        "tlq_cooling": [vp, ctypes.c_double],
+        "tlq_init_state": [vp],
+        "tlq_advance": [vp, dp, dp, ctypes.POINTER(ip)],
+        "tlq_advance_n": [vp, ip],
+        "tlq_get_time": [vp, dp, dp, ctypes.POINTER(ip)],
+        "tlq_sync": [vp],
+        "tlq_check_state": [vp, ctypes.POINTER(ctypes.c_longlong)],
+        "tlq_fallback_fraction": [vp, dp],
+        "tlq_timer_start": [vp],
+        "tlq_timer_stop": [vp, dp],
+        "tlq_kernel_timing": [vp, ip],
+        "tlq_kernel_stats": [vp, ip, ctypes.c_char_p, ip, dp, ctypes.POINTER(ctypes.c_longlong)],
+        "tlq_layout": [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ip),
+                       ctypes.POINTER(ip)],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    L.tlq_last_error.argtypes = []
+    L.tlq_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = _lib.tlq_last_error().decode() if _lib is not None else ""
+        raise TlqError(rc, msg)
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def to_c_params(p: P.HydroParams) -> TlqParams:
+    c = TlqParams()
+    for n in ("nx", "ny", "nz", "limiter", "solver", "precision", "cooling"):
+        setattr(c, n, int(getattr(p, n)))
+    for n in ("xl", "xr", "yl", "yr", "zl", "zr", "gamma", "mu0", "cfl", "eta", "cooling_limit",
+              "softening"):
+        setattr(c, n, float(getattr(p, n)))
+    for q in range(6):
+        c.bc[q] = int(p.bc[q])
+    return c
+
+
+class HydroCore:
+    """One problem on the GPU (one ``tlq_ctx``)."""
+
+    def __init__(self, params: P.HydroParams, cooling_table: Optional[Tuple[Sequence[float],
+                                                                           Sequence[float]]] = None):
+        L = load_library()
+        self.p = params
+        self._ctx = ctypes.c_void_p()
+        cp = to_c_params(params)
+        _check(L.tlq_create(ctypes.byref(cp), ctypes.byref(self._ctx)))
+        if params.winds:
+            arr = (TlqWind * len(params.winds))(*[TlqWind(*w.as_tuple()) for w in params.winds])
+            _check(L.tlq_set_winds(self._ctx, arr, len(params.winds)))
+        if params.masses:
+            m = np.array([[q.x, q.y, q.z, q.mass] for q in params.masses], dtype=np.float64)
+            m = np.ascontiguousarray(m.ravel())
+            _check(L.tlq_set_point_masses(self._ctx, _dptr(m), len(params.masses)))
+        if cooling_table is not None:
+            T = np.ascontiguousarray(cooling_table[0], dtype=np.float64)
+            Lg = np.ascontiguousarray(cooling_table[1], dtype=np.float64)
+            _check(L.tlq_set_cooling_table(self._ctx, _dptr(T), _dptr(Lg), len(T)))
+        self.shape = (params.nz + 4, params.ny + 4, params.nx + 4, 5)
+
+    # lifecycle ---------------------------------------------------------------
+    def close(self) -> None:
+        if self._ctx:
+            _check(_lib.tlq_destroy(self._ctx))
+            self._ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # copies ------------------------------------------------------------------
+    def upload(self, buf, host: np.ndarray) -> None:
+        b = BUFFERS[buf] if isinstance(buf, str) else int(buf)
+        a = np.ascontiguousarray(host, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError(f"expected shape {self.shape}, got {a.shape}")
+        _check(_lib.tlq_upload(self._ctx, b, _dptr(a)))
+
+    def download(self, buf) -> np.ndarray:
+        b = BUFFERS[buf] if isinstance(buf, str) else int(buf)
+        out = np.empty(self.shape, dtype=np.float64)
+        _check(_lib.tlq_download(self._ctx, b, _dptr(out)))
+        return out
+
+    # reference routines ----------------------------------------------------------
+    def calc_prims(self, src="U") -> None:
+        _check(_lib.tlq_calc_prims(self._ctx, BUFFERS[src] if isinstance(src, str) else src))
+
+    def time_step(self) -> float:
+        dt = ctypes.c_double()
+        _check(_lib.tlq_time_step(self._ctx, ctypes.byref(dt)))
+        return dt.value
+
+    def fluxes(self, src="U", choice: int = 1) -> None:
+        _check(_lib.tlq_fluxes(self._ctx, BUFFERS[src] if isinstance(src, str) else src, choice))
+
+    def godunov(self, dst, dt: float) -> None:
+        _check(_lib.tlq_godunov(self._ctx, BUFFERS[dst] if isinstance(dst, str) else dst, dt))
+
+    def borders(self, buf, order: int) -> None:
+        _check(_lib.tlq_borders(self._ctx, BUFFERS[buf] if isinstance(buf, str) else buf, order))
+
+    def step(self, t: float, it: int, dt: float) -> Tuple[float, int]:
+        tt, ii = ctypes.c_double(t), ctypes.c_int(it)
+        _check(_lib.tlq_step(self._ctx, ctypes.byref(tt), ctypes.byref(ii), dt))
+        return tt.value, ii.value
+
+    def cooling(self, dt: float) -> None:
+        _check(_lib.tlq_cooling(self._ctx, dt))
+
+    # fused path ---------------------------------------------------------------------
+    def set_state(self, u0: np.ndarray) -> None:
+        """Upload U and run mainHD.f90:38-44 (borders(U,1), calc_prims, first CFL)."""
+        self.upload("U", u0)
+        _check(_lib.tlq_init_state(self._ctx))
+
+    def advance(self, n: int = 1) -> None:
+        _check(_lib.tlq_advance_n(self._ctx, int(n)))
+
+    def time(self) -> Tuple[float, float, int]:
+        """(dt of the last step, t, it) -- synchronises."""
+        dt, t, it = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        _check(_lib.tlq_get_time(self._ctx, ctypes.byref(dt), ctypes.byref(t), ctypes.byref(it)))
+        return dt.value, t.value, it.value
+
+    def run_until(self, tfin: float, max_steps: int = 10 ** 9) -> int:
+        """mainHD.f90:61 loop: step while t < tfin (no clamp of the last dt)."""
+        _, t, it0 = self.time()
+        n = 0
+        while t < tfin and n < max_steps:
+            self.advance(1)
+            n += 1
+            _, t, _ = self.time()
+        return n
+
+    def sync(self) -> None:
+        _check(_lib.tlq_sync(self._ctx))
+
+    def check_state(self) -> int:
+        bad = ctypes.c_longlong()
+        _check(_lib.tlq_check_state(self._ctx, ctypes.byref(bad)))
+        return bad.value
+
+    def fallback_fraction(self) -> float:
+        f = ctypes.c_double()
+        _check(_lib.tlq_fallback_fraction(self._ctx, ctypes.byref(f)))
+        return f.value
+
+    # timing ---------------------------------------------------------------------------
+    def timer_start(self) -> None:
+        _check(_lib.tlq_timer_start(self._ctx))
+
+    def timer_stop(self) -> float:
+        ms = ctypes.c_double()
+        _check(_lib.tlq_timer_stop(self._ctx, ctypes.byref(ms)))
+        return ms.value
+
+    def kernel_timing(self, enable: bool) -> None:
+        _check(_lib.tlq_kernel_timing(self._ctx, 1 if enable else 0))
+
+    def kernel_stats(self) -> dict:
+        out = {}
+        for q in range(4):
+            name = ctypes.create_string_buffer(32)
+            ms, n = ctypes.c_double(), ctypes.c_longlong()
+            _check(_lib.tlq_kernel_stats(self._ctx, q, name, 32, ctypes.byref(ms), ctypes.byref(n)))
+            out[name.value.decode()] = (ms.value, n.value)
+        return out
+
+    def layout(self) -> Tuple[int, int, int]:
+        cells, px, ox = ctypes.c_longlong(), ctypes.c_int(), ctypes.c_int()
+        _check(_lib.tlq_layout(self._ctx, ctypes.byref(cells), ctypes.byref(px), ctypes.byref(ox)))
+        return cells.value, px.value, ox.value
