@@ -182,3 +182,34 @@ def build_and_run(name: str, p, u0: np.ndarray, tfin: float, cooling: bool = Fal
     shape = u0.shape
     prims = [np.fromfile(f, dtype="<f8").reshape(shape) for f in files]
     return {"prims": prims, "log": res.stdout}
+
+
+def compile_default() -> Optional[str]:
+    """Compile the reference for a small default problem (the checker build
+    done by ``__graft_entry__.build()``); returns the executable path, or None
+    when the reference sources or flang are absent."""
+    if not available():
+        return None
+    import sys
+    sys.path.insert(0, os.path.dirname(HERE))
+    from tlaloque_amd import params as P
+    p = P.HydroParams(nx=8, ny=6, nz=5, bc=(P.BC_OUTFLOW,) * 6)
+    case = os.path.join(OUT, "default")
+    src, obj = os.path.join(case, "src"), os.path.join(case, "obj")
+    os.makedirs(src, exist_ok=True)
+    os.makedirs(obj, exist_ok=True)
+    with open(os.path.join(src, "parameters.f90"), "w") as fh:
+        fh.write(_parameters_module(p, 0.1, None))
+    with open(os.path.join(src, "user.f90"), "w") as fh:
+        fh.write(_user_module(p))
+    objs = []
+    for s in REF_SOURCES_BEFORE_USER + ["user"] + REF_SOURCES_AFTER_USER:
+        path = (os.path.join(src, "parameters.f90") if s is None else
+                os.path.join(src, "user.f90") if s == "user" else os.path.join(REF, s))
+        o = os.path.join(obj, os.path.basename(path).replace(".f90", ".o"))
+        subprocess.run([FLANG, *FFLAGS, "-module-dir", obj, "-c", path, "-o", o], check=True,
+                       cwd=obj, capture_output=True)
+        objs.append(o)
+    exe = os.path.join(case, "tlaloque_ref")
+    subprocess.run([FLANG, *FFLAGS, *objs, "-o", exe], check=True, capture_output=True)
+    return exe

@@ -1,0 +1,171 @@
+"""Initial states of the benchmark and test problems.
+
+Each builder returns ``(params, u0)`` with ``u0`` the conserved state in the
+reference's Fortran layout (numpy C shape (nz+4, ny+4, nx+4, 5), ghosts
+included), the form ``user_initconds`` fills (user/user.f90:22-52).
+
+  sod_planar   3D planar shock tube along x, 128x32x32, gamma 1.4, FP64, HLLC,
+               outflow, CFL 0.4, t = 0.2 (exact solution: oracle/riemann.py).
+  blast        point explosion, gamma 5/3, rho = 1, p = 1e-5, E = 1 spread
+               uniformly over the cells within 3 cells of the centre, outflow.
+  wind         one stellar wind (radius 5 cells, 1000 km/s, 1e-6 Msun/yr) in a
+               uniform n = 1 cm^-3, T = 1e4 K medium, cgs, tabulated cooling.
+  clumpy       lognormal density (mean n = 1 cm^-3, sigma_ln rho = 1) from a
+               seeded Gaussian random field with P(k) ~ k^-4, velocities of the
+               same spectrum at RMS Mach 2, T = 1e4 K, a central 1e4 Msun point
+               mass softened over 2 cells, periodic boundaries, cooling.
+  cluster      8 winds at seeded positions within 0.2 box of the centre
+               (v_w in U[500, 2000] km/s, Mdot log-uniform in [1e-7, 1e-5]
+               Msun/yr), n = 10 cm^-3, T = 100 K, point-mass gravity, cooling.
+
+Sizes are arguments so that tests run the same set-ups small.
+"""
+from __future__ import annotations
+
+from typing import Tuple
+
+import numpy as np
+
+from . import params as P
+
+
+def cons_from_prims(p: P.HydroParams, rho, vel, pres) -> np.ndarray:
+    u = np.empty(p.shape + (5,), dtype=np.float64)
+    u[..., 0] = rho
+    u[..., 1:4] = rho[..., None] * vel
+    u[..., 4] = 0.5 * rho * (vel ** 2).sum(-1) + pres / (p.gamma - 1.0)
+    return u
+
+
+def centres(p: P.HydroParams):
+    """Cell-centre coordinates of indices -1..n+2 (source/HydroCore.f90:35-75)."""
+    x = P.xcoord(p, np.arange(-1, p.nx + 3))
+    y = P.ycoord(p, np.arange(-1, p.ny + 3))
+    z = P.zcoord(p, np.arange(-1, p.nz + 3))
+    return x, y, z
+
+
+def sod_planar(nx=128, ny=32, nz=32, precision=P.PREC_FP64) -> Tuple[P.HydroParams, np.ndarray]:
+    L = 1.0
+    p = P.HydroParams(nx=nx, ny=ny, nz=nz, xl=0.0, xr=L, yl=0.0, yr=L * ny / nx, zl=0.0,
+                      zr=L * nz / nx, gamma=1.4, cfl=0.4, bc=(P.BC_OUTFLOW,) * 6,
+                      precision=precision)
+    x, _, _ = centres(p)
+    left = (x < 0.5)[None, None, :] * np.ones(p.shape, dtype=bool)
+    rho = np.where(left, 1.0, 0.125)
+    pres = np.where(left, 1.0, 0.1)
+    vel = np.zeros(p.shape + (3,))
+    return p, cons_from_prims(p, rho, vel, pres)
+
+
+SOD_TFIN = 0.2
+
+
+def blast(n=256, precision=P.PREC_FP64, solver=P.SOLVER_HLLC, energy=1.0, rho0=1.0, p0=1e-5,
+          r_cells=3.0) -> Tuple[P.HydroParams, np.ndarray]:
+    p = P.HydroParams(nx=n, ny=n, nz=n, xl=-0.5, xr=0.5, yl=-0.5, yr=0.5, zl=-0.5, zr=0.5,
+                      gamma=5.0 / 3.0, cfl=0.4, bc=(P.BC_OUTFLOW,) * 6, precision=precision,
+                      solver=solver)
+    x, y, z = centres(p)
+    r = np.sqrt(x[None, None, :] ** 2 + y[None, :, None] ** 2 + z[:, None, None] ** 2)
+    inside = r < r_cells * p.dx
+    ncells = int(inside[2:-2, 2:-2, 2:-2].sum())
+    vol = ncells * p.dx * p.dy * p.dz
+    rho = np.full(p.shape, rho0)
+    pres = np.where(inside, (p.gamma - 1.0) * energy / vol, p0)
+    return p, cons_from_prims(p, rho, np.zeros(p.shape + (3,)), pres)
+
+
+def blast_time(radius=0.4, energy=1.0, rho0=1.0, xi0=1.151666417967) -> float:
+    """Time at which the Sedov shock reaches ``radius``: R = xi0 (E t^2/rho)^(1/5)."""
+    return float((radius / xi0) ** 2.5 * np.sqrt(rho0 / energy))
+
+
+def _grf(n: int, slope: float, rng: np.random.Generator, dtype=np.float32) -> np.ndarray:
+    """Zero-mean unit-variance periodic Gaussian random field with P(k) ~ k^slope."""
+    kx = np.fft.fftfreq(n).astype(dtype)
+    kz = np.fft.rfftfreq(n).astype(dtype)
+    k2 = kx[:, None, None] ** 2 + kx[None, :, None] ** 2 + kz[None, None, :] ** 2
+    k2[0, 0, 0] = 1.0
+    amp = k2 ** (slope / 4.0)  # sqrt(P) with P ~ k^slope
+    amp[0, 0, 0] = 0.0
+    noise = rng.standard_normal((n, n, n), dtype=dtype)
+    try:
+        import scipy.fft as sfft
+        f = sfft.rfftn(noise, workers=-1)
+        f *= amp
+        g = sfft.irfftn(f, s=(n, n, n), workers=-1)
+    except ImportError:  # pragma: no cover
+        f = np.fft.rfftn(noise) * amp
+        g = np.fft.irfftn(f, s=(n, n, n))
+    g = g.astype(dtype)
+    g -= g.mean()
+    g /= g.std()
+    return g
+
+
+def _pad_periodic(a: np.ndarray) -> np.ndarray:
+    """Interior field (n, n, n) -> (n+4)^3 with two periodic ghost layers."""
+    return np.pad(a, 2, mode="wrap")
+
+
+def clumpy(n=512, precision=P.PREC_FP32, seed=2024, box_pc=10.0, cooling=P.COOL_EXACT,
+           mach=2.0, sigma=1.0, temp=1e4, mass_msun=1e4, soft_cells=2.0):
+    L = box_pc * P.PC
+    p = P.HydroParams(nx=n, ny=n, nz=n, xl=-L / 2, xr=L / 2, yl=-L / 2, yr=L / 2, zl=-L / 2,
+                      zr=L / 2, gamma=5.0 / 3.0, cfl=0.4, bc=(P.BC_PERIODIC,) * 6,
+                      precision=precision, cooling=cooling,
+                      masses=[P.PointMass(0.0, 0.0, 0.0, mass_msun * P.MSUN)],
+                      softening=soft_cells * L / n)
+    rng = np.random.default_rng(seed)
+    g = _grf(n, -4.0, rng)
+    nd = np.exp(sigma * g.astype(np.float64))
+    nd /= nd.mean()
+    cs = np.sqrt(p.gamma * P.KB * temp / (p.mu0 * P.AMU))
+    vel = np.empty((n, n, n, 3), dtype=np.float32)
+    for c in range(3):
+        vel[..., c] = _grf(n, -4.0, rng)
+    vel *= np.float32(mach * cs / np.sqrt((vel.astype(np.float64) ** 2).sum(-1).mean()))
+    rho = _pad_periodic(nd * p.mu0 * P.AMU)
+    pres = _pad_periodic(nd * P.KB * temp)
+    v = np.stack([_pad_periodic(vel[..., c].astype(np.float64)) for c in range(3)], axis=-1)
+    return p, cons_from_prims(p, rho, v, pres)
+
+
+def wind(n=256, precision=P.PREC_FP32, box_pc=1.0, cooling=P.COOL_EXACT, temp_wind=1e4):
+    L = box_pc * P.PC
+    dx = L / n
+    w = P.Wind(xc=0.5 * dx, yc=0.5 * dx, zc=0.5 * dx, vx=0.0, vy=0.0, vz=0.0, radius=5.0 * dx,
+               mdot=1e-6 * P.MSUN / P.YEAR, vinf=1000.0 * P.KMS, temp=temp_wind, mu=1.0)
+    p = P.HydroParams(nx=n, ny=n, nz=n, xl=-L / 2, xr=L / 2, yl=-L / 2, yr=L / 2, zl=-L / 2,
+                      zr=L / 2, gamma=5.0 / 3.0, cfl=0.4, bc=(P.BC_OUTFLOW,) * 6,
+                      precision=precision, cooling=cooling, winds=[w])
+    rho = np.full(p.shape, 1.0 * p.mu0 * P.AMU)
+    pres = np.full(p.shape, 1.0 * P.KB * 1e4)
+    return p, cons_from_prims(p, rho, np.zeros(p.shape + (3,)), pres)
+
+
+def cluster(n=1024, precision=P.PREC_FP32, seed=7, box_pc=4.0, cooling=P.COOL_SURROGATE,
+            nstars=8, star_mass_msun=8.0):
+    L = box_pc * P.PC
+    dx = L / n
+    rng = np.random.default_rng(seed)
+    winds, masses = [], []
+    for _ in range(nstars):
+        while True:
+            pos = rng.uniform(-0.2 * L, 0.2 * L, 3)
+            if np.linalg.norm(pos) <= 0.2 * L:
+                break
+        pos = (np.floor(pos / dx) + 0.5) * dx + 0.25 * dx  # off cell centres (r > 0)
+        v = rng.uniform(500.0, 2000.0) * P.KMS
+        mdot = 10.0 ** rng.uniform(-7.0, -5.0) * P.MSUN / P.YEAR
+        winds.append(P.Wind(xc=pos[0], yc=pos[1], zc=pos[2], vx=0.0, vy=0.0, vz=0.0,
+                            radius=5.0 * dx, mdot=mdot, vinf=v, temp=1e4, mu=1.0))
+        masses.append(P.PointMass(pos[0], pos[1], pos[2], star_mass_msun * P.MSUN))
+    p = P.HydroParams(nx=n, ny=n, nz=n, xl=-L / 2, xr=L / 2, yl=-L / 2, yr=L / 2, zl=-L / 2,
+                      zr=L / 2, gamma=5.0 / 3.0, cfl=0.4, bc=(P.BC_OUTFLOW,) * 6,
+                      precision=precision, cooling=cooling, winds=winds, masses=masses,
+                      softening=2.0 * dx)
+    rho = np.full(p.shape, 10.0 * p.mu0 * P.AMU)
+    pres = np.full(p.shape, 10.0 * P.KB * 100.0)
+    return p, cons_from_prims(p, rho, np.zeros(p.shape + (3,)), pres)
