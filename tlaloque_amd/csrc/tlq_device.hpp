@@ -137,11 +137,17 @@ template <class R> __device__ __forceinline__ R lim_avg(int opt, R a, R b) {
             if (a * b <= 0.0) return 0.0;
             return a * b * (a + b) / (a * a + b * b);
         } else {
-            // a*b (a+b)/(a^2+b^2) written with the ratio t = a/b so that
-            // differences of cgs densities (~1e-25) do not underflow in FP32.
+            // a*b (a+b)/(a^2+b^2) = r (a+b)/(1+r^2) with r = min(|a|,|b|) /
+            // max(|a|,|b|) in (0, 1]: no a*b product to underflow for cgs
+            // density differences (~1e-25), no overflow, and no reciprocal
+            // of a subnormal (returns 0 when the larger difference is
+            // subnormal, where the product form underflows to 0 as well).
             if (!((a > 0.f && b > 0.f) || (a < 0.f && b < 0.f))) return 0.f;
-            const R t = rdiv(a, b);
-            return rdiv((a + b) * t, __builtin_fmaf(t, t, 1.f));
+            const R fa = __builtin_fabsf(a), fb = __builtin_fabsf(b);
+            const R big = fa > fb ? fa : fb, sml = fa > fb ? fb : fa;
+            if (!(big >= 1.17549435e-38f)) return 0.f;
+            const R r = sml * __builtin_amdgcn_rcpf(big);
+            return (a + b) * r * __builtin_amdgcn_rcpf(__builtin_fmaf(r, r, 1.f));
         }
     case LIM_MINMOD:
         s = __builtin_copysign(R(1), a);

@@ -14,11 +14,17 @@ struct Tiling;
 
 // Tile shapes of the fused kernels per precision.
 template <class R> struct TileShape;
+#ifndef TLQ_F32_TY
+#define TLQ_F32_TY 8
+#endif
+#ifndef TLQ_F64_TY
+#define TLQ_F64_TY 4
+#endif
 template <> struct TileShape<float> {
-    static constexpr int TX = 64, TY = 8, ZC = 64;
+    static constexpr int TX = 64, TY = TLQ_F32_TY, ZC = 64;
 };
 template <> struct TileShape<double> {
-    static constexpr int TX = 64, TY = 4, ZC = 64;
+    static constexpr int TX = 64, TY = TLQ_F64_TY, ZC = 64;
 };
 
 template <class R> struct Launch {
