@@ -5,6 +5,7 @@
 // description, and one HIP stream on which every call is ordered.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -12,7 +13,7 @@
 #include <vector>
 
 #include "../../include/tlq.h"
-#include "tlq_kernels.hpp"
+#include "tlq_fused.hpp"
 #include "tlq_launch.hpp"
 
 using namespace tlq;
@@ -209,12 +210,13 @@ template <class R> int do_advance(tlq_ctx *c, int nsteps) {
         {
             KTimer kt(c, 0);
             HIPCHK(Launch<R>::stage1(c->stream, c->dprob, c->tiling, cp<R>(c->u[par]), mp<R>(c->uh),
-                                     c->st, par));
+                                     c->st, par, c->host.solver));
         }
         {
             KTimer kt(c, 1);
             HIPCHK(Launch<R>::stage2(c->stream, c->dprob, c->tiling, cp<R>(c->u[par]),
-                                     cp<R>(c->uh), mp<R>(c->u[par ^ 1]), c->st, par));
+                                     cp<R>(c->uh), mp<R>(c->u[par ^ 1]), c->st, par,
+                                     c->host.limiter, c->host.solver));
         }
         c->cur ^= 1;
     }
@@ -422,6 +424,10 @@ int tlq_create(const tlq_params *p, tlq_ctx **out) {
     P.solver = p->solver;
     for (int q = 0; q < 6; ++q) P.bc[q] = p->bc[q];
     P.cooling = p->cooling;
+    P.f_cT = (float)(p->mu0 * AMU / KB);
+    P.f_inv_mu_amu = (float)(1.0 / (p->mu0 * AMU));
+    P.f_inv_gm1 = (float)(1.0 / (p->gamma - 1.0));
+    P.f_floor_f = (float)(1.0 - p->cooling_limit);
 
     auto bail = [&](int code, const char *m) {
         tlq_destroy(c);
@@ -511,8 +517,20 @@ int tlq_set_winds(tlq_ctx *c, const tlq_wind *w, int n) {
 int tlq_set_point_masses(tlq_ctx *c, const double *xyzm, int n) {
     if (int rc = check_ctx(c)) return rc;
     if (n < 0 || n > MAX_MASSES || (n > 0 && !xyzm)) return fail(TLQ_ERR_ARG, "bad mass list");
-    for (int q = 0; q < 4 * n; ++q) c->host.masses[q] = xyzm[q];
-    c->host.nmasses = n;
+    Problem &P = c->host;
+    for (int q = 0; q < 4 * n; ++q) P.masses[q] = xyzm[q];
+    P.nmasses = n;
+    // FP32 gravity in units of L = dx (see gravity_fast)
+    const double L = P.dx;
+    for (int m = 0; m < n; ++m) {
+        P.g_A[m] = (float)(2.0 * GR * xyzm[4 * m + 3] / (L * L));
+        P.g_X0[m] = (float)((P.xl - P.dx / 2.0 - xyzm[4 * m + 0]) / L);
+        P.g_Y0[m] = (float)((P.yl - P.dy / 2.0 - xyzm[4 * m + 1]) / L);
+        P.g_Z0[m] = (float)((P.zl - P.dz / 2.0 - xyzm[4 * m + 2]) / L);
+    }
+    P.g_dY = (float)(P.dy / L);
+    P.g_dZ = (float)(P.dz / L);
+    P.g_e2 = (float)((P.softening / L) * (P.softening / L));
     return push_problem(c);
 }
 
@@ -528,6 +546,23 @@ int tlq_set_cooling_table(tlq_ctx *c, const double *T, const double *L, int n) {
     }
     c->host.ntable = n;
     c->host.tmin = T[0];
+    Problem &P = c->host;
+    for (int q = 0; q < n; ++q) {
+        P.tab_Tf[q] = (float)T[q];
+        P.tab_lgTf[q] = (float)P.tab_logT[q];
+        P.tab_Lf[q] = (float)L[q];
+        P.tab_slopef[q] = q ? (float)((L[q] - L[q - 1]) / (P.tab_logT[q] - P.tab_logT[q - 1])) : 0.f;
+    }
+    P.f_tmin = (float)T[0];
+    P.f_tmax = (float)T[n - 1];
+    // direct indexing when every node lies within 0.4 spacings of a uniform grid
+    const double d = (P.tab_logT[n - 1] - P.tab_logT[0]) / (n - 1);
+    double dev = 0.0;
+    for (int q = 0; q < n; ++q)
+        dev = std::max(dev, std::fabs(P.tab_logT[q] - (P.tab_logT[0] + q * d)));
+    P.tab_uniform = dev < 0.4 * d ? 1 : 0;
+    P.f_lgT0 = (float)P.tab_logT[0];
+    P.f_inv_dlg = (float)(1.0 / d);
     return push_problem(c);
 }
 

@@ -60,6 +60,18 @@ struct Problem {
     double tab_logT[MAX_TABLE];  // log10 T_i
     double tab_T[MAX_TABLE];     // T_i
     double tab_L[MAX_TABLE];     // log10 Lambda_i
+    // FP32-mode constants (host-filled): cooling ...
+    float f_cT;           // mu0 AMU / KB   (T = p f_cT / rho)
+    float f_inv_mu_amu;   // 1 / (mu0 AMU)  (n = rho / (mu0 AMU))
+    float f_inv_gm1;      // 1 / (gamma - 1)
+    float f_tmin, f_tmax, f_floor_f;  // T_0, T_last, 1 - cooling_limit
+    int tab_uniform;      // table equally spaced in log10 T (direct indexing)
+    float f_lgT0, f_inv_dlg;
+    float tab_Tf[MAX_TABLE], tab_lgTf[MAX_TABLE], tab_Lf[MAX_TABLE], tab_slopef[MAX_TABLE];
+    // ... and gravity in units of L = dx: cell centre X_i = g_X0[m] + i (same for Y, Z
+    // with steps g_dY, g_dZ); acceleration factor g_A[m] = 2 G M / L^2.
+    float g_A[MAX_MASSES], g_X0[MAX_MASSES], g_Y0[MAX_MASSES], g_Z0[MAX_MASSES];
+    float g_dY, g_dZ, g_e2;
 };
 
 // ---------------------------------------------------------------------------
@@ -350,6 +362,136 @@ __device__ __forceinline__ double cool_exact(const Problem &P, double rho, doubl
     if (f < 1.0 - P.cooling_limit) f = 1.0 - P.cooling_limit;
     if (temp * f < 10.0) f = 10.0 / temp;
     return p * f;
+}
+
+// ---------------------------------------------------------------------------
+// FP32-mode fast paths (the FP64 mode keeps the reference's arithmetic).
+
+// 1/r in FP64 from the FP32 hardware reciprocal and one Newton step
+// (relative error ~1e-14).
+__device__ __forceinline__ double inv64(float r) {
+    const double rd = r;
+    double y = (double)__builtin_amdgcn_rcpf(r);
+    const double e = __builtin_fma(-rd, y, 1.0);
+    return __builtin_fma(y, e, y);
+}
+
+// calc_prims in FP32 with the FP64-guarded pressure recovery.
+__device__ __forceinline__ void cons2prim_fast(double gm1, const float u[5], float w[5]) {
+    const float inv = __builtin_amdgcn_rcpf(u[0]);
+    w[0] = u[0];
+    w[1] = u[1] * inv;
+    w[2] = u[2] * inv;
+    w[3] = u[3] * inv;
+    const double m1 = u[1], m2 = u[2], m3 = u[3];
+    double p = gm1 * ((double)u[4] - 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) * inv64(u[0]));
+    w[4] = p > 1e-20 ? (float)p : 1e-20f;
+}
+
+// apply_cooling for one cell in FP32 (cooling.f90:67-112): the table row is
+// found by direct indexing when the table is equally spaced in log10 T
+// (Z1.0.dat is: 0.01 dex), else by bisection; 10^x and exp via exp2/log2.
+__device__ __forceinline__ float cool_fast(const Problem &P, float rho, float p, float dt,
+                                           bool *cooled) {
+    const float rinv = __builtin_amdgcn_rcpf(rho);
+    const float temp = p * P.f_cT * rinv;
+    if (!(temp >= P.f_tmin)) {
+        *cooled = false;
+        return p;
+    }
+    *cooled = true;
+    const float lg = __builtin_amdgcn_logf(temp) * 0.301029995663981f;  // log10 T
+    float lgl;
+    if (!(temp < P.f_tmax)) {
+        lgl = P.tab_Lf[P.ntable - 1];
+    } else {
+        int hi;
+        if (P.tab_uniform) {
+            hi = (int)((lg - P.f_lgT0) * P.f_inv_dlg) + 1;
+            hi = hi < 1 ? 1 : (hi > P.ntable - 1 ? P.ntable - 1 : hi);
+            if (!(P.tab_Tf[hi] > temp) && hi < P.ntable - 1) ++hi;
+            if (hi > 1 && P.tab_Tf[hi - 1] > temp) --hi;
+        } else {
+            int lo = 0;
+            hi = P.ntable - 1;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (P.tab_Tf[mid] > temp) hi = mid;
+                else lo = mid;
+            }
+        }
+        lgl = P.tab_Lf[hi - 1] + P.tab_slopef[hi] * (lg - P.tab_lgTf[hi - 1]);
+    }
+    const float lam = __builtin_amdgcn_exp2f(lgl * 3.32192809488736f);  // 10^lgl
+    const float n = rho * P.f_inv_mu_amu;
+    const float eth = p * P.f_inv_gm1;
+    const float ce = lam * n * n * __builtin_amdgcn_rcpf(eth);
+    float f = __builtin_amdgcn_exp2f(-ce * dt * 1.44269504088896f);  // exp(-ce dt)
+    if (f < P.f_floor_f) f = P.f_floor_f;
+    if (temp * f < 10.f) f = 10.f * __builtin_amdgcn_rcpf(temp);
+    return p * f;
+}
+
+// S_Solver's central-difference gradient of the point-mass potential
+// (sources.f90:886-888) without cancellation, in FP32: for Phi = -GM/r,
+// (Phi(x+dx) - Phi(x-dx)) / (2dx) = 2 G M X / (r+ r- (r+ + r-)), X = x - x_c,
+// r+- the (softened) distances of the two neighbour centres -- the same
+// quantity as the reference's difference, rewritten with r-^2 - r+^2 = -4 dx X.
+__device__ __forceinline__ void gravity_fast(const Problem &P, int i, int j, int k, float g[3]) {
+    g[0] = g[1] = g[2] = 0.f;
+    for (int m = 0; m < P.nmasses; ++m) {
+        const float X = P.g_X0[m] + (float)i, Y = P.g_Y0[m] + P.g_dY * (float)j,
+                    Z = P.g_Z0[m] + P.g_dZ * (float)k;
+        const float X2 = X * X, Y2 = Y * Y, Z2 = Z * Z, e2 = P.g_e2;
+        const float dY = P.g_dY, dZ = P.g_dZ;
+        float c[3];
+        {
+            const float a = (X + 1.f) * (X + 1.f) + Y2 + Z2 + e2, b = (X - 1.f) * (X - 1.f) + Y2 + Z2 + e2;
+            const float ra = __builtin_amdgcn_sqrtf(a), rb = __builtin_amdgcn_sqrtf(b);
+            c[0] = X * __builtin_amdgcn_rcpf(ra * rb * (ra + rb));
+        }
+        {
+            const float a = X2 + (Y + dY) * (Y + dY) + Z2 + e2, b = X2 + (Y - dY) * (Y - dY) + Z2 + e2;
+            const float ra = __builtin_amdgcn_sqrtf(a), rb = __builtin_amdgcn_sqrtf(b);
+            c[1] = Y * __builtin_amdgcn_rcpf(ra * rb * (ra + rb));
+        }
+        {
+            const float a = X2 + Y2 + (Z + dZ) * (Z + dZ) + e2, b = X2 + Y2 + (Z - dZ) * (Z - dZ) + e2;
+            const float ra = __builtin_amdgcn_sqrtf(a), rb = __builtin_amdgcn_sqrtf(b);
+            c[2] = Z * __builtin_amdgcn_rcpf(ra * rb * (ra + rb));
+        }
+        g[0] += P.g_A[m] * c[0];
+        g[1] += P.g_A[m] * c[1];
+        g[2] += P.g_A[m] * c[2];
+    }
+}
+
+// Winds whose sphere can reach the box of cell indices [i0,i1]x[j0,j1]x[k0,k1]
+// (bit w set); lets blocks far from every wind skip the per-cell test.
+__device__ __forceinline__ unsigned long long winds_mask(const Problem &P, int i0, int i1, int j0,
+                                                         int j1, int k0, int k1) {
+    unsigned long long m = 0;
+    const double x0 = xcoord(P, i0), x1 = xcoord(P, i1), y0 = ycoord(P, j0), y1 = ycoord(P, j1),
+                 z0 = zcoord(P, k0), z1 = zcoord(P, k1);
+    for (int w = 0; w < P.nwinds; ++w) {
+        const WindDev &wd = P.winds[w];
+        const double cx = fmin(fmax(wd.xc, x0), x1), cy = fmin(fmax(wd.yc, y0), y1),
+                     cz = fmin(fmax(wd.zc, z0), z1);
+        const double d2 = (cx - wd.xc) * (cx - wd.xc) + (cy - wd.yc) * (cy - wd.yc) +
+                          (cz - wd.zc) * (cz - wd.zc);
+        if (d2 <= wd.radius * wd.radius * (1.0 + 1e-9)) m |= 1ull << w;
+    }
+    return m;
+}
+
+// winds_at restricted to the winds of `mask` (last wind wins).
+__device__ __forceinline__ bool winds_at_mask(const Problem &P, unsigned long long mask, int i,
+                                              int j, int k, double out[5]) {
+    if (!mask) return false;
+    const double x = xcoord(P, i), y = ycoord(P, j), z = zcoord(P, k);
+    for (int w = 63 - __builtin_clzll(mask); w >= 0; --w)
+        if (((mask >> w) & 1ull) && wind_state(P, P.winds[w], x, y, z, out)) return true;
+    return false;
 }
 
 // Order-preserving map of a non-negative double onto uint64 for atomicMax.

@@ -1,0 +1,621 @@
+// tlq_fused.hpp -- the fused time step: two kernels per iteration of the
+// reference's main loop (source/mainHD.f90:74-99).
+//
+//   k_stage1: U^{1/2} = U^n - dt/2 div F1(U^n) (+ dt/2 S)   HLLC.f90:92-219, Godunov
+//             + two-layer face ghosts of U^{1/2}             boundariesII
+//             + winds                                        user_boundaries
+//   k_stage2: UP = U^n - dt div F2(U^{1/2}) (+ dt S)         HLLC.f90:221-491, Godunov
+//             + one-layer face ghosts of U^{n+1} from UP     boundariesI, step
+//             + winds, U^{n+1} = UP + eta Lap(U^n)           step
+//             + cooling of U^{n+1}                           cooling.f90
+//             + CFL maxima of the cooled state               time_step (next step)
+//
+// A block owns TX x TY columns of cells and marches along z through ZC
+// planes.  The z stencil of each column lives in registers; the in-plane
+// stencil goes through LDS, one plane at a time.  Global loads for plane
+// k+1 are issued at the start of plane k (prefetch) and the block barriers
+// wait on LDS traffic only, so those loads stay in flight across the plane's
+// compute.  Every interior stencil reaches ghosts along one axis only, so
+// only face ghosts are read and written.
+#pragma once
+
+#include "tlq_kernels.hpp"
+
+namespace tlq {
+
+// Launch geometry: blocks cover TX x TY columns and ZC planes.  Block ids are
+// remapped so that each of the 8 XCDs (which receive blocks round-robin) gets
+// a contiguous range of tiles, keeping tiles that share halos in one L2.
+struct Tiling {
+    int nbx, nby, nbz;  // tiles per axis
+    int ntiles;         // nbx*nby*nbz
+    int per_xcd;        // ceil(ntiles/8); the grid has 8*per_xcd blocks
+};
+
+__device__ __forceinline__ bool tile_of_block(const Tiling &T, int &bx, int &by, int &bz) {
+    const int b = blockIdx.x;
+    const int L = (b & 7) * T.per_xcd + (b >> 3);
+    if (L >= T.ntiles) return false;
+    bx = L % T.nbx;
+    const int r = L / T.nbx;
+    by = r % T.nby;
+    bz = r / T.nby;
+    return true;
+}
+
+// Block barrier for LDS hand-off: waits for this wave's LDS operations only,
+// so prefetched global loads are not drained.
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// Face-ghost images of an interior cell along one axis (index a in 1..n):
+// calls f(ghost_layer, negate) for every ghost layer the boundary fill copies
+// from this cell -- two layers (boundariesII) or one (boundariesI).
+template <class Fn>
+__device__ __forceinline__ void ghost_images(int a, int n, int bcl, int bch, bool two, Fn &&f) {
+    if (bcl == BC_OUTFLOW) {
+        if (a == 1) {
+            f(0, false);
+            if (two) f(-1, false);
+        }
+    } else if (bcl == BC_REFLECTIVE) {
+        if (a == 1) f(0, true);
+        if (two && a == 2) f(-1, true);
+    } else if (bcl == BC_PERIODIC) {
+        if (a == n) f(0, false);
+        if (two && a == n - 1) f(-1, false);
+    }
+    if (bch == BC_OUTFLOW) {
+        if (a == n) {
+            f(n + 1, false);
+            if (two) f(n + 2, false);
+        }
+    } else if (bch == BC_REFLECTIVE) {
+        if (a == n) f(n + 1, true);
+        if (two && a == n - 1) f(n + 2, true);
+    } else if (bch == BC_PERIODIC) {
+        if (a == 1) f(n + 1, false);
+        if (two && a == 2) f(n + 2, false);
+    }
+}
+
+// Writes the pre-wind value v of interior cell (i, j, k) into the face ghosts
+// the boundary fill copies from it, then lets the winds of `wmask` override
+// ghosts inside a wind sphere (boundaries, then user_boundaries:
+// boundaries.f90:11-24).
+template <class R>
+__device__ __forceinline__ void write_face_ghosts(const Problem &P, unsigned long long wmask,
+                                                  R *__restrict__ out, int i, int j, int k,
+                                                  const double v[5], bool two) {
+    const Grid &g = P.g;
+    auto put = [&](int gi, int gj, int gk, int comp, bool neg) {
+        double w[5];
+        if (!winds_at_mask(P, wmask, gi, gj, gk, w)) {
+#pragma unroll
+            for (int e = 0; e < 5; ++e) w[e] = v[e];
+            if (neg) w[comp] = -w[comp];
+        }
+        const long long c = g.idx(gi, gj, gk);
+#pragma unroll
+        for (int e = 0; e < 5; ++e) out[e * g.nc + c] = (R)w[e];
+    };
+    if (i <= 2 || i >= g.nx - 1)
+        ghost_images(i, g.nx, P.bc[0], P.bc[1], two, [&](int q, bool neg) { put(q, j, k, 1, neg); });
+    if (j <= 2 || j >= g.ny - 1)
+        ghost_images(j, g.ny, P.bc[2], P.bc[3], two, [&](int q, bool neg) { put(i, q, k, 2, neg); });
+    if (k <= 2 || k >= g.nz - 1)
+        ghost_images(k, g.nz, P.bc[4], P.bc[5], two, [&](int q, bool neg) { put(i, j, q, 3, neg); });
+}
+
+// calc_prims of one cell: the reference's arithmetic in FP64 mode, the
+// guarded fast path in FP32 mode.
+template <class R>
+__device__ __forceinline__ void prims(const Problem &P, const R u[5], R w[5]) {
+    if constexpr (sizeof(R) == 8) cons2prim(P.gamma, u, w);
+    else cons2prim_fast(P.gamma - 1.0, u, w);
+}
+
+// Conserved state of cell (i, j, k), or a uniform dummy outside the array
+// (only tiles that overhang the mesh read such cells, for faces they discard).
+template <class R>
+__device__ __forceinline__ void load_raw(const Problem &P, const R *__restrict__ A, int i, int j,
+                                         int k, R u[5]) {
+    const Grid &g = P.g;
+    if (i >= -1 && i <= g.nx + 2 && j >= -1 && j <= g.ny + 2 && k >= -1 && k <= g.nz + 2) {
+        load5(A, g.nc, g.idx(i, j, k), u);
+    } else {
+        u[0] = R(1);
+        u[1] = u[2] = u[3] = R(0);
+        u[4] = R(1);
+    }
+}
+
+// Slope average with the limiter fixed at compile time (LIM > 0) or read
+// at run time (LIM == 0).
+template <int LIM, class R> __device__ __forceinline__ R avg(int lim, R a, R b) {
+    return lim_avg<R>(LIM > 0 ? LIM : lim, a, b);
+}
+
+// Limited reconstruction of the face between b and c from the four states
+// a, b, c, d along the axis (limiter, HLLC.f90:574-582).
+template <int LIM, class R>
+__device__ __forceinline__ void reconstruct(int lim, const R a[5], const R b[5], const R c[5],
+                                            const R d[5], R wl[5], R wr[5]) {
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+        const R dl = b[e] - a[e], dm = c[e] - b[e], dr = d[e] - c[e];
+        wl[e] = b[e] + R(0.5) * avg<LIM>(lim, dl, dm);
+        wr[e] = c[e] - R(0.5) * avg<LIM>(lim, dm, dr);
+    }
+}
+
+template <int D, int SOL, class R>
+__device__ __forceinline__ void riemann_t(R gamma, R gratio, const R wl[5], const R wr[5],
+                                          const R ul[5], const R ur[5], R f[5]) {
+    if constexpr (SOL == SOLVER_HLL) hll_face<D>(gamma, gratio, wl, wr, ul, ur, f);
+    else hllc_face<D>(gamma, gratio, wl, wr, ul, ur, f);
+}
+
+// Godunov update of one cell: U - dtdx dF - dtdy dG - dtdz dH.  FP64 mode
+// keeps the reference's operation order (HLLC.f90:37-38); FP32 mode forms
+// the flux-difference sum in FP32 and accumulates it onto U in FP64.
+template <class R>
+__device__ __forceinline__ double godunov1(R u, R fp, R fm, R gp, R gm, R hp, R hm, double dtdx,
+                                           double dtdy, double dtdz) {
+    if constexpr (sizeof(R) == 8) {
+        return u - dtdx * (fp - fm) - dtdy * (gp - gm) - dtdz * (hp - hm);
+    } else {
+        const float d = (float)dtdx * (fp - fm) + (float)dtdy * (gp - gm) + (float)dtdz * (hp - hm);
+        return (double)u - (double)d;
+    }
+}
+
+// Gravity source S (sources.f90:886-893) from the cell's primitives.
+template <class R>
+__device__ __forceinline__ void gravity_src(const Problem &P, int i, int j, int k, const R w[5],
+                                            double s[5]) {
+    if constexpr (sizeof(R) == 8) {
+        gravity_source(P, i, j, k, w[0], w[1], w[2], w[3], s);
+    } else {
+        float g[3];
+        gravity_fast(P, i, j, k, g);
+        const float rg0 = w[0] * g[0], rg1 = w[0] * g[1], rg2 = w[0] * g[2];
+        s[0] = 0.0;
+        s[1] = -rg0;
+        s[2] = -rg1;
+        s[3] = -rg2;
+        s[4] = -(w[1] * rg0 + w[2] * rg1 + w[3] * rg2);
+    }
+}
+
+// Halo bookkeeping: thread `tid` loads one cell of the ring around the tile.
+//   stage 1: ring of width 1 (no corners), conserved -> sW (prims) and sU
+//   stage 2: ring of width 2 of U^{1/2} (no corners) -> sW, and its inner
+//            ring -> sU; then the width-1 ring of U^n -> sN
+struct HaloTask {
+    int lx, ly;  // offset from the tile origin (cell x0+lx, y0+ly)
+    int kind;    // 0 none, 1 U^{1/2} (sW [+ sU if inner]), 2 U^n (sN)
+};
+
+template <int TX, int TY>
+__device__ __forceinline__ HaloTask halo_task1(int h) {
+    HaloTask t{0, 0, 0};
+    if (h < 2 * TY) {
+        t.ly = h >> 1;
+        t.lx = (h & 1) ? TX : -1;
+        t.kind = 1;
+    } else if (h < 2 * (TX + TY)) {
+        const int q = h - 2 * TY;
+        t.lx = q >> 1;
+        t.ly = (q & 1) ? TY : -1;
+        t.kind = 1;
+    }
+    return t;
+}
+
+template <int TX, int TY>
+__device__ __forceinline__ HaloTask halo_task2(int h) {
+    constexpr int NWH = 4 * TY + 4 * TX, NNH = 2 * TY + 2 * TX;
+    HaloTask t{0, 0, 0};
+    if (h < 4 * TY) {
+        t.ly = h >> 2;
+        const int s = h & 3;
+        t.lx = s < 2 ? s - 2 : TX + (s - 2);
+        t.kind = 1;
+    } else if (h < NWH) {
+        const int q = h - 4 * TY;
+        t.lx = q >> 2;
+        const int s = q & 3;
+        t.ly = s < 2 ? s - 2 : TY + (s - 2);
+        t.kind = 1;
+    } else if (h < NWH + NNH) {
+        const int q = h - NWH;
+        if (q < 2 * TY) {
+            t.ly = q >> 1;
+            t.lx = (q & 1) ? TX : -1;
+        } else {
+            const int r = q - 2 * TY;
+            t.lx = r >> 1;
+            t.ly = (r & 1) ? TY : -1;
+        }
+        t.kind = 2;
+    }
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+template <class R, int TX, int TY, int ZC, int SOL>
+__global__ __launch_bounds__(TX *TY) void k_stage1(const Problem *__restrict__ Pp, Tiling T,
+                                                    const R *__restrict__ U,
+                                                    R *__restrict__ Uh,
+                                                    StepState *__restrict__ st, int parity) {
+    constexpr int NT = TX * TY;
+    static_assert(2 * (TX + TY) <= NT, "one halo cell per thread");
+    constexpr int LX = TX + 2, LY = TY + 2;
+    __shared__ R sW[5][LY][LX];
+    __shared__ R sU[5][LY][LX];
+    __shared__ R sF[5][TY][TX + 1];
+    __shared__ R sG[5][TY + 1][TX];
+    __shared__ unsigned long long s_wmask;
+
+    const Problem &P = *Pp;
+    const Grid &g = P.g;
+    int bx, by, bz;
+    if (!tile_of_block(T, bx, by, bz)) return;
+    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+    const int x0 = 1 + bx * TX, y0 = 1 + by * TY, z0 = 1 + bz * ZC;
+    const int z1 = min(z0 + ZC - 1, g.nz);
+    const int i = x0 + tx, j = y0 + ty;
+    const bool active = i <= g.nx && j <= g.ny;
+    const HaloTask ht = halo_task1<TX, TY>(tid);
+
+    const double dt = dt_from_maxima(P, st->maxima[parity]);
+    const double dth = dt / 2.0;
+    if (blockIdx.x == 0 && tid < 4) st->maxima[parity ^ 1][tid] = 0ull;
+    const double dtdx = dth / P.dx, dtdy = dth / P.dy, dtdz = dth / P.dz;
+    const R gam = (R)P.gamma, gr = (R)(P.gamma / (P.gamma - 1.0));
+    const bool grav = P.nmasses > 0;
+    if (tid == 0)
+        s_wmask = P.nwinds ? winds_mask(P, x0 - 2, x0 + TX + 1, y0 - 2, y0 + TY + 1, z0 - 2, z1 + 2)
+                           : 0ull;
+
+    R uc[5], wc[5], hm[5], pf_n[5], pf_h[5];
+    {
+        R up[5], wp[5];
+        load_raw(P, U, i, j, z0 - 1, up);
+        prims(P, up, wp);
+        load_raw(P, U, i, j, z0, uc);
+        prims(P, uc, wc);
+        riemann_t<2, SOL>(gam, gr, wp, wc, up, uc, hm);
+    }
+    load_raw(P, U, i, j, z0 + 1, pf_n);
+    if (ht.kind) load_raw(P, U, x0 + ht.lx, y0 + ht.ly, z0, pf_h);
+
+    for (int k = z0; k <= z1; ++k) {
+        R un[5], wn[5], hr[5], hp[5];
+#pragma unroll
+        for (int e = 0; e < 5; ++e) {
+            un[e] = pf_n[e];
+            hr[e] = pf_h[e];
+        }
+        if (k < z1) {  // prefetch plane k+1
+            load_raw(P, U, i, j, k + 2, pf_n);
+            if (ht.kind) load_raw(P, U, x0 + ht.lx, y0 + ht.ly, k + 1, pf_h);
+        }
+        prims(P, un, wn);
+        riemann_t<2, SOL>(gam, gr, wc, wn, uc, un, hp);
+        lds_barrier();
+#pragma unroll
+        for (int e = 0; e < 5; ++e) {
+            sW[e][ty + 1][tx + 1] = wc[e];
+            sU[e][ty + 1][tx + 1] = uc[e];
+        }
+        if (ht.kind) {
+            R wh[5];
+            prims(P, hr, wh);
+#pragma unroll
+            for (int e = 0; e < 5; ++e) {
+                sW[e][ht.ly + 1][ht.lx + 1] = wh[e];
+                sU[e][ht.ly + 1][ht.lx + 1] = hr[e];
+            }
+        }
+        lds_barrier();
+        for (int q = tid; q < (TX + 1) * TY + TX * (TY + 1); q += NT) {
+            R wl[5], wr[5], ul[5], ur[5], f[5];
+            if (q < (TX + 1) * TY) {  // x face left of tile cell fx in row fy
+                const int fy = q / (TX + 1), fx = q % (TX + 1);
+#pragma unroll
+                for (int e = 0; e < 5; ++e) {
+                    wl[e] = sW[e][fy + 1][fx];
+                    wr[e] = sW[e][fy + 1][fx + 1];
+                    ul[e] = sU[e][fy + 1][fx];
+                    ur[e] = sU[e][fy + 1][fx + 1];
+                }
+                riemann_t<0, SOL>(gam, gr, wl, wr, ul, ur, f);
+#pragma unroll
+                for (int e = 0; e < 5; ++e) sF[e][fy][fx] = f[e];
+            } else {  // y face below tile row fy in column fx
+                const int r = q - (TX + 1) * TY;
+                const int fy = r / TX, fx = r % TX;
+#pragma unroll
+                for (int e = 0; e < 5; ++e) {
+                    wl[e] = sW[e][fy][fx + 1];
+                    wr[e] = sW[e][fy + 1][fx + 1];
+                    ul[e] = sU[e][fy][fx + 1];
+                    ur[e] = sU[e][fy + 1][fx + 1];
+                }
+                riemann_t<1, SOL>(gam, gr, wl, wr, ul, ur, f);
+#pragma unroll
+                for (int e = 0; e < 5; ++e) sG[e][fy][fx] = f[e];
+            }
+        }
+        lds_barrier();
+        if (active) {
+            double v[5];
+#pragma unroll
+            for (int e = 0; e < 5; ++e)
+                v[e] = godunov1<R>(uc[e], sF[e][ty][tx + 1], sF[e][ty][tx], sG[e][ty + 1][tx],
+                                   sG[e][ty][tx], hp[e], hm[e], dtdx, dtdy, dtdz);
+            if (grav) {
+                double s[5];
+                gravity_src<R>(P, i, j, k, wc, s);
+#pragma unroll
+                for (int e = 0; e < 5; ++e) v[e] = v[e] + dth * s[e];
+            }
+            const unsigned long long wmask = s_wmask;
+            double o[5];
+            if (!winds_at_mask(P, wmask, i, j, k, o)) {
+#pragma unroll
+                for (int e = 0; e < 5; ++e) o[e] = v[e];
+            }
+            const long long c = g.idx(i, j, k);
+#pragma unroll
+            for (int e = 0; e < 5; ++e) Uh[e * g.nc + c] = (R)o[e];
+            write_face_ghosts(P, wmask, Uh, i, j, k, v, true);
+        }
+#pragma unroll
+        for (int e = 0; e < 5; ++e) {
+            hm[e] = hp[e];
+            uc[e] = un[e];
+            wc[e] = wn[e];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+template <class R, int TX, int TY, int ZC, int LIM, int SOL>
+__global__ __launch_bounds__(TX *TY) void k_stage2(const Problem *__restrict__ Pp, Tiling T,
+                                                    const R *__restrict__ U,
+                                                    const R *__restrict__ Uh,
+                                                    R *__restrict__ Unew,
+                                                    StepState *__restrict__ st, int parity) {
+    constexpr int NT = TX * TY;
+    static_assert(6 * (TX + TY) <= NT, "one halo cell per thread");
+    constexpr int WX = TX + 4, WY = TY + 4;  // two-cell halo (primitives of U^{1/2})
+    constexpr int CX = TX + 2, CY = TY + 2;  // one-cell halo (conserved U^{1/2}, U^n)
+    __shared__ R sW[5][WY][WX];
+    __shared__ R sU[5][CY][CX];
+    __shared__ R sN[5][CY][CX];
+    __shared__ R sF[5][TY][TX + 1];
+    __shared__ R sG[5][TY + 1][TX];
+    __shared__ unsigned long long s_wmask;
+
+    const Problem &P = *Pp;
+    const Grid &g = P.g;
+    int bx, by, bz;
+    if (!tile_of_block(T, bx, by, bz)) return;
+    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+    const int x0 = 1 + bx * TX, y0 = 1 + by * TY, z0 = 1 + bz * ZC;
+    const int z1 = min(z0 + ZC - 1, g.nz);
+    const int i = x0 + tx, j = y0 + ty;
+    const bool active = i <= g.nx && j <= g.ny;
+    const HaloTask ht = halo_task2<TX, TY>(tid);
+    const R *__restrict__ hsrc = ht.kind == 2 ? U : Uh;
+
+    const double dt = dt_from_maxima(P, st->maxima[parity]);
+    const double dtdx = dt / P.dx, dtdy = dt / P.dy, dtdz = dt / P.dz;
+    const R gam = (R)P.gamma, gr = (R)(P.gamma / (P.gamma - 1.0));
+    const int lim = P.limiter;
+    const bool grav = P.nmasses > 0;
+    const bool cool = P.cooling != 0 && P.ntable > 1;
+    const double g1 = P.gamma - 1.0;
+    const double eta = P.eta;
+    if (blockIdx.x == 0 && tid == 0) {
+        st->time[parity ^ 1] = st->time[parity] + dt;
+        st->dt = dt;
+        st->it = st->it + 1;
+    }
+    if (tid == 0)
+        s_wmask = P.nwinds ? winds_mask(P, x0 - 2, x0 + TX + 1, y0 - 2, y0 + TY + 1, z0 - 2, z1 + 2)
+                           : 0ull;
+
+    // z pipeline of U^{1/2}: primitives at k-1, k, k+1 and conserved at k,
+    // k+1; U^n at k-1, k; the z flux at k-1/2.
+    R wm[5], wc[5], wp[5], uc[5], up[5], hm[5], nm[5], nc[5];
+    R pf_q[5], pf_n[5], pf_h[5];
+    {
+        R wmm[5], um[5], t5[5];
+        load_raw(P, Uh, i, j, z0 - 2, t5);
+        prims(P, t5, wmm);
+        load_raw(P, Uh, i, j, z0 - 1, um);
+        prims(P, um, wm);
+        load_raw(P, Uh, i, j, z0, uc);
+        prims(P, uc, wc);
+        load_raw(P, Uh, i, j, z0 + 1, up);
+        prims(P, up, wp);
+        R wl[5], wr[5];
+        reconstruct<LIM>(lim, wmm, wm, wc, wp, wl, wr);
+        riemann_t<2, SOL>(gam, gr, wl, wr, um, uc, hm);
+        load_raw(P, U, i, j, z0 - 1, nm);
+        load_raw(P, U, i, j, z0, nc);
+    }
+    load_raw(P, Uh, i, j, z0 + 2, pf_q);
+    load_raw(P, U, i, j, z0 + 1, pf_n);
+    if (ht.kind) load_raw(P, hsrc, x0 + ht.lx, y0 + ht.ly, z0, pf_h);
+
+    float mx = 0.f, my = 0.f, mz = 0.f;
+    double mxd = 0.0, myd = 0.0, mzd = 0.0;
+    for (int k = z0; k <= z1; ++k) {
+        R uq[5], wq[5], np[5], hr[5], hp[5];
+#pragma unroll
+        for (int e = 0; e < 5; ++e) {
+            uq[e] = pf_q[e];
+            np[e] = pf_n[e];
+            hr[e] = pf_h[e];
+        }
+        if (k < z1) {  // prefetch plane k+1
+            load_raw(P, Uh, i, j, k + 3, pf_q);
+            load_raw(P, U, i, j, k + 2, pf_n);
+            if (ht.kind) load_raw(P, hsrc, x0 + ht.lx, y0 + ht.ly, k + 1, pf_h);
+        }
+        prims(P, uq, wq);
+        {
+            R wl[5], wr[5];
+            reconstruct<LIM>(lim, wm, wc, wp, wq, wl, wr);
+            riemann_t<2, SOL>(gam, gr, wl, wr, uc, up, hp);
+        }
+        lds_barrier();
+#pragma unroll
+        for (int e = 0; e < 5; ++e) {
+            sW[e][ty + 2][tx + 2] = wc[e];
+            sU[e][ty + 1][tx + 1] = uc[e];
+            sN[e][ty + 1][tx + 1] = nc[e];
+        }
+        if (ht.kind == 1) {
+            R wh[5];
+            prims(P, hr, wh);
+#pragma unroll
+            for (int e = 0; e < 5; ++e) sW[e][ht.ly + 2][ht.lx + 2] = wh[e];
+            if (ht.lx >= -1 && ht.lx <= TX && ht.ly >= -1 && ht.ly <= TY) {
+#pragma unroll
+                for (int e = 0; e < 5; ++e) sU[e][ht.ly + 1][ht.lx + 1] = hr[e];
+            }
+        } else if (ht.kind == 2) {
+#pragma unroll
+            for (int e = 0; e < 5; ++e) sN[e][ht.ly + 1][ht.lx + 1] = hr[e];
+        }
+        lds_barrier();
+        for (int q = tid; q < (TX + 1) * TY + TX * (TY + 1); q += NT) {
+            R a[5], b[5], c[5], d[5], ul[5], ur[5], wl[5], wr[5], f[5];
+            if (q < (TX + 1) * TY) {  // x face left of tile cell fx in row fy
+                const int fy = q / (TX + 1), fx = q % (TX + 1);
+#pragma unroll
+                for (int e = 0; e < 5; ++e) {
+                    a[e] = sW[e][fy + 2][fx];
+                    b[e] = sW[e][fy + 2][fx + 1];
+                    c[e] = sW[e][fy + 2][fx + 2];
+                    d[e] = sW[e][fy + 2][fx + 3];
+                    ul[e] = sU[e][fy + 1][fx];
+                    ur[e] = sU[e][fy + 1][fx + 1];
+                }
+                reconstruct<LIM>(lim, a, b, c, d, wl, wr);
+                riemann_t<0, SOL>(gam, gr, wl, wr, ul, ur, f);
+#pragma unroll
+                for (int e = 0; e < 5; ++e) sF[e][fy][fx] = f[e];
+            } else {  // y face below tile row fy in column fx
+                const int r = q - (TX + 1) * TY;
+                const int fy = r / TX, fx = r % TX;
+#pragma unroll
+                for (int e = 0; e < 5; ++e) {
+                    a[e] = sW[e][fy][fx + 2];
+                    b[e] = sW[e][fy + 1][fx + 2];
+                    c[e] = sW[e][fy + 2][fx + 2];
+                    d[e] = sW[e][fy + 3][fx + 2];
+                    ul[e] = sU[e][fy][fx + 1];
+                    ur[e] = sU[e][fy + 1][fx + 1];
+                }
+                reconstruct<LIM>(lim, a, b, c, d, wl, wr);
+                riemann_t<1, SOL>(gam, gr, wl, wr, ul, ur, f);
+#pragma unroll
+                for (int e = 0; e < 5; ++e) sG[e][fy][fx] = f[e];
+            }
+        }
+        lds_barrier();
+        if (active) {
+            double v[5];
+#pragma unroll
+            for (int e = 0; e < 5; ++e)
+                v[e] = godunov1<R>(nc[e], sF[e][ty][tx + 1], sF[e][ty][tx], sG[e][ty + 1][tx],
+                                   sG[e][ty][tx], hp[e], hm[e], dtdx, dtdy, dtdz);
+            if (grav) {
+                double s[5];
+                gravity_src<R>(P, i, j, k, wc, s);
+#pragma unroll
+                for (int e = 0; e < 5; ++e) v[e] = v[e] + dt * s[e];
+            }
+            const unsigned long long wmask = s_wmask;
+            // ghosts of U^{n+1} = boundary fill of the pre-wind UP, then winds
+            write_face_ghosts(P, wmask, Unew, i, j, k, v, false);
+            double o[5];
+            if (!winds_at_mask(P, wmask, i, j, k, o)) {
+#pragma unroll
+                for (int e = 0; e < 5; ++e) o[e] = v[e];
+            }
+            // step: U^{n+1} = UP + eta * Lap(U^n)  (HydroCore.f90:184-193)
+            R oq[5];
+#pragma unroll
+            for (int e = 0; e < 5; ++e) {
+                const R lap = sN[e][ty + 1][tx + 2] + sN[e][ty + 1][tx] + sN[e][ty + 2][tx + 1] +
+                              sN[e][ty][tx + 1] + np[e] + nm[e] - R(6) * nc[e];
+                oq[e] = (R)(o[e] + eta * (double)lap);
+            }
+            const long long cc = g.idx(i, j, k);
+            // calc_prims + cooling (cooling.f90:64-117) + time_step maxima
+            if constexpr (sizeof(R) == 8) {
+                const double rho = oq[0];
+                const double vx = oq[1] / oq[0], vy = oq[2] / oq[0], vz = oq[3] / oq[0];
+                double p = pressure64(P.gamma, rho, oq[1], oq[2], oq[3], oq[4]);
+                if (cool) {
+                    bool cooled;
+                    const double pn = cool_exact(P, rho, p, dt, &cooled);
+                    if (cooled) {
+                        const double ek = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
+                        oq[4] = ek + pn / g1;
+                        p = pn;
+                    }
+                }
+                const double cs = sqrt(P.gamma * p / rho);
+                mxd = fmax(mxd, fabs(vx) + cs);
+                myd = fmax(myd, fabs(vy) + cs);
+                mzd = fmax(mzd, fabs(vz) + cs);
+            } else {
+                const float rinv = __builtin_amdgcn_rcpf(oq[0]);
+                const float vx = oq[1] * rinv, vy = oq[2] * rinv, vz = oq[3] * rinv;
+                const double m1 = oq[1], m2 = oq[2], m3 = oq[3];
+                const double ek = 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) * inv64(oq[0]);
+                double pd = g1 * ((double)oq[4] - ek);
+                float p = pd > 1e-20 ? (float)pd : 1e-20f;
+                if (cool) {
+                    bool cooled;
+                    const float pn = cool_fast(P, oq[0], p, (float)dt, &cooled);
+                    if (cooled) {
+                        oq[4] = (float)(ek + (double)pn * (double)P.f_inv_gm1);
+                        p = pn;
+                    }
+                }
+                const float cs = __builtin_amdgcn_sqrtf((float)P.gamma * p * rinv);
+                mx = fmaxf(mx, fabsf(vx) + cs);
+                my = fmaxf(my, fabsf(vy) + cs);
+                mz = fmaxf(mz, fabsf(vz) + cs);
+            }
+#pragma unroll
+            for (int e = 0; e < 5; ++e) Unew[e * g.nc + cc] = oq[e];
+        }
+#pragma unroll
+        for (int e = 0; e < 5; ++e) {
+            hm[e] = hp[e];
+            wm[e] = wc[e];
+            wc[e] = wp[e];
+            wp[e] = wq[e];
+            uc[e] = up[e];
+            up[e] = uq[e];
+            nm[e] = nc[e];
+            nc[e] = np[e];
+        }
+    }
+    if constexpr (sizeof(R) == 8) block_max3<NT>(mxd, myd, mzd, st->maxima[parity ^ 1]);
+    else block_max3<NT>((double)mx, (double)my, (double)mz, st->maxima[parity ^ 1]);
+}
+
+}  // namespace tlq

@@ -18,13 +18,13 @@ template <class R> struct TileShape;
 #define TLQ_F32_TY 8
 #endif
 #ifndef TLQ_F64_TY
-#define TLQ_F64_TY 4
+#define TLQ_F64_TY 8
 #endif
 template <> struct TileShape<float> {
     static constexpr int TX = 64, TY = TLQ_F32_TY, ZC = 64;
 };
 template <> struct TileShape<double> {
-    static constexpr int TX = 64, TY = TLQ_F64_TY, ZC = 64;
+    static constexpr int TX = 32, TY = TLQ_F64_TY, ZC = 64;
 };
 
 template <class R> struct Launch {
@@ -52,9 +52,10 @@ template <class R> struct Launch {
     static hipError_t check(hipStream_t s, const Problem *P, const R *U, unsigned long long *bad,
                             long long ninterior);
     static hipError_t stage1(hipStream_t s, const Problem *P, const Tiling &T, const R *U, R *Uh,
-                             StepState *st, int parity);
+                             StepState *st, int parity, int solver);
     static hipError_t stage2(hipStream_t s, const Problem *P, const Tiling &T, const R *U,
-                             const R *Uh, R *Unew, StepState *st, int parity);
+                             const R *Uh, R *Unew, StepState *st, int parity, int limiter,
+                             int solver);
 };
 
 }  // namespace tlq

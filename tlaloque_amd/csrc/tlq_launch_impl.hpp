@@ -2,7 +2,7 @@
 // precision translation unit).
 #pragma once
 
-#include "tlq_kernels.hpp"
+#include "tlq_fused.hpp"
 #include "tlq_launch.hpp"
 
 namespace tlq {
@@ -22,8 +22,9 @@ hipError_t Launch<R>::calc_prims(hipStream_t s, const Problem *P, const R *U, R 
 template <class R>
 hipError_t Launch<R>::maxima(hipStream_t s, const Problem *P, const R *A, int prims,
                              unsigned long long *out, long long n) {
-    hipLaunchKernelGGL((k_maxima<R, 256>), dim3(grid_for(n, 256 * 4)), dim3(256), 0, s, P, A, prims,
-                       out);
+    unsigned nb = grid_for(n, 256 * 16);
+    if (nb > 2048) nb = 2048;
+    hipLaunchKernelGGL((k_maxima<R, 256>), dim3(nb), dim3(256), 0, s, P, A, prims, out);
     return hipGetLastError();
 }
 template <class R>
@@ -98,18 +99,41 @@ hipError_t Launch<R>::check(hipStream_t s, const Problem *P, const R *U, unsigne
 }
 template <class R>
 hipError_t Launch<R>::stage1(hipStream_t s, const Problem *P, const Tiling &T, const R *U, R *Uh,
-                             StepState *st, int parity) {
+                             StepState *st, int parity, int solver) {
     using TS = TileShape<R>;
-    hipLaunchKernelGGL((k_stage1<R, TS::TX, TS::TY, TS::ZC>), dim3(8 * T.per_xcd),
-                       dim3(TS::TX * TS::TY), 0, s, P, T, U, Uh, st, parity);
+    const dim3 grid(8 * T.per_xcd), block(TS::TX * TS::TY);
+    if (solver == SOLVER_HLL)
+        hipLaunchKernelGGL((k_stage1<R, TS::TX, TS::TY, TS::ZC, SOLVER_HLL>), grid, block, 0, s, P,
+                           T, U, Uh, st, parity);
+    else
+        hipLaunchKernelGGL((k_stage1<R, TS::TX, TS::TY, TS::ZC, SOLVER_HLLC>), grid, block, 0, s,
+                           P, T, U, Uh, st, parity);
     return hipGetLastError();
 }
 template <class R>
 hipError_t Launch<R>::stage2(hipStream_t s, const Problem *P, const Tiling &T, const R *U,
-                             const R *Uh, R *Unew, StepState *st, int parity) {
+                             const R *Uh, R *Unew, StepState *st, int parity, int limiter,
+                             int solver) {
     using TS = TileShape<R>;
-    hipLaunchKernelGGL((k_stage2<R, TS::TX, TS::TY, TS::ZC>), dim3(8 * T.per_xcd),
-                       dim3(TS::TX * TS::TY), 0, s, P, T, U, Uh, Unew, st, parity);
+    const dim3 grid(8 * T.per_xcd), block(TS::TX * TS::TY);
+    // van Leer (the reference default) is compiled in; other limiters are
+    // selected at run time inside one kernel.
+    const bool vl = limiter == LIM_VANLEER;
+    if (solver == SOLVER_HLL) {
+        if (vl)
+            hipLaunchKernelGGL((k_stage2<R, TS::TX, TS::TY, TS::ZC, LIM_VANLEER, SOLVER_HLL>), grid,
+                               block, 0, s, P, T, U, Uh, Unew, st, parity);
+        else
+            hipLaunchKernelGGL((k_stage2<R, TS::TX, TS::TY, TS::ZC, 0, SOLVER_HLL>), grid, block, 0,
+                               s, P, T, U, Uh, Unew, st, parity);
+    } else {
+        if (vl)
+            hipLaunchKernelGGL((k_stage2<R, TS::TX, TS::TY, TS::ZC, LIM_VANLEER, SOLVER_HLLC>),
+                               grid, block, 0, s, P, T, U, Uh, Unew, st, parity);
+        else
+            hipLaunchKernelGGL((k_stage2<R, TS::TX, TS::TY, TS::ZC, 0, SOLVER_HLLC>), grid, block, 0,
+                               s, P, T, U, Uh, Unew, st, parity);
+    }
     return hipGetLastError();
 }
 
