@@ -149,17 +149,16 @@ template <class R> __device__ __forceinline__ R lim_avg(int opt, R a, R b) {
             if (a * b <= 0.0) return 0.0;
             return a * b * (a + b) / (a * a + b * b);
         } else {
-            // a*b (a+b)/(a^2+b^2) = r (a+b)/(1+r^2) with r = min(|a|,|b|) /
-            // max(|a|,|b|) in (0, 1]: no a*b product to underflow for cgs
-            // density differences (~1e-25), no overflow, and no reciprocal
-            // of a subnormal (returns 0 when the larger difference is
-            // subnormal, where the product form underflows to 0 as well).
-            if (!((a > 0.f && b > 0.f) || (a < 0.f && b < 0.f))) return 0.f;
-            const R fa = __builtin_fabsf(a), fb = __builtin_fabsf(b);
-            const R big = fa > fb ? fa : fb, sml = fa > fb ? fb : fa;
-            if (!(big >= 1.17549435e-38f)) return 0.f;
-            const R r = sml * __builtin_amdgcn_rcpf(big);
-            return (a + b) * r * __builtin_amdgcn_rcpf(__builtin_fmaf(r, r, 1.f));
+            // a*b (a+b)/(a^2+b^2) evaluated on a/m, b/m with m = max(|a|,|b|):
+            // no a*b product to underflow for cgs density differences (~1e-25),
+            // no overflow, and no reciprocal of a subnormal (0 is returned when
+            // m is subnormal, where the product form underflows to 0 as well).
+            const R m = fmaxf(__builtin_fabsf(a), __builtin_fabsf(b));
+            const R im = __builtin_amdgcn_rcpf(m);
+            const R as = a * im, bs = b * im;
+            const R ab = as * bs;
+            const R r = ab * (a + b) * __builtin_amdgcn_rcpf(__builtin_fmaf(as, as, bs * bs));
+            return (ab > 0.f && m >= 1.17549435e-38f) ? r : 0.f;
         }
     case LIM_MINMOD:
         s = __builtin_copysign(R(1), a);
@@ -205,36 +204,49 @@ __device__ __forceinline__ void wavespeed(R gamma, const R wl[5], const R wr[5],
 }
 
 // One HLLC face (HLLC.f90:99-135, 275-314): wl/wr reconstructed primitive
-// states, ul/ur the cell-centred conserved states of the two cells.
+// states, ul/ur the cell-centred conserved states of the two cells.  The
+// region is selected without branches (supersonic left, supersonic right,
+// star left, star right: the reference's if-chain), so a wavefront whose
+// lanes fall in different regions runs one path.  FP64 keeps the reference's
+// arithmetic; FP32 writes the star energy
+//   rs (E/rho + (S*-u)(S* + p/(rho (S-u))))
+//     = q E + rs (S*-u) S* + (S*-u) p / (S - S*),  q = (S-u)/(S-S*), rs = rho q
+// with one reciprocal.
 template <int D, class R>
 __device__ __forceinline__ void hllc_face(R gamma, R gratio, const R wl[5], const R wr[5],
                                           const R ul[5], const R ur[5], R f[5]) {
     constexpr int n = D + 1;
     R sl, sr, sst;
     wavespeed<D>(gamma, wl, wr, sl, sr, sst);
-    if (sl >= R(0)) {
-        prim2f<D>(gratio, wl, f);
-    } else if (sr <= R(0)) {
-        prim2f<D>(gratio, wr, f);
-    } else {
-        // Star region: the side is chosen by the sign of the contact speed.
-        const bool left = sst >= R(0);
-        const R *w = left ? wl : wr;
-        const R *u = left ? ul : ur;
-        const R s = left ? sl : sr;
-        R fs[5];
-        if (left) prim2f<D>(gratio, wl, fs);
-        else prim2f<D>(gratio, wr, fs);
-        const R sw = s - w[n];
-        const R rs = rdiv(w[0] * sw, s - sst);
-        R us[5];
+    const bool supl = sl >= R(0), supr = sr <= R(0);
+    const bool left = supl || (!supr && sst >= R(0));
+    const bool star = !supl && !supr;
+    R w[5], u[5];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+        w[e] = left ? wl[e] : wr[e];
+        u[e] = left ? ul[e] : ur[e];
+    }
+    const R s = left ? sl : sr;
+    R fs[5], us[5];
+    prim2f<D>(gratio, w, fs);
+    const R sw = s - w[n];
+    if constexpr (sizeof(R) == 8) {
+        const R rs = w[0] * sw / (s - sst);
         us[0] = rs;
 #pragma unroll
         for (int c = 1; c <= 3; ++c) us[c] = c == n ? rs * sst : rs * w[c];
-        us[4] = rs * (rdiv(u[4], w[0]) + (sst - w[n]) * (sst + rdiv(w[4], w[0] * sw)));
+        us[4] = rs * (u[4] / w[0] + (sst - w[n]) * (sst + w[4] / (w[0] * sw)));
+    } else {
+        const R inv = __builtin_amdgcn_rcpf(s - sst);
+        const R q = sw * inv, rs = w[0] * q, du = sst - w[n];
+        us[0] = rs;
 #pragma unroll
-        for (int e = 0; e < 5; ++e) f[e] = fs[e] + s * (us[e] - u[e]);
+        for (int c = 1; c <= 3; ++c) us[c] = c == n ? rs * sst : rs * w[c];
+        us[4] = q * u[4] + rs * du * sst + du * w[4] * inv;
     }
+#pragma unroll
+    for (int e = 0; e < 5; ++e) f[e] = star ? fs[e] + s * (us[e] - u[e]) : fs[e];
 }
 
 // One HLL face (HLL.f90:106-128, 197-223).
@@ -270,8 +282,8 @@ __device__ __forceinline__ void riemann(int solver, R gamma, R gratio, const R w
 
 // Spherical wind state at a point (winds.f90:69-83); returns false outside
 // the wind's radius.  Computed in FP64.
-__device__ __forceinline__ bool wind_state(const Problem &P, const WindDev &wd, double x, double y,
-                                           double z, double out[5]) {
+__device__ __noinline__ bool wind_state(const Problem &P, const WindDev &wd, double x, double y,
+                                        double z, double out[5]) {
     const double dxw = x - wd.xc, dyw = y - wd.yc, dzw = z - wd.zc;
     const double dist = sqrt(dxw * dxw + dyw * dyw + dzw * dzw);
     if (!(dist < wd.radius)) return false;

@@ -394,12 +394,27 @@ __global__ __launch_bounds__(TX *TY) void k_stage2(const Problem *__restrict__ P
     static_assert(6 * (TX + TY) <= NT, "one halo cell per thread");
     constexpr int WX = TX + 4, WY = TY + 4;  // two-cell halo (primitives of U^{1/2})
     constexpr int CX = TX + 2, CY = TY + 2;  // one-cell halo (conserved U^{1/2}, U^n)
-    __shared__ R sW[5][WY][WX];
+    // The primitive plane is dead once the face states are built, so the
+    // flux arrays share its storage.
+    union PlaneOrFlux {
+        R W[5][WY][WX];
+        struct {
+            R F[5][TY][TX + 1];
+            R G[5][TY + 1][TX];
+        } f;
+    };
+    __shared__ PlaneOrFlux sPF;
     __shared__ R sU[5][CY][CX];
     __shared__ R sN[5][CY][CX];
-    __shared__ R sF[5][TY][TX + 1];
-    __shared__ R sG[5][TY + 1][TX];
+    // Reconstructed face states of each cell: x faces of cells -1..TX in the
+    // tile rows, y faces of rows -1..TY in the tile columns (m: the cell's
+    // lower face, p: its upper face).
+    __shared__ R sXm[5][TY][TX + 2], sXp[5][TY][TX + 2];
+    __shared__ R sYm[5][TY + 2][TX], sYp[5][TY + 2][TX];
     __shared__ unsigned long long s_wmask;
+    auto &sW = sPF.W;
+    auto &sF = sPF.f.F;
+    auto &sG = sPF.f.G;
 
     const Problem &P = *Pp;
     const Grid &g = P.g;
@@ -430,12 +445,13 @@ __global__ __launch_bounds__(TX *TY) void k_stage2(const Problem *__restrict__ P
         s_wmask = P.nwinds ? winds_mask(P, x0 - 2, x0 + TX + 1, y0 - 2, y0 + TY + 1, z0 - 2, z1 + 2)
                            : 0ull;
 
-    // z pipeline of U^{1/2}: primitives at k-1, k, k+1 and conserved at k,
-    // k+1; U^n at k-1, k; the z flux at k-1/2.
-    R wm[5], wc[5], wp[5], uc[5], up[5], hm[5], nm[5], nc[5];
+    // z pipeline of U^{1/2}: primitives at k, k+1, conserved at k, k+1, the
+    // half slope of cell k (limiter, HLLC.f90:574-582); U^n at k-1, k; the z
+    // flux at k-1/2.
+    R wc[5], wp[5], uc[5], up[5], hz[5], hm[5], nm[5], nc[5];
     R pf_q[5], pf_n[5], pf_h[5];
     {
-        R wmm[5], um[5], t5[5];
+        R wmm[5], wm[5], um[5], t5[5], hzm[5], wl[5], wr[5];
         load_raw(P, Uh, i, j, z0 - 2, t5);
         prims(P, t5, wmm);
         load_raw(P, Uh, i, j, z0 - 1, um);
@@ -444,8 +460,13 @@ __global__ __launch_bounds__(TX *TY) void k_stage2(const Problem *__restrict__ P
         prims(P, uc, wc);
         load_raw(P, Uh, i, j, z0 + 1, up);
         prims(P, up, wp);
-        R wl[5], wr[5];
-        reconstruct<LIM>(lim, wmm, wm, wc, wp, wl, wr);
+#pragma unroll
+        for (int e = 0; e < 5; ++e) {
+            hzm[e] = R(0.5) * avg<LIM>(lim, wm[e] - wmm[e], wc[e] - wm[e]);
+            hz[e] = R(0.5) * avg<LIM>(lim, wc[e] - wm[e], wp[e] - wc[e]);
+            wl[e] = wm[e] + hzm[e];
+            wr[e] = wc[e] - hz[e];
+        }
         riemann_t<2, SOL>(gam, gr, wl, wr, um, uc, hm);
         load_raw(P, U, i, j, z0 - 1, nm);
         load_raw(P, U, i, j, z0, nc);
@@ -470,9 +491,15 @@ __global__ __launch_bounds__(TX *TY) void k_stage2(const Problem *__restrict__ P
             if (ht.kind) load_raw(P, hsrc, x0 + ht.lx, y0 + ht.ly, k + 1, pf_h);
         }
         prims(P, uq, wq);
+        R hzp[5];
         {
             R wl[5], wr[5];
-            reconstruct<LIM>(lim, wm, wc, wp, wq, wl, wr);
+#pragma unroll
+            for (int e = 0; e < 5; ++e) {
+                hzp[e] = R(0.5) * avg<LIM>(lim, wp[e] - wc[e], wq[e] - wp[e]);
+                wl[e] = wc[e] + hz[e];
+                wr[e] = wp[e] - hzp[e];
+            }
             riemann_t<2, SOL>(gam, gr, wl, wr, uc, up, hp);
         }
         lds_barrier();
@@ -496,41 +523,69 @@ __global__ __launch_bounds__(TX *TY) void k_stage2(const Problem *__restrict__ P
             for (int e = 0; e < 5; ++e) sN[e][ht.ly + 1][ht.lx + 1] = hr[e];
         }
         lds_barrier();
-        for (int q = tid; q < (TX + 1) * TY + TX * (TY + 1); q += NT) {
-            R a[5], b[5], c[5], d[5], ul[5], ur[5], wl[5], wr[5], f[5];
-            if (q < (TX + 1) * TY) {  // x face left of tile cell fx in row fy
-                const int fy = q / (TX + 1), fx = q % (TX + 1);
+        // Face states: one limited slope per cell and axis (limiter,
+        // HLLC.f90:574-582: the slope of cell i is average(P_i - P_{i-1},
+        // P_{i+1} - P_i) for both of its faces).
+        auto xstate = [&](int lx, int ly) {
 #pragma unroll
-                for (int e = 0; e < 5; ++e) {
-                    a[e] = sW[e][fy + 2][fx];
-                    b[e] = sW[e][fy + 2][fx + 1];
-                    c[e] = sW[e][fy + 2][fx + 2];
-                    d[e] = sW[e][fy + 2][fx + 3];
-                    ul[e] = sU[e][fy + 1][fx];
-                    ur[e] = sU[e][fy + 1][fx + 1];
-                }
-                reconstruct<LIM>(lim, a, b, c, d, wl, wr);
-                riemann_t<0, SOL>(gam, gr, wl, wr, ul, ur, f);
-#pragma unroll
-                for (int e = 0; e < 5; ++e) sF[e][fy][fx] = f[e];
-            } else {  // y face below tile row fy in column fx
-                const int r = q - (TX + 1) * TY;
-                const int fy = r / TX, fx = r % TX;
-#pragma unroll
-                for (int e = 0; e < 5; ++e) {
-                    a[e] = sW[e][fy][fx + 2];
-                    b[e] = sW[e][fy + 1][fx + 2];
-                    c[e] = sW[e][fy + 2][fx + 2];
-                    d[e] = sW[e][fy + 3][fx + 2];
-                    ul[e] = sU[e][fy][fx + 1];
-                    ur[e] = sU[e][fy + 1][fx + 1];
-                }
-                reconstruct<LIM>(lim, a, b, c, d, wl, wr);
-                riemann_t<1, SOL>(gam, gr, wl, wr, ul, ur, f);
-#pragma unroll
-                for (int e = 0; e < 5; ++e) sG[e][fy][fx] = f[e];
+            for (int e = 0; e < 5; ++e) {
+                const R a = sW[e][ly + 2][lx + 1], b = sW[e][ly + 2][lx + 2], c = sW[e][ly + 2][lx + 3];
+                const R h = R(0.5) * avg<LIM>(lim, b - a, c - b);
+                sXm[e][ly][lx + 1] = b - h;
+                sXp[e][ly][lx + 1] = b + h;
             }
+        };
+        auto ystate = [&](int lx, int ly) {
+#pragma unroll
+            for (int e = 0; e < 5; ++e) {
+                const R a = sW[e][ly + 1][lx + 2], b = sW[e][ly + 2][lx + 2], c = sW[e][ly + 3][lx + 2];
+                const R h = R(0.5) * avg<LIM>(lim, b - a, c - b);
+                sYm[e][ly + 1][lx] = b - h;
+                sYp[e][ly + 1][lx] = b + h;
+            }
+        };
+        xstate(tx, ty);
+        ystate(tx, ty);
+        if (tid < 2 * TX) ystate(tid % TX, tid < TX ? -1 : TY);
+        else if (tid < 2 * TX + 2 * TY) {
+            const int h = tid - 2 * TX;
+            xstate((h & 1) ? TX : -1, h >> 1);
         }
+        lds_barrier();
+        // Faces: the lower x and y face of every tile cell, then the upper y
+        // faces of the last row and the upper x faces of the last column.
+        auto xface = [&](int fx, int fy) {  // between tile cells fx-1 and fx
+            R wl[5], wr[5], ul[5], ur[5], f[5];
+#pragma unroll
+            for (int e = 0; e < 5; ++e) {
+                wl[e] = sXp[e][fy][fx];
+                wr[e] = sXm[e][fy][fx + 1];
+                ul[e] = sU[e][fy + 1][fx];
+                ur[e] = sU[e][fy + 1][fx + 1];
+            }
+            riemann_t<0, SOL>(gam, gr, wl, wr, ul, ur, f);
+#pragma unroll
+            for (int e = 0; e < 5; ++e) sF[e][fy][fx] = f[e];
+        };
+        auto yface = [&](int fx, int fy) {  // between tile rows fy-1 and fy
+            R wl[5], wr[5], ul[5], ur[5], f[5];
+#pragma unroll
+            for (int e = 0; e < 5; ++e) {
+                wl[e] = sYp[e][fy][fx];
+                wr[e] = sYm[e][fy + 1][fx];
+                ul[e] = sU[e][fy][fx + 1];
+                ur[e] = sU[e][fy + 1][fx + 1];
+            }
+            riemann_t<1, SOL>(gam, gr, wl, wr, ul, ur, f);
+#pragma unroll
+            for (int e = 0; e < 5; ++e) sG[e][fy][fx] = f[e];
+        };
+        // sW is read above and sF/sG overwrite it below: all face states
+        // must be built before any face is written.
+        xface(tx, ty);
+        yface(tx, ty);
+        if (tid < TX) yface(tid, TY);
+        else if (tid < TX + TY) xface(TX, tid - TX);
         lds_barrier();
         if (active) {
             double v[5];
@@ -605,7 +660,7 @@ __global__ __launch_bounds__(TX *TY) void k_stage2(const Problem *__restrict__ P
 #pragma unroll
         for (int e = 0; e < 5; ++e) {
             hm[e] = hp[e];
-            wm[e] = wc[e];
+            hz[e] = hzp[e];
             wc[e] = wp[e];
             wp[e] = wq[e];
             uc[e] = up[e];
