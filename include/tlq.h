@@ -60,6 +60,10 @@ extern "C" {
 #define TLQ_COOL_OFF 0
 #define TLQ_COOL_EXACT 1
 #define TLQ_COOL_SURROGATE 2
+/* The reference formula integrated to convergence: substeps of at most
+ * 0.01 t_cool with Lambda re-read every substep (the surrogate's target and
+ * fallback; one substep is exactly cooling.f90:67-112). */
+#define TLQ_COOL_SUBCYCLED 3
 
 /* Device arrays of the reference's globals (source/globals.f90:5-11). */
 #define TLQ_BUF_U 0
@@ -141,6 +145,24 @@ int tlq_check_state(tlq_ctx *ctx, long long *bad);
  * last surrogate-cooling step. */
 int tlq_fallback_fraction(tlq_ctx *ctx, double *frac);
 
+/* Learned cooling surrogate (MLP 3 -> 64 -> 64 -> 1, GELU, bf16 weights,
+ * FP32 accumulation; TLQ_MLP_NPARAMS parameters laid out as W1[64][3], b1[64],
+ * W2[64][64], b2[64], W3[64], b3).  No counterpart in the reference, which
+ * has only the exact formula; the training targets come from that formula
+ * (TLQ_COOL_SUBCYCLED).  Inputs per sample: log10 n [cm^-3], log10 T [K],
+ * log10(dt / t_cool); output log10(T_new / T). */
+#define TLQ_MLP_NPARAMS 4481
+int tlq_surrogate_init(tlq_ctx *ctx, unsigned long long seed);
+int tlq_surrogate_set_params(tlq_ctx *ctx, const float *theta, int n);
+int tlq_surrogate_get_params(tlq_ctx *ctx, float *theta, int n);
+int tlq_surrogate_generate(tlq_ctx *ctx, long long nsamples, unsigned long long seed);
+int tlq_surrogate_samples(tlq_ctx *ctx, long long first, long long n, float *x, float *y);
+int tlq_surrogate_train(tlq_ctx *ctx, int steps, int batch, double lr_max, double lr_min,
+                        double *final_loss);
+int tlq_surrogate_eval(tlq_ctx *ctx, const float *x, long long n, float *out);
+int tlq_surrogate_grad(tlq_ctx *ctx, const float *x, const float *y, int n, double *grad,
+                       double *loss);
+
 /* Timing: HIP events on the context's stream.  tlq_timer_* brackets any
  * enqueued work; with kernel timing on, every kernel launch of the fused step
  * is bracketed and its durations accumulate per kernel (names as in
@@ -150,6 +172,11 @@ int tlq_timer_stop(tlq_ctx *ctx, double *ms);
 int tlq_kernel_timing(tlq_ctx *ctx, int enable);
 int tlq_kernel_stats(tlq_ctx *ctx, int index, char *name, int name_len, double *total_ms,
                      long long *launches);
+
+/* Streaming-copy probe on the context's stream: copies `bytes` bytes between
+ * two device buffers `iters` times and returns the mean time of one copy
+ * (calibrates HBM counters and bandwidth; no counterpart in the reference). */
+int tlq_copy_probe(tlq_ctx *ctx, long long bytes, int iters, double *ms_per_copy);
 
 /* Sizes: bytes of one device field and the number of cells per field. */
 int tlq_layout(tlq_ctx *ctx, long long *cells_per_field, int *pitch_x, int *origin_x);

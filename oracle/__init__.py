@@ -92,6 +92,14 @@ def lib():
         L.orc_hll_face.argtypes = [pp, dp, dp, dp, dp, ctypes.c_int, dp]
         L.orc_advance.argtypes = [pp, dp, dp, dp, dp, dp, dp, dp, dp]
         L.orc_advance.restype = ctypes.c_double
+        L.orc_cool_subcycled.argtypes = [pp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_int)]
+        L.orc_cool_subcycled.restype = ctypes.c_double
+        L.orc_mlp_nparams.restype = ctypes.c_int
+        L.orc_mlp.argtypes = [ctypes.POINTER(ctypes.c_float), dp, dp, ctypes.c_int, dp, dp]
+        L.orc_mlp.restype = ctypes.c_double
+        L.orc_mlp_set_quant.argtypes = [ctypes.c_int]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         _lib = L
@@ -218,6 +226,14 @@ class Oracle:
         lib().orc_gravity_source(self.P, _ptr(np.ascontiguousarray(prim)), _ptr(s))
         return s
 
+    def cool_subcycled(self, n: float, T: float, dt: float, eps: float = 0.01,
+                       max_sub: int = 100000):
+        """Cooling integrated to convergence; returns (T_new, substeps)."""
+        ns = ctypes.c_int()
+        T1 = lib().orc_cool_subcycled(self.P, float(n), float(T), float(dt), float(eps),
+                                      int(max_sub), ctypes.byref(ns))
+        return T1, ns.value
+
     def hllc_face(self, wl, wr, ul, ur, dim, solver=None):
         f = np.full(5, np.nan)
         args = [np.ascontiguousarray(a, dtype=np.float64) for a in (wl, wr, ul, ur)]
@@ -233,3 +249,22 @@ def limiter_avg(lim: int, a: np.ndarray, b: np.ndarray) -> np.ndarray:
     out = np.empty_like(a)
     lib().orc_limiter_avg(int(lim), _ptr(a), _ptr(b), _ptr(out), a.size)
     return out
+
+
+def mlp(theta: np.ndarray, x: np.ndarray, y=None, grad: bool = False, quant: bool = True):
+    """The cooling surrogate network in FP64 (oracle of the GPU kernels).
+
+    theta: float32 parameters; x: (B, 3) inputs (log10 n, log10 T,
+    log10 dt/t_cool); y: targets.  Returns (out, loss, grad)."""
+    L = lib()
+    L.orc_mlp_set_quant(1 if quant else 0)
+    th = np.ascontiguousarray(theta, dtype=np.float32)
+    xx = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+    B = xx.shape[0]
+    out = np.empty(B)
+    yy = None if y is None else np.ascontiguousarray(y, dtype=np.float64)
+    g = np.empty(L.orc_mlp_nparams()) if grad else None
+    loss = L.orc_mlp(th.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), _ptr(xx), _ptr(yy), B,
+                     _ptr(out), _ptr(g))
+    L.orc_mlp_set_quant(1)
+    return out, loss, g

@@ -535,3 +535,135 @@ double orc_advance(const orc_params *p, double *u, double *up, double *upp, doub
     if (p->cooling) orc_cooling(p, prim, u, dt);         /* :96-98 */
     return dt;
 }
+
+/* ------------------------------------------------------------------------
+ * Cooling integrated to convergence: apply_cooling's per-cell formula
+ * (cooling.f90:67-112) applied in substeps h <= eps * t_cool(T), with
+ * Lambda re-read from the table at the current temperature every substep,
+ * until dt is used up or T drops below the table's first temperature (the
+ * reference's gate).  With eps >= dt/t_cool it is one substep, i.e. exactly
+ * apply_cooling.  Works on (n, T): p = n KB T, E_th = p/(gamma-1).
+ * Returns the new temperature; *nsub receives the substep count. */
+double orc_cool_subcycled(const orc_params *p, double n, double T, double dt, double eps,
+                          int max_sub, int *nsub) {
+    const double Tmin = p->table_T[0];
+    double left = dt;
+    int s = 0;
+    while (left > 0.0 && T >= Tmin && s < max_sub) {
+        const double lam = orc_find_coef(p, T);
+        const double eth = n * ORC_KB * T / (p->gamma - 1.0);
+        const double ce = lam * (n * n) / eth;
+        double h = eps / ce;
+        if (h > left) h = left;
+        double f = exp(-ce * h);
+        if (f < 1.0 - p->cooling_limit) f = 1.0 - p->cooling_limit;
+        if (T * f < 10.0) f = 10.0 / T;
+        T = T * f;
+        left -= h;
+        ++s;
+    }
+    if (nsub) *nsub = s;
+    return T;
+}
+
+/* ------------------------------------------------------------------------
+ * The cooling surrogate: an MLP 3 -> 64 -> 64 -> 1 with GELU (tanh form),
+ * inputs (log10 n, log10 T, log10(dt/t_cool)) normalised to [-1, 1] over the
+ * training box, output r = log10(T_new / T).  Parameters, flat float array:
+ *   W1[64][3] | b1[64] | W2[64][64] (row = output) | b2[64] | W3[64] | b3
+ * The GPU kernels round W1..b3 to bf16 for the forward pass, the layer-1
+ * activations and the layer-2 pre-activation gradients to bf16 as MFMA
+ * operands, and accumulate in FP32.  This FP64 restatement applies the same
+ * bf16 roundings at the same points, so its gradients match the GPU's to
+ * FP32 accumulation error. */
+#define MLP_H 64
+#define MLP_NP (MLP_H * 3 + MLP_H + MLP_H * MLP_H + MLP_H + MLP_H + 1)
+static const double MLP_C[3] = {0.5, 5.25, -2.0}, MLP_S[3] = {3.5, 3.25, 2.0};
+
+int orc_mlp_nparams(void) { return MLP_NP; }
+
+static double bf16r(double x) {
+    float f = (float)x;
+    unsigned u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) != 0x7f800000u) u = (u + 0x7fffu + ((u >> 16) & 1u)) & 0xffff0000u;
+    memcpy(&f, &u, 4);
+    return (double)f;
+}
+
+static double gelu(double x) {
+    const double u = 2.0 * 0.7978845608028654 * (x + 0.044715 * x * x * x);
+    return x / (1.0 + exp(-u));
+}
+
+static double gelu_d(double x) {
+    const double u = 2.0 * 0.7978845608028654 * (x + 0.044715 * x * x * x);
+    const double s = 1.0 / (1.0 + exp(-u));
+    const double du = 2.0 * 0.7978845608028654 * (1.0 + 3.0 * 0.044715 * x * x);
+    return s + x * s * (1.0 - s) * du;
+}
+
+/* Forward (out, B values) and, when y != NULL, the mean-squared-error loss
+ * and its gradient with respect to the parameters (grad, MLP_NP values). */
+static int g_quant = 1;
+void orc_mlp_set_quant(int q) { g_quant = q; }
+#define bf16r(v) (g_quant ? bf16r(v) : (double)(v))
+
+double orc_mlp(const float *theta, const double *x, const double *y, int B, double *out,
+               double *grad) {
+    const float *W1 = theta, *b1 = W1 + 3 * MLP_H, *W2 = b1 + MLP_H, *b2 = W2 + MLP_H * MLP_H,
+                *W3 = b2 + MLP_H, *b3 = W3 + MLP_H;
+    double *gW1 = 0, *gb1 = 0, *gW2 = 0, *gb2 = 0, *gW3 = 0, *gb3 = 0;
+    if (grad) {
+        memset(grad, 0, sizeof(double) * MLP_NP);
+        gW1 = grad;
+        gb1 = gW1 + 3 * MLP_H;
+        gW2 = gb1 + MLP_H;
+        gb2 = gW2 + MLP_H * MLP_H;
+        gW3 = gb2 + MLP_H;
+        gb3 = gW3 + MLP_H;
+    }
+    double loss = 0.0;
+    for (int b = 0; b < B; ++b) {
+        double xn[3], z1[MLP_H], h1q[MLP_H], z2[MLP_H], h2[MLP_H];
+        for (int c = 0; c < 3; ++c) xn[c] = (x[3 * b + c] - MLP_C[c]) / MLP_S[c];
+        for (int i = 0; i < MLP_H; ++i) {
+            double z = bf16r(b1[i]);
+            for (int c = 0; c < 3; ++c) z += bf16r(W1[3 * i + c]) * xn[c];
+            z1[i] = z;
+            h1q[i] = bf16r(gelu(z));
+        }
+        double o = bf16r(b3[0]);
+        for (int j = 0; j < MLP_H; ++j) {
+            double z = 0.0;
+            for (int i = 0; i < MLP_H; ++i) z += bf16r(W2[MLP_H * j + i]) * h1q[i];
+            z2[j] = z + bf16r(b2[j]);
+            h2[j] = gelu(z2[j]);
+            o += bf16r(W3[j]) * h2[j];
+        }
+        if (out) out[b] = o;
+        if (!y) continue;
+        const double d = o - y[b];
+        loss += d * d / B;
+        if (!grad) continue;
+        const double dout = 2.0 * d / B;
+        gb3[0] += dout;
+        double dz2q[MLP_H];
+        for (int j = 0; j < MLP_H; ++j) {
+            gW3[j] += dout * h2[j];
+            const double dz2 = dout * bf16r(W3[j]) * gelu_d(z2[j]);
+            dz2q[j] = bf16r(dz2);
+            gb2[j] += dz2q[j];
+            for (int i = 0; i < MLP_H; ++i) gW2[MLP_H * j + i] += dz2q[j] * h1q[i];
+        }
+        for (int i = 0; i < MLP_H; ++i) {
+            double dh1 = 0.0;
+            for (int j = 0; j < MLP_H; ++j) dh1 += bf16r(W2[MLP_H * j + i]) * dz2q[j];
+            const double dz1 = dh1 * gelu_d(z1[i]);
+            gb1[i] += dz1;
+            for (int c = 0; c < 3; ++c) gW1[3 * i + c] += dz1 * xn[c];
+        }
+    }
+    return loss;
+}
+#undef bf16r

@@ -78,6 +78,16 @@ void orc_hll_face(const orc_params *p, const double *pl, const double *pr,
 double orc_advance(const orc_params *p, double *u, double *up, double *upp, double *prim,
                    double *ff, double *gg, double *hh, double *src);
 
+/* Cooling integrated to convergence (the reference formula sub-cycled). */
+double orc_cool_subcycled(const orc_params *p, double n, double T, double dt, double eps,
+                          int max_sub, int *nsub);
+/* Cooling surrogate MLP: parameter count, forward, loss and gradient. */
+int orc_mlp_nparams(void);
+double orc_mlp(const float *theta, const double *x, const double *y, int B, double *out,
+               double *grad);
+/* 1 (default): bf16 roundings as the GPU kernels; 0: plain FP64 network. */
+void orc_mlp_set_quant(int q);
+
 /* OpenMP thread count for the oracle loops (1 = serial). */
 void orc_set_threads(int n);
 int orc_get_threads(void);

@@ -1,0 +1,123 @@
+"""CPU checks of the oracle's cooling routines and of the surrogate network oracle.
+
+What pins what:
+  * find_coef reproduces Z1.0.dat at its nodes; apply_cooling is pinned by the
+    reference fixtures (test_oracle_ref.py: cooling, winds_cooling).
+  * The converged integrator with one substep is apply_cooling (compared with
+    the fixture-pinned orc_cooling), and as eps -> 0 it converges (first
+    order) to the exact solution of dT/dt = -(gamma-1) n Lambda(T)/k_B, whose
+    time integral t = k_B/((gamma-1) n) int_{T1}^{T0} dT/Lambda(T) is
+    evaluated independently by quadrature.
+  * The network oracle's backpropagation equals central finite differences of
+    its forward pass (FP64, without bf16 rounding).
+"""
+import numpy as np
+import pytest
+from scipy import integrate, optimize
+
+import oracle
+from helpers import cooling_table
+from tlaloque_amd import params as P
+
+
+def _cool_problem():
+    p = P.HydroParams(nx=4, ny=4, nz=4, cooling=P.COOL_EXACT)
+    return p, oracle.Oracle(p, table=cooling_table())
+
+
+def test_find_coef_reproduces_table_nodes():
+    _, o = _cool_problem()
+    T, L = cooling_table()
+    for q in range(0, len(T) - 1, 7):
+        assert o.find_coef(T[q]) == pytest.approx(10.0 ** L[q], rel=1e-12)
+
+
+def test_apply_cooling_bounds():
+    p, o = _cool_problem()
+    rng = np.random.default_rng(5)
+    shape = p.shape + (5,)
+    n = 10 ** rng.uniform(-2, 3, p.shape)
+    T = 10 ** rng.uniform(3, 8.9, p.shape)
+    prim = np.zeros(shape)
+    prim[..., 0] = n * P.AMU
+    prim[..., 1:4] = rng.normal(0, 1e6, p.shape + (3,))
+    prim[..., 4] = n * P.KB * T
+    u = np.zeros(shape)
+    u[..., 0] = prim[..., 0]
+    u[..., 1:4] = prim[..., :1] * prim[..., 1:4]
+    u[..., 4] = 0.5 * prim[..., 0] * (prim[..., 1:4] ** 2).sum(-1) + prim[..., 4] / (p.gamma - 1)
+    before = prim.copy()
+    o.cooling(prim, u, 3e12)
+    sl = (slice(2, -2),) * 3
+    T0 = (before[..., 4] / (n * P.KB))[sl]
+    T1 = (prim[..., 4] / (n * P.KB))[sl]
+    cold = T0 < 1e4
+    assert np.array_equal(T1[cold], T0[cold])                       # gate (cooling.f90:71)
+    assert np.all(T1[~cold] <= T0[~cold] * (1 + 1e-14))
+    assert np.all(T1[~cold] >= np.maximum(0.5 * T0[~cold], 10.0) * (1 - 1e-14))
+    assert (T1[~cold] < 0.99 * T0[~cold]).sum() > 10                 # it does cool
+
+
+def test_subcycled_with_one_substep_is_apply_cooling():
+    p, o = _cool_problem()
+    rng = np.random.default_rng(6)
+    for _ in range(200):
+        n = 10 ** rng.uniform(-3, 4)
+        T = 10 ** rng.uniform(4, 8.9)
+        dt = 10 ** rng.uniform(8, 14)
+        T1, ns = o.cool_subcycled(n, T, dt, eps=1e300)
+        assert ns == 1
+        prim = np.zeros(p.shape + (5,))
+        prim[..., 0] = n * p.mu0 * P.AMU
+        prim[..., 4] = n * P.KB * T
+        u = np.zeros_like(prim)
+        u[..., 0] = prim[..., 0]
+        u[..., 4] = prim[..., 4] / (p.gamma - 1)
+        o.cooling(prim, u, dt)
+        Tref = prim[3, 3, 3, 4] / (n * P.KB)
+        assert T1 == pytest.approx(Tref, rel=1e-13)
+
+
+def test_subcycled_converges_to_exact_cooling_integral():
+    p, o = _cool_problem()
+    lam = lambda T: o.find_coef(T)  # noqa: E731
+    g1 = p.gamma - 1.0
+    for n, T0, frac in [(1.0, 3e6, 0.5), (50.0, 2e5, 0.3), (0.1, 1e8, 0.8), (5.0, 3e4, 0.6)]:
+        # time to cool from T0 to T0*frac, by quadrature of 1/Lambda
+        Tt = T0 * frac
+        grid = np.geomspace(Tt, T0, 4001)
+        t_exact = P.KB / (g1 * n) * integrate.simpson(1.0 / np.array([lam(T) for T in grid]), x=grid)
+        errs = []
+        for eps in (0.02, 0.002):
+            T1, ns = o.cool_subcycled(n, T0, t_exact, eps=eps)
+            errs.append(abs(T1 / Tt - 1.0))
+            assert ns > 1
+        assert errs[0] < 2e-2 and errs[1] < 2e-3
+        assert errs[1] < 0.3 * errs[0]                                # first-order convergence
+
+
+def test_network_oracle_backprop_matches_finite_differences():
+    rng = np.random.default_rng(0)
+    th = (rng.standard_normal(4481) * 0.2).astype(np.float32)
+    x = np.stack([rng.uniform(-3, 4, 32), rng.uniform(2, 8.5, 32), rng.uniform(-4, 0, 32)], 1)
+    y = rng.uniform(-1, 0, 32)
+    _, _, g = oracle.mlp(th, x, y, grad=True, quant=False)
+    for idx in rng.choice(4481, 40, replace=False).tolist() + [0, 192, 256, 4352, 4416, 4480]:
+        h = max(1e-2 * abs(float(th[idx])), 1e-3)
+        tp, tm = th.copy(), th.copy()
+        tp[idx] += np.float32(h)
+        tm[idx] -= np.float32(h)
+        lp = oracle.mlp(tp, x, y, quant=False)[1]
+        lm = oracle.mlp(tm, x, y, quant=False)[1]
+        fd = (lp - lm) / (float(tp[idx]) - float(tm[idx]))
+        assert g[idx] == pytest.approx(fd, rel=2e-4, abs=1e-7), idx
+
+
+def test_network_oracle_quantised_forward_close_to_fp64():
+    rng = np.random.default_rng(1)
+    th = (rng.standard_normal(4481) * 0.2).astype(np.float32)
+    x = np.stack([rng.uniform(-3, 4, 256), rng.uniform(2, 8.5, 256), rng.uniform(-4, 0, 256)], 1)
+    a = oracle.mlp(th, x)[0]
+    b = oracle.mlp(th, x, quant=False)[0]
+    assert np.abs(a - b).max() <= 0.05 * np.abs(b).max()
+    assert np.abs(a - b).max() > 0.0                                 # the rounding is applied

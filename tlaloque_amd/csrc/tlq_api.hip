@@ -15,6 +15,7 @@
 #include "../../include/tlq.h"
 #include "tlq_fused.hpp"
 #include "tlq_launch.hpp"
+#include "tlq_surrogate.hpp"
 
 using namespace tlq;
 
@@ -69,6 +70,14 @@ struct tlq_ctx {
     double ktotal[kNumKernels] = {};
     long long kcount[kNumKernels] = {};
     double fallback = 0.0;
+    // learned cooling surrogate
+    float *sur_theta = nullptr, *sur_m = nullptr, *sur_v = nullptr, *sur_gout = nullptr;
+    unsigned short *sur_thq = nullptr;
+    float *sur_X = nullptr, *sur_Y = nullptr, *sur_partial = nullptr, *sur_loss = nullptr;
+    long long sur_n = 0;
+    int sur_t = 0;
+    bool sur_ready = false;
+    unsigned long long *cool_counts = nullptr;  // [cooled cells, fallback cells]
 };
 
 namespace {
@@ -217,6 +226,12 @@ template <class R> int do_advance(tlq_ctx *c, int nsteps) {
             HIPCHK(Launch<R>::stage2(c->stream, c->dprob, c->tiling, cp<R>(c->u[par]),
                                      cp<R>(c->uh), mp<R>(c->u[par ^ 1]), c->st, par,
                                      c->host.limiter, c->host.solver));
+        }
+        if (c->host.cooling >= TLQ_COOL_SURROGATE) {
+            KTimer kt(c, 3);
+            HIPCHK(hipMemsetAsync(c->cool_counts, 0, 2 * sizeof(unsigned long long), c->stream));
+            HIPCHK(sur_cool_pass<R>(c->stream, c->dprob, c->g, mp<R>(c->u[par ^ 1]), c->sur_thq,
+                                    c->st, par, c->cool_counts, c->host.cooling));
         }
         c->cur ^= 1;
     }
@@ -385,7 +400,7 @@ int tlq_create(const tlq_params *p, tlq_ctx **out) {
         return fail(TLQ_ERR_ARG, "bad solver id");
     if (p->precision != TLQ_FP64 && p->precision != TLQ_FP32)
         return fail(TLQ_ERR_ARG, "bad precision id");
-    if (p->cooling < TLQ_COOL_OFF || p->cooling > TLQ_COOL_SURROGATE)
+    if (p->cooling < TLQ_COOL_OFF || p->cooling > TLQ_COOL_SUBCYCLED)
         return fail(TLQ_ERR_ARG, "bad cooling id");
     if (!(p->gamma > 1.0) || !(p->cfl > 0.0) || !(p->xr > p->xl) || !(p->yr > p->yl) ||
         !(p->zr > p->zl) || !(p->mu0 > 0.0))
@@ -438,7 +453,8 @@ int tlq_create(const tlq_params *p, tlq_ctx **out) {
     if (hipMalloc(&c->dprob, sizeof(Problem)) != hipSuccess ||
         hipMalloc(&c->st, sizeof(StepState)) != hipSuccess ||
         hipMalloc(&c->scratch_max, 8 * sizeof(unsigned long long)) != hipSuccess ||
-        hipMalloc(&c->scratch_cnt, sizeof(unsigned long long)) != hipSuccess)
+        hipMalloc(&c->scratch_cnt, sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&c->cool_counts, 2 * sizeof(unsigned long long)) != hipSuccess)
         return bail(TLQ_ERR_NOMEM, "hipMalloc failed");
     const size_t bytes = (size_t)5 * g.nc * c->esize;
     for (int q = 0; q < 2; ++q) {
@@ -495,6 +511,11 @@ int tlq_destroy(tlq_ctx *c) {
     if (c->st) hipFree(c->st);
     if (c->scratch_max) hipFree(c->scratch_max);
     if (c->scratch_cnt) hipFree(c->scratch_cnt);
+    if (c->cool_counts) hipFree(c->cool_counts);
+    for (void *q : {(void *)c->sur_theta, (void *)c->sur_m, (void *)c->sur_v, (void *)c->sur_gout,
+                    (void *)c->sur_thq, (void *)c->sur_X, (void *)c->sur_Y,
+                    (void *)c->sur_partial, (void *)c->sur_loss})
+        if (q) (void)hipFree(q);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -651,6 +672,8 @@ int tlq_advance_n(tlq_ctx *c, int n) {
     if (n < 0) return fail(TLQ_ERR_ARG, "negative step count");
     if (c->host.cooling != TLQ_COOL_OFF && c->host.ntable < 2)
         return fail(TLQ_ERR_STATE, "cooling enabled but no table set");
+    if (c->host.cooling == TLQ_COOL_SURROGATE && !c->sur_ready)
+        return fail(TLQ_ERR_STATE, "surrogate cooling needs trained or loaded parameters");
     return DISPATCH(c, do_advance, n);
 }
 
@@ -687,7 +710,10 @@ int tlq_check_state(tlq_ctx *c, long long *bad) {
 int tlq_fallback_fraction(tlq_ctx *c, double *frac) {
     if (int rc = check_ctx(c)) return rc;
     if (!frac) return fail(TLQ_ERR_ARG, "null output");
-    *frac = c->fallback;
+    unsigned long long n[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(n, c->cool_counts, sizeof n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *frac = n[0] ? (double)n[1] / (double)n[0] : 0.0;
     return TLQ_OK;
 }
 
@@ -729,6 +755,191 @@ int tlq_kernel_stats(tlq_ctx *c, int index, char *name, int name_len, double *to
     }
     if (total_ms) *total_ms = c->ktotal[index];
     if (launches) *launches = c->kcount[index];
+    return TLQ_OK;
+}
+
+// ---- learned cooling surrogate -------------------------------------------
+
+static int sur_alloc(tlq_ctx *c) {
+    if (c->sur_theta) return TLQ_OK;
+    const int np = mlp::NP;
+    if (hipMalloc(&c->sur_theta, np * 4) != hipSuccess || hipMalloc(&c->sur_m, np * 4) != hipSuccess ||
+        hipMalloc(&c->sur_v, np * 4) != hipSuccess || hipMalloc(&c->sur_gout, np * 4) != hipSuccess ||
+        hipMalloc(&c->sur_thq, np * 2) != hipSuccess ||
+        hipMalloc(&c->sur_partial, (size_t)1024 * (np + 1) * 4) != hipSuccess ||
+        hipMalloc(&c->sur_loss, 4) != hipSuccess)
+        return fail(TLQ_ERR_NOMEM, "surrogate allocation failed");
+    HIPCHK(hipMemsetAsync(c->sur_m, 0, np * 4, c->stream));
+    HIPCHK(hipMemsetAsync(c->sur_v, 0, np * 4, c->stream));
+    return TLQ_OK;
+}
+
+int tlq_surrogate_set_params(tlq_ctx *c, const float *theta, int n) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!theta || n != mlp::NP) return fail(TLQ_ERR_ARG, "expected TLQ_MLP_NPARAMS parameters");
+    if (int rc = sur_alloc(c)) return rc;
+    HIPCHK(hipMemcpyAsync(c->sur_theta, theta, n * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(c->sur_m, 0, n * 4, c->stream));
+    HIPCHK(hipMemsetAsync(c->sur_v, 0, n * 4, c->stream));
+    HIPCHK(sur_quantize(c->stream, c->sur_theta, c->sur_thq));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->sur_t = 0;
+    c->sur_ready = true;
+    return TLQ_OK;
+}
+
+int tlq_surrogate_get_params(tlq_ctx *c, float *theta, int n) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!theta || n != mlp::NP) return fail(TLQ_ERR_ARG, "expected TLQ_MLP_NPARAMS parameters");
+    if (!c->sur_theta) return fail(TLQ_ERR_STATE, "surrogate not initialised");
+    HIPCHK(hipMemcpyAsync(theta, c->sur_theta, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return TLQ_OK;
+}
+
+int tlq_surrogate_init(tlq_ctx *c, unsigned long long seed) {
+    // Uniform initialisation with the fan-in scale (Glorot-style bounds), zero biases.
+    std::vector<float> th(mlp::NP, 0.f);
+    unsigned long long z = seed * 0x9E3779B97F4A7C15ull + 1;
+    auto u01 = [&]() {
+        z ^= z << 13;
+        z ^= z >> 7;
+        z ^= z << 17;
+        return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    };
+    const double a1 = std::sqrt(6.0 / (3 + 64)), a2 = std::sqrt(6.0 / (64 + 64)),
+                 a3 = std::sqrt(6.0 / (64 + 1));
+    for (int q = 0; q < 3 * 64; ++q) th[mlp::OW1 + q] = (float)(a1 * (2 * u01() - 1));
+    for (int q = 0; q < 64 * 64; ++q) th[mlp::OW2 + q] = (float)(a2 * (2 * u01() - 1));
+    for (int q = 0; q < 64; ++q) th[mlp::OW3 + q] = (float)(a3 * (2 * u01() - 1));
+    return tlq_surrogate_set_params(c, th.data(), mlp::NP);
+}
+
+int tlq_surrogate_generate(tlq_ctx *c, long long n, unsigned long long seed) {
+    if (int rc = check_ctx(c)) return rc;
+    if (n < 64) return fail(TLQ_ERR_ARG, "need at least 64 samples");
+    if (c->host.ntable < 2) return fail(TLQ_ERR_STATE, "cooling table not set");
+    if (c->sur_X) {
+        (void)hipFree(c->sur_X);
+        (void)hipFree(c->sur_Y);
+        c->sur_X = c->sur_Y = nullptr;
+    }
+    if (hipMalloc(&c->sur_X, (size_t)n * 12) != hipSuccess || hipMalloc(&c->sur_Y, (size_t)n * 4) != hipSuccess)
+        return fail(TLQ_ERR_NOMEM, "sample allocation failed");
+    HIPCHK(sur_generate(c->stream, c->dprob, n, seed, c->sur_X, c->sur_Y, 0.01, 100000));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->sur_n = n;
+    return TLQ_OK;
+}
+
+int tlq_surrogate_samples(tlq_ctx *c, long long first, long long n, float *x, float *y) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!c->sur_X || first < 0 || n < 0 || first + n > c->sur_n) return fail(TLQ_ERR_ARG, "bad range");
+    if (x) HIPCHK(hipMemcpyAsync(x, c->sur_X + 3 * first, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream));
+    if (y) HIPCHK(hipMemcpyAsync(y, c->sur_Y + first, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return TLQ_OK;
+}
+
+int tlq_surrogate_train(tlq_ctx *c, int steps, int batch, double lr_max, double lr_min,
+                        double *final_loss) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!c->sur_X) return fail(TLQ_ERR_STATE, "generate samples first");
+    if (steps < 1 || batch < 64 || batch > c->sur_n || batch > 1024 * 64)
+        return fail(TLQ_ERR_ARG, "bad steps/batch");
+    if (!c->sur_theta) {
+        int rc = tlq_surrogate_init(c, 1234);
+        if (rc) return rc;
+    }
+    const int nb = std::min(1024, (batch + 63) / 64);
+    const long long span = c->sur_n - batch;
+    for (int s = 0; s < steps; ++s) {
+        const double ph = steps > 1 ? (double)s / (steps - 1) : 1.0;
+        const float lr = (float)(lr_min + 0.5 * (lr_max - lr_min) * (1.0 + std::cos(M_PI * ph)));
+        const long long off = span > 0 ? ((long long)s * batch) % (span + 1) : 0;
+        HIPCHK(sur_grad(c->stream, c->sur_thq, c->sur_X, c->sur_Y, off, batch, c->sur_partial, nb));
+        HIPCHK(sur_adam(c->stream, c->sur_partial, nb, c->sur_theta, c->sur_thq, c->sur_m, c->sur_v,
+                        ++c->sur_t, lr, c->sur_loss));
+    }
+    float l = 0.f;
+    HIPCHK(hipMemcpyAsync(&l, c->sur_loss, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (final_loss) *final_loss = l;
+    c->sur_ready = true;
+    return TLQ_OK;
+}
+
+int tlq_surrogate_eval(tlq_ctx *c, const float *x, long long n, float *out) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!c->sur_ready) return fail(TLQ_ERR_STATE, "surrogate has no parameters");
+    if (!x || !out || n < 1) return fail(TLQ_ERR_ARG, "bad arguments");
+    float *dx = nullptr, *dy = nullptr;
+    HIPCHK(hipMalloc(&dx, (size_t)n * 12));
+    HIPCHK(hipMalloc(&dy, (size_t)n * 4));
+    HIPCHK(hipMemcpyAsync(dx, x, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(sur_eval(c->stream, c->sur_thq, dx, n, dy));
+    HIPCHK(hipMemcpyAsync(out, dy, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+    return TLQ_OK;
+}
+
+int tlq_surrogate_grad(tlq_ctx *c, const float *x, const float *y, int n, double *grad,
+                       double *loss) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!c->sur_ready) return fail(TLQ_ERR_STATE, "surrogate has no parameters");
+    if (!x || !y || !grad || n < 1 || n > 1024 * 64) return fail(TLQ_ERR_ARG, "bad arguments");
+    float *dx = nullptr, *dy = nullptr;
+    HIPCHK(hipMalloc(&dx, (size_t)n * 12));
+    HIPCHK(hipMalloc(&dy, (size_t)n * 4));
+    HIPCHK(hipMemcpyAsync(dx, x, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(dy, y, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    const int nb = std::min(1024, (n + 63) / 64);
+    HIPCHK(sur_grad(c->stream, c->sur_thq, dx, dy, 0, n, c->sur_partial, nb));
+    HIPCHK(sur_adam(c->stream, c->sur_partial, nb, nullptr, nullptr, c->sur_gout, nullptr, 0, 0.f,
+                    c->sur_loss));
+    std::vector<float> g(mlp::NP);
+    float l = 0.f;
+    HIPCHK(hipMemcpyAsync(g.data(), c->sur_gout, mlp::NP * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&l, c->sur_loss, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int q = 0; q < mlp::NP; ++q) grad[q] = g[q];
+    if (loss) *loss = l;
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+    return TLQ_OK;
+}
+
+__global__ void k_stream_copy(const float4 *__restrict__ a, float4 *__restrict__ b, long long n) {
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x)
+        b[t] = a[t];
+}
+
+int tlq_copy_probe(tlq_ctx *c, long long bytes, int iters, double *ms) {
+    if (int rc = check_ctx(c)) return rc;
+    if (bytes < 16 || iters < 1 || !ms) return fail(TLQ_ERR_ARG, "bad probe arguments");
+    const long long n = bytes / 16;
+    float4 *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, n * 16) != hipSuccess) return fail(TLQ_ERR_NOMEM, "probe alloc");
+    if (hipMalloc(&b, n * 16) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(TLQ_ERR_NOMEM, "probe alloc");
+    }
+    HIPCHK(hipMemsetAsync(a, 0, n * 16, c->stream));
+    const unsigned nb = 256 * 8 * 4;
+    hipLaunchKernelGGL(k_stream_copy, dim3(nb), dim3(256), 0, c->stream, a, b, n);  // warm-up
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    for (int q = 0; q < iters; ++q)
+        hipLaunchKernelGGL(k_stream_copy, dim3(nb), dim3(256), 0, c->stream, a, b, n);
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, c->ev0, c->ev1));
+    *ms = f / iters;
+    HIPCHK(hipFree(a));
+    HIPCHK(hipFree(b));
     return TLQ_OK;
 }
 

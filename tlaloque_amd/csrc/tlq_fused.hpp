@@ -433,7 +433,10 @@ __global__ __launch_bounds__(TX *TY) void k_stage2(const Problem *__restrict__ P
     const R gam = (R)P.gamma, gr = (R)(P.gamma / (P.gamma - 1.0));
     const int lim = P.limiter;
     const bool grav = P.nmasses > 0;
-    const bool cool = P.cooling != 0 && P.ntable > 1;
+    // Cooling modes 2 and 3 run in a pass of their own after this kernel,
+    // which then also forms the CFL maxima.
+    const bool cool = P.cooling == 1 && P.ntable > 1;
+    const bool maxima_here = P.cooling < 2;
     const double g1 = P.gamma - 1.0;
     const double eta = P.eta;
     if (blockIdx.x == 0 && tid == 0) {
@@ -669,8 +672,10 @@ __global__ __launch_bounds__(TX *TY) void k_stage2(const Problem *__restrict__ P
             nc[e] = np[e];
         }
     }
-    if constexpr (sizeof(R) == 8) block_max3<NT>(mxd, myd, mzd, st->maxima[parity ^ 1]);
-    else block_max3<NT>((double)mx, (double)my, (double)mz, st->maxima[parity ^ 1]);
+    if (maxima_here) {
+        if constexpr (sizeof(R) == 8) block_max3<NT>(mxd, myd, mzd, st->maxima[parity ^ 1]);
+        else block_max3<NT>((double)mx, (double)my, (double)mz, st->maxima[parity ^ 1]);
+    }
 }
 
 }  // namespace tlq

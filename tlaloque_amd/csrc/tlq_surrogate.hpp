@@ -1,0 +1,51 @@
+// tlq_surrogate.hpp -- the learned radiative-cooling surrogate.
+//
+// The network maps (log10 n, log10 T, log10(dt/t_cool)) to r = log10(T_new/T)
+// for one cooling step: an MLP 3 -> 64 -> 64 -> 1 with GELU (tanh form),
+// bf16 weights and FP32 accumulation.  Its training targets come from the
+// reference's cooling formula (apply_cooling, cooling.f90:67-112) integrated
+// to convergence: the formula is applied in substeps of at most eps t_cool
+// with Lambda re-read from the table every substep (one substep is exactly
+// apply_cooling).  Layer 2 -- 4096 of the network's 4352 multiply-adds --
+// runs on the MFMA units (v_mfma_f32_16x16x16_bf16), in training (forward
+// and both backward products) and in inference.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "tlq_device.hpp"
+
+namespace tlq {
+namespace mlp {
+
+constexpr int H = 64;
+constexpr int OW1 = 0, OB1 = 3 * H, OW2 = OB1 + H, OB2 = OW2 + H * H, OW3 = OB2 + H,
+              OB3 = OW3 + H, NP = OB3 + 1;  // 4481 parameters
+// input normalisation: x_n = (x - C) / S maps the training box onto [-1, 1]
+constexpr float C0 = 0.5f, C1 = 5.25f, C2 = -2.0f, S0 = 3.5f, S1 = 3.25f, S2 = 2.0f;
+constexpr float X0LO = -3.f, X0HI = 4.f, X1LO = 2.f, X1HI = 8.5f, X2LO = -4.f, X2HI = 0.f;
+
+}  // namespace mlp
+
+// Launchers (tlq_surrogate.hip).
+hipError_t sur_generate(hipStream_t s, const Problem *P, long long n, unsigned long long seed,
+                        float *X, float *Y, double eps, int max_sub);
+hipError_t sur_grad(hipStream_t s, const unsigned short *thq, const float *X, const float *Y,
+                    long long off, int B, float *partial, int nblocks);
+hipError_t sur_adam(hipStream_t s, const float *partial, int nblocks, float *theta,
+                    unsigned short *thq, float *m, float *v, int t, float lr, float *loss_out);
+hipError_t sur_quantize(hipStream_t s, const float *theta, unsigned short *thq);
+hipError_t sur_eval(hipStream_t s, const unsigned short *thq, const float *X, long long n,
+                    float *out);
+
+// Cooling pass over the interior after the fused corrector: mode 2 = surrogate
+// with per-cell fallback, mode 3 = the converged exact integrator.  Also
+// produces the next step's CFL maxima.  counts[0] += cells at T >= T_min,
+// counts[1] += cells that took the exact fallback.
+struct StepState;
+template <class R>
+hipError_t sur_cool_pass(hipStream_t s, const Problem *P, const Grid &g, R *U,
+                         const unsigned short *thq, StepState *st, int parity,
+                         unsigned long long *counts, int mode);
+
+}  // namespace tlq

@@ -44,8 +44,11 @@ EXPORTS = (
     "tlq_fluxes", "tlq_godunov", "tlq_borders", "tlq_step", "tlq_cooling", "tlq_init_state",
     "tlq_advance", "tlq_advance_n", "tlq_get_time", "tlq_sync", "tlq_check_state",
     "tlq_fallback_fraction", "tlq_timer_start", "tlq_timer_stop", "tlq_kernel_timing",
-    "tlq_kernel_stats", "tlq_layout",
+    "tlq_kernel_stats", "tlq_layout", "tlq_copy_probe", "tlq_surrogate_init",
+    "tlq_surrogate_set_params", "tlq_surrogate_get_params", "tlq_surrogate_generate",
+    "tlq_surrogate_samples", "tlq_surrogate_train", "tlq_surrogate_eval", "tlq_surrogate_grad",
 )
+MLP_NPARAMS = 4481
 
 
 class TlqError(RuntimeError):
@@ -84,6 +87,7 @@ def load_library(path: Optional[str] = None):
         raise TlqError(-5, f"{path} is not built: run python -m tlaloque_amd.build")
     L = ctypes.CDLL(path)
     vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+    fp = ctypes.POINTER(ctypes.c_float)
     sig = {
         "tlq_create": [ctypes.POINTER(TlqParams), ctypes.POINTER(vp)],
         "tlq_destroy": [vp],
@@ -112,6 +116,15 @@ def load_library(path: Optional[str] = None):
         "tlq_kernel_stats": [vp, ip, ctypes.c_char_p, ip, dp, ctypes.POINTER(ctypes.c_longlong)],
         "tlq_layout": [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ip),
                        ctypes.POINTER(ip)],
+        "tlq_copy_probe": [vp, ctypes.c_longlong, ip, dp],
+        "tlq_surrogate_init": [vp, ctypes.c_ulonglong],
+        "tlq_surrogate_set_params": [vp, fp, ip],
+        "tlq_surrogate_get_params": [vp, fp, ip],
+        "tlq_surrogate_generate": [vp, ctypes.c_longlong, ctypes.c_ulonglong],
+        "tlq_surrogate_samples": [vp, ctypes.c_longlong, ctypes.c_longlong, fp, fp],
+        "tlq_surrogate_train": [vp, ip, ip, ctypes.c_double, ctypes.c_double, dp],
+        "tlq_surrogate_eval": [vp, fp, ctypes.c_longlong, fp],
+        "tlq_surrogate_grad": [vp, fp, fp, ip, dp, dp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -131,6 +144,11 @@ def _check(rc: int) -> None:
 
 def _dptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
 def to_c_params(p: P.HydroParams) -> TlqParams:
@@ -284,6 +302,57 @@ class HydroCore:
             _check(_lib.tlq_kernel_stats(self._ctx, q, name, 32, ctypes.byref(ms), ctypes.byref(n)))
             out[name.value.decode()] = (ms.value, n.value)
         return out
+
+    def copy_probe(self, nbytes: int, iters: int = 10) -> float:
+        """Mean ms of one device-to-device copy of nbytes (streaming kernel)."""
+        ms = ctypes.c_double()
+        _check(_lib.tlq_copy_probe(self._ctx, int(nbytes), int(iters), ctypes.byref(ms)))
+        return ms.value
+
+    # learned cooling surrogate -------------------------------------------------
+    def surrogate_init(self, seed: int = 1234) -> None:
+        _check(_lib.tlq_surrogate_init(self._ctx, int(seed)))
+
+    def surrogate_params(self, theta: Optional[np.ndarray] = None) -> np.ndarray:
+        """Set (theta given) or get the FP32 master parameters."""
+        if theta is not None:
+            t = np.ascontiguousarray(theta, dtype=np.float32)
+            _check(_lib.tlq_surrogate_set_params(self._ctx, _fptr(t), t.size))
+            return t
+        out = np.empty(MLP_NPARAMS, dtype=np.float32)
+        _check(_lib.tlq_surrogate_get_params(self._ctx, _fptr(out), out.size))
+        return out
+
+    def surrogate_generate(self, n: int, seed: int = 7) -> None:
+        _check(_lib.tlq_surrogate_generate(self._ctx, int(n), int(seed)))
+
+    def surrogate_samples(self, first: int, n: int) -> Tuple[np.ndarray, np.ndarray]:
+        x = np.empty((n, 3), dtype=np.float32)
+        y = np.empty(n, dtype=np.float32)
+        _check(_lib.tlq_surrogate_samples(self._ctx, int(first), int(n), _fptr(x), _fptr(y)))
+        return x, y
+
+    def surrogate_train(self, steps: int, batch: int = 4096, lr_max: float = 3e-3,
+                        lr_min: float = 3e-5) -> float:
+        loss = ctypes.c_double()
+        _check(_lib.tlq_surrogate_train(self._ctx, int(steps), int(batch), lr_max, lr_min,
+                                        ctypes.byref(loss)))
+        return loss.value
+
+    def surrogate_eval(self, x: np.ndarray) -> np.ndarray:
+        xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 3)
+        out = np.empty(xx.shape[0], dtype=np.float32)
+        _check(_lib.tlq_surrogate_eval(self._ctx, _fptr(xx), xx.shape[0], _fptr(out)))
+        return out
+
+    def surrogate_grad(self, x: np.ndarray, y: np.ndarray) -> Tuple[np.ndarray, float]:
+        xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 3)
+        yy = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.empty(MLP_NPARAMS, dtype=np.float64)
+        loss = ctypes.c_double()
+        _check(_lib.tlq_surrogate_grad(self._ctx, _fptr(xx), _fptr(yy), xx.shape[0], _dptr(g),
+                                       ctypes.byref(loss)))
+        return g, loss.value
 
     def layout(self) -> Tuple[int, int, int]:
         cells, px, ox = ctypes.c_longlong(), ctypes.c_int(), ctypes.c_int()

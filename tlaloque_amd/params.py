@@ -32,7 +32,7 @@ LIMITER_NONE, LIMITER_VANLEER, LIMITER_MINMOD, LIMITER_UMIST, LIMITER_WOODWARD, 
 
 SOLVER_HLLC, SOLVER_HLL = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1          # FP32: FP32 storage and sweeps, FP64 guards
-COOL_OFF, COOL_EXACT, COOL_SURROGATE = 0, 1, 2
+COOL_OFF, COOL_EXACT, COOL_SURROGATE, COOL_SUBCYCLED = 0, 1, 2, 3
 
 
 @dataclass

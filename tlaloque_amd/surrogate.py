@@ -1,0 +1,73 @@
+"""Training and use of the learned cooling surrogate (host side).
+
+The network (3 -> 64 -> 64 -> 1, GELU, bf16 weights, FP32 accumulation) maps
+(log10 n, log10 T, log10(dt/t_cool)) to log10(T_new/T).  Its training data
+are generated on the GPU from the reference's cooling formula integrated to
+convergence (cooling mode TLQ_COOL_SUBCYCLED), and it is trained on the GPU
+by the library's own MFMA forward/backward kernels with Adam
+(include/tlq.h: tlq_surrogate_*).  A trained context then runs the fused
+step with cooling mode TLQ_COOL_SURROGATE, falling back per cell to the
+converged integrator where the network's inputs leave the training box or
+its answer fails the bounds check.
+"""
+from __future__ import annotations
+
+import dataclasses
+import os
+import time
+from typing import Optional
+
+import numpy as np
+
+from . import params as P
+from .hydro import HydroCore
+
+DEFAULT_SAMPLES = 1_000_000      # training set size of the cooling-medium configuration
+DEFAULT_STEPS = 6000
+DEFAULT_BATCH = 4096
+
+
+@dataclasses.dataclass
+class TrainReport:
+    samples: int
+    steps: int
+    batch: int
+    final_loss: float
+    seconds: float
+    heldout_rms: float      # dex, network vs converged integrator on held-out samples
+    heldout_max: float
+
+
+def table_path() -> str:
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "Z1.0.dat")
+
+
+def train(core: Optional[HydroCore] = None, samples: int = DEFAULT_SAMPLES,
+          steps: int = DEFAULT_STEPS, batch: int = DEFAULT_BATCH, seed: int = 2024,
+          heldout: int = 65536) -> tuple:
+    """Generate data and train on the GPU; returns (params, TrainReport).
+
+    With no ``core`` a small scratch context (any mesh works: the network does
+    not depend on it) is created with the Z1.0 cooling table."""
+    own = core is None
+    if own:
+        p = P.HydroParams(nx=8, ny=8, nz=8, cooling=P.COOL_SUBCYCLED)
+        core = HydroCore(p, cooling_table=P.load_cooling_table(table_path()))
+    try:
+        # held-out set: an independent draw, never seen in training
+        core.surrogate_generate(heldout, seed=seed + 99)
+        x, y = core.surrogate_samples(0, heldout)
+        t0 = time.perf_counter()
+        core.surrogate_generate(samples, seed=seed)
+        core.surrogate_init(seed=seed + 1)
+        loss = core.surrogate_train(steps, batch=batch)
+        theta = core.surrogate_params()
+        el = time.perf_counter() - t0
+        pred = core.surrogate_eval(x)
+        err = np.abs(pred - y)
+        rep = TrainReport(samples, steps, batch, loss, el, float(np.sqrt((err ** 2).mean())),
+                          float(err.max()))
+        return theta, rep
+    finally:
+        if own:
+            core.close()
