@@ -141,8 +141,10 @@ int tlq_sync(tlq_ctx *ctx);
 /* Diagnostics: device-side check that every interior density is finite and
  * positive and every energy finite; *bad receives the count of bad cells. */
 int tlq_check_state(tlq_ctx *ctx, long long *bad);
-/* Fraction of interior cells that took the exact-cooling fallback in the
- * last surrogate-cooling step. */
+/* Of the interior cells at T >= T_min with dt > 0.01 t_cool in the last
+ * surrogate-cooling step (the cells the network is evaluated for; below that
+ * one substep of cooling.f90:67-112 is the converged answer), the fraction
+ * that took the exact-integrator fallback. */
 int tlq_fallback_fraction(tlq_ctx *ctx, double *frac);
 
 /* Learned cooling surrogate (MLP 3 -> 64 -> 64 -> 1, GELU, bf16 weights,

@@ -578,7 +578,7 @@ double orc_cool_subcycled(const orc_params *p, double n, double T, double dt, do
  * FP32 accumulation error. */
 #define MLP_H 64
 #define MLP_NP (MLP_H * 3 + MLP_H + MLP_H * MLP_H + MLP_H + MLP_H + 1)
-static const double MLP_C[3] = {0.5, 5.25, -2.0}, MLP_S[3] = {3.5, 3.25, 2.0};
+static const double MLP_C[3] = {0.5, 6.5, 0.0}, MLP_S[3] = {3.5, 2.5, 2.0};
 
 int orc_mlp_nparams(void) { return MLP_NP; }
 

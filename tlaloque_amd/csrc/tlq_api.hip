@@ -576,6 +576,19 @@ int tlq_set_cooling_table(tlq_ctx *c, const double *T, const double *L, int n) {
     }
     P.f_tmin = (float)T[0];
     P.f_tmax = (float)T[n - 1];
+    {  // cumulative cooling integral Y at the nodes (closed form per power-law segment)
+        double y = 0.0;
+        P.tab_Ycum[0] = 0.0;
+        P.tab_ycoef[0] = 0.f;
+        for (int q = 1; q < n; ++q) {
+            const double sl = (L[q] - L[q - 1]) / (P.tab_logT[q] - P.tab_logT[q - 1]);
+            const double coef = T[q - 1] / std::pow(10.0, L[q - 1]);
+            const double a = 1.0 - sl, ratio = T[q] / T[q - 1];
+            y += std::fabs(a) > 1e-12 ? coef * (std::pow(ratio, a) - 1.0) / a : coef * std::log(ratio);
+            P.tab_Ycum[q] = y;
+            P.tab_ycoef[q] = (float)coef;
+        }
+    }
     // direct indexing when every node lies within 0.4 spacings of a uniform grid
     const double d = (P.tab_logT[n - 1] - P.tab_logT[0]) / (n - 1);
     double dev = 0.0;

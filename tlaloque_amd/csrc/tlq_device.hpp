@@ -68,6 +68,11 @@ struct Problem {
     int tab_uniform;      // table equally spaced in log10 T (direct indexing)
     float f_lgT0, f_inv_dlg;
     float tab_Tf[MAX_TABLE], tab_lgTf[MAX_TABLE], tab_Lf[MAX_TABLE], tab_slopef[MAX_TABLE];
+    // Y(T) = int_{T_0}^{T} dT'/Lambda(T') at the nodes (double: differences of
+    // nearby values are taken), and T_{i-1}/Lambda_{i-1} of segment i (Lambda
+    // is a power law on each segment of the table).
+    double tab_Ycum[MAX_TABLE];
+    float tab_ycoef[MAX_TABLE];
     // ... and gravity in units of L = dx: cell centre X_i = g_X0[m] + i (same for Y, Z
     // with steps g_dY, g_dZ); acceleration factor g_A[m] = 2 G M / L^2.
     float g_A[MAX_MASSES], g_X0[MAX_MASSES], g_Y0[MAX_MASSES], g_Z0[MAX_MASSES];

@@ -57,31 +57,60 @@ __device__ double cool_sub_d(const Problem &P, double n, double T, double dt, do
     return T;
 }
 
-// Lambda(T) from the table in FP32 (row search as cool_fast).
+// Table segment of temperature T (first row with T_i > T, as find_coef),
+// clamped to the last segment for T >= T_max.
+__device__ __forceinline__ int table_row(const Problem &P, float temp, float lg) {
+    if (!(temp < P.f_tmax)) return P.ntable - 1;
+    int hi;
+    if (P.tab_uniform) {
+        hi = (int)((lg - P.f_lgT0) * P.f_inv_dlg) + 1;
+        hi = hi < 1 ? 1 : (hi > P.ntable - 1 ? P.ntable - 1 : hi);
+        if (!(P.tab_Tf[hi] > temp) && hi < P.ntable - 1) ++hi;
+        if (hi > 1 && P.tab_Tf[hi - 1] > temp) --hi;
+    } else {
+        int lo = 0;
+        hi = P.ntable - 1;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (P.tab_Tf[mid] > temp) hi = mid;
+            else lo = mid;
+        }
+    }
+    return hi;
+}
+
+// Lambda(T) from the table in FP32 (find_coef's interpolation).
 __device__ __forceinline__ float lambda_f(const Problem &P, float temp) {
     const float lg = __builtin_amdgcn_logf(temp) * 0.301029995663981f;
-    float lgl;
-    if (!(temp < P.f_tmax)) {
-        lgl = P.tab_Lf[P.ntable - 1];
-    } else {
-        int hi;
-        if (P.tab_uniform) {
-            hi = (int)((lg - P.f_lgT0) * P.f_inv_dlg) + 1;
-            hi = hi < 1 ? 1 : (hi > P.ntable - 1 ? P.ntable - 1 : hi);
-            if (!(P.tab_Tf[hi] > temp) && hi < P.ntable - 1) ++hi;
-            if (hi > 1 && P.tab_Tf[hi - 1] > temp) --hi;
-        } else {
-            int lo = 0;
-            hi = P.ntable - 1;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (P.tab_Tf[mid] > temp) hi = mid;
-                else lo = mid;
-            }
-        }
-        lgl = P.tab_Lf[hi - 1] + P.tab_slopef[hi] * (lg - P.tab_lgTf[hi - 1]);
-    }
+    if (!(temp < P.f_tmax)) return __builtin_amdgcn_exp2f(P.tab_Lf[P.ntable - 1] * 3.32192809488736f);
+    const int hi = table_row(P, temp, lg);
+    const float lgl = P.tab_Lf[hi - 1] + P.tab_slopef[hi] * (lg - P.tab_lgTf[hi - 1]);
     return __builtin_amdgcn_exp2f(lgl * 3.32192809488736f);
+}
+
+// Y(T) = int_{T_0}^{T} dT'/Lambda(T'): on each table segment Lambda is a
+// power law (linear in log-log), so Y is closed form.  The time to cool from
+// T0 to T1 at density n is t = k_B / ((gamma-1) n) (Y(T0) - Y(T1)).  Y is
+// split into its value at the segment's lower node (double) and the part
+// inside the segment (float), so that Y(T0) - Y(T1) keeps its accuracy when
+// T1 is close to T0.
+struct YSplit {
+    double node;
+    float part;
+};
+__device__ __forceinline__ YSplit cool_Y(const Problem &P, float temp) {
+    if (!(temp > P.f_tmin)) return {0.0, 0.f};
+    const float l2 = __builtin_amdgcn_logf(temp);
+    const int hi = table_row(P, temp, l2 * 0.301029995663981f);
+    const float a = 1.f - P.tab_slopef[hi];
+    const float lr = l2 - __builtin_amdgcn_logf(P.tab_Tf[hi - 1]);  // log2(T/T_{hi-1})
+    const float seg = fabsf(a) > 1e-4f ? (__builtin_amdgcn_exp2f(a * lr) - 1.f) / a
+                                       : lr * 0.6931471805599453f;
+    return {P.tab_Ycum[hi - 1], P.tab_ycoef[hi] * seg};
+}
+__device__ __forceinline__ float cool_dY(const Problem &P, float T0, float T1) {
+    const YSplit a = cool_Y(P, T0), b = cool_Y(P, T1);
+    return (float)(a.node - b.node) + (a.part - b.part);
 }
 
 __device__ float cool_sub_f(const Problem &P, float n, float T, float dt, float eps, int max_sub) {
@@ -101,8 +130,9 @@ __device__ float cool_sub_f(const Problem &P, float n, float T, float dt, float 
 }
 
 // ---------------------------------------------------------------------------
-// Training data: uniform samples of the training box, targets from the FP64
-// converged integrator.  Counter-based generator (splitmix64 of seed, index).
+// Training data: uniform samples of the training box, targets g = r / r1 from
+// the FP64 converged integrator.  Counter-based generator (splitmix64 of
+// seed, index).
 __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
     z += 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -128,7 +158,7 @@ __global__ void k_gen_samples(const Problem *__restrict__ Pp, long long n, unsig
             const double eth = nd * KB * T / (P.gamma - 1.0);
             const double ce = find_coef(P, T) * nd * nd / eth;
             const double dt = pow(10.0, x2) / ce;
-            r = log10(cool_sub_d(P, nd, T, dt, eps, max_sub) / T);
+            r = log10(cool_sub_d(P, nd, T, dt, eps, max_sub) / T) / first_order(x2);
         }
         X[3 * t + 0] = (float)x0;
         X[3 * t + 1] = (float)x1;
@@ -435,12 +465,17 @@ __global__ __launch_bounds__(64) void k_mlp_eval(const unsigned short *__restric
 
 // ---------------------------------------------------------------------------
 // Cooling pass of the fused step (modes 2 and 3): one wavefront per 64 cells
-// of an x row.  Mode 2 evaluates the surrogate for every cell at T >= T_min
-// and keeps its answer when
+// of an x row.  Cells at T >= T_min with dt <= eps t_cool take one substep
+// (the converged integrator takes no more).  For the others mode 3 runs the
+// converged integrator and mode 2 evaluates the surrogate (skipped by a
+// wavefront with no such cell) and keeps its answer T1 when
 //   * the inputs lie in the training box,
-//   * 10 K <= T_new <= T (cooling only lowers T, never below the floor), and
-//   * for dt/t_cool < 0.05 it agrees with one reference step to first order
-//     (|r - r1| <= 0.1 |r1| + 1e-4, r1 = log10 of one apply_cooling factor);
+//   * 10 K <= T1 <= T0 (cooling only lowers T, never below the floor), and
+//   * the exact cooling time from T0 to T1, t = k_B (Y(T0) - Y(T1)) / ((gamma-1) n),
+//     is within 0.05 min(dt, t_cool(T1)) of dt, so that ln T1 is within 5% of
+//     its change (at most 0.05) of the exact solution -- or, for T1 below the
+//     gate T_min (where the converged integrator stops), the gas reaches T_min
+//     within that time;
 // other cells take the exact converged integrator (the fallback).  Then the
 // CFL maxima of the cooled state (time_step of the next step).
 constexpr float COOL_EPS = 0.01f;
@@ -486,18 +521,27 @@ __global__ __launch_bounds__(64) void k_cool_pass(const Problem *__restrict__ Pp
         const double T = p * P.mu0 * AMU / (rho * KB);
         const bool hot = in && T >= P.tmin;
         double Tn = T;
+        // dt <= eps t_cool: the converged integrator takes exactly one
+        // substep, i.e. the reference's formula -- no network, no fallback
+        bool big = hot;
+        if (hot) {
+            if constexpr (sizeof(R) == 8) {
+                big = find_coef(P, T) * nd * nd / (p / (P.gamma - 1.0)) * dt > (double)COOL_EPS;
+            } else {
+                const float ce = lambda_f(P, (float)T) * (float)nd * (float)nd *
+                                 (float)(P.gamma - 1.0) / (float)p;
+                big = ce * (float)dt > COOL_EPS;
+            }
+        }
         bool need_exact = false;
-        if (MODE == 2) {
+        if (MODE == 2 && __any(big)) {
             float x[3] = {0.f, 0.f, 0.f};
-            float r1 = 0.f, dtc = 0.f;
-            if (hot) {
+            if (big) {
                 const float lam = lambda_f(P, (float)T);
                 const float ce = lam * (float)nd * (float)nd * (float)(P.gamma - 1.0) / (float)p;
-                dtc = ce * (float)dt;
                 x[0] = log10f((float)nd);
                 x[1] = log10f((float)T);
-                x[2] = log10f(dtc);
-                r1 = log10f(fmaxf(__expf(-dtc), (float)(1.0 - P.cooling_limit)));
+                x[2] = log10f(ce * (float)dt);
             }
             float xn[3];
             normalise(x, xn);
@@ -508,24 +552,43 @@ __global__ __launch_bounds__(64) void k_cool_pass(const Problem *__restrict__ Pp
             float o = b3;
             for (int q = 0; q < H; ++q) o += sh.w3[q] * gelu(sh.f[lane][q] + sh.b2[q]);
             __syncthreads();
-            if (hot) {
+            if (big) {
                 const bool inbox = x[0] >= X0LO && x[0] <= X0HI && x[1] >= X1LO && x[1] <= X1HI &&
-                                   x[2] >= X2LO && x[2] <= X2HI;
+                                   x[2] <= X2HI;
+                const float d = __builtin_amdgcn_exp2f(x[2] * 3.32192809488736f);
+                const float rh = -0.4342944819032518f * d / (1.f + d) * o;  // log10(T1/T0)
                 const float rmin = log10f(10.f / (float)T);
-                bool ok = inbox && o <= 1e-3f && o >= rmin - 1e-3f;
-                if (ok && dtc < 0.05f) ok = fabsf(o - r1) <= 0.1f * fabsf(r1) + 1e-4f;
-                if (ok) Tn = T * (double)__builtin_amdgcn_exp2f(fminf(o, 0.f) * 3.32192809488736f);
+                bool ok = inbox && rh <= 1e-3f && rh >= rmin - 1e-3f;
+                const float T0 = (float)T;
+                const float T1 = T0 * __builtin_amdgcn_exp2f(fminf(rh, 0.f) * 3.32192809488736f);
+                if (ok) {
+                    const float kn = (float)(KB / (P.gamma - 1.0)) / (float)nd;
+                    const float fdt = (float)dt;
+                    // a time error dt' shifts ln T1 by dt'/t_cool(T1): allow 5% of
+                    // the change in ln T, at most 0.05
+                    const float Te = fmaxf(T1, P.f_tmin);
+                    const float tol = 0.05f * fminf(fdt, kn * Te / lambda_f(P, Te));
+                    if (T1 <= P.f_tmin)
+                        ok = T1 >= 0.97f * P.f_tmin && kn * cool_dY(P, T0, P.f_tmin) <= fdt + tol;
+                    else
+                        ok = fabsf(kn * cool_dY(P, T0, T1) - fdt) <= tol;
+                }
+                if (ok) Tn = (double)T1;
                 else need_exact = true;
             }
-        } else {
-            need_exact = hot;
+        } else if (MODE != 2) {
+            need_exact = big;
         }
-        if (need_exact) {
+        if (hot && !big) {  // one substep: apply_cooling's formula
+            if constexpr (sizeof(R) == 8) Tn = cool_sub_d(P, nd, T, dt, COOL_EPS, 1);
+            else Tn = cool_sub_f(P, (float)nd, (float)T, (float)dt, COOL_EPS, 1);
+        }
+        if (need_exact) {  // the converged integrator (mode 3, and mode 2's fallback)
             if constexpr (sizeof(R) == 8) Tn = cool_sub_d(P, nd, T, dt, COOL_EPS, COOL_MAXSUB);
             else Tn = cool_sub_f(P, (float)nd, (float)T, (float)dt, COOL_EPS, COOL_MAXSUB);
         }
         if (hot) {
-            ++ncool;
+            ncool += big;
             if (MODE == 2 && need_exact) ++nfall;
             const double pn = p * (Tn / T);
             const R e5 = (R)(ek + pn / (P.gamma - 1.0));

@@ -1,7 +1,10 @@
 // tlq_surrogate.hpp -- the learned radiative-cooling surrogate.
 //
-// The network maps (log10 n, log10 T, log10(dt/t_cool)) to r = log10(T_new/T)
-// for one cooling step: an MLP 3 -> 64 -> 64 -> 1 with GELU (tanh form),
+// The network maps (log10 n, log10 T, log10(dt/t_cool)) to g = r / r1 for one
+// cooling step, r = log10(T_new/T) and r1 = -log10(e) d / (1 + d), d =
+// dt/t_cool (r1 is r to first order in d and bounded for large d; learning
+// the ratio makes the network's error relative to the change of T, which is
+// what the per-cell check measures): an MLP 3 -> 64 -> 64 -> 1 with GELU (tanh form),
 // bf16 weights and FP32 accumulation.  Its training targets come from the
 // reference's cooling formula (apply_cooling, cooling.f90:67-112) integrated
 // to convergence: the formula is applied in substeps of at most eps t_cool
@@ -13,6 +16,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include "tlq_device.hpp"
 
 namespace tlq {
@@ -22,8 +27,17 @@ constexpr int H = 64;
 constexpr int OW1 = 0, OB1 = 3 * H, OW2 = OB1 + H, OB2 = OW2 + H * H, OW3 = OB2 + H,
               OB3 = OW3 + H, NP = OB3 + 1;  // 4481 parameters
 // input normalisation: x_n = (x - C) / S maps the training box onto [-1, 1]
-constexpr float C0 = 0.5f, C1 = 5.25f, C2 = -2.0f, S0 = 3.5f, S1 = 3.25f, S2 = 2.0f;
-constexpr float X0LO = -3.f, X0HI = 4.f, X1LO = 2.f, X1HI = 8.5f, X2LO = -4.f, X2HI = 0.f;
+// training box: n in [1e-3, 1e4] cm^-3, T in [1e4, 1e9] K (the table's range:
+// nothing cools below T_min) and dt/t_cool in [eps, 100] (below eps one
+// substep is the converged answer)
+constexpr float C0 = 0.5f, C1 = 6.5f, C2 = 0.0f, S0 = 3.5f, S1 = 2.5f, S2 = 2.0f;
+constexpr float X0LO = -3.f, X0HI = 4.f, X1LO = 4.f, X1HI = 9.f, X2LO = -2.f, X2HI = 2.f;
+
+// r1(x2) = -log10(e) d / (1 + d), d = 10^x2: the scale of the network's output
+__host__ __device__ inline double first_order(double x2) {
+    const double d = std::pow(10.0, x2);
+    return -0.4342944819032518 * d / (1.0 + d);
+}
 
 }  // namespace mlp
 
@@ -40,8 +54,9 @@ hipError_t sur_eval(hipStream_t s, const unsigned short *thq, const float *X, lo
 
 // Cooling pass over the interior after the fused corrector: mode 2 = surrogate
 // with per-cell fallback, mode 3 = the converged exact integrator.  Also
-// produces the next step's CFL maxima.  counts[0] += cells at T >= T_min,
-// counts[1] += cells that took the exact fallback.
+// produces the next step's CFL maxima.  counts[0] += cells at T >= T_min with
+// dt > eps t_cool (those handed to the network in mode 2), counts[1] += cells
+// among them that took the exact fallback.
 struct StepState;
 template <class R>
 hipError_t sur_cool_pass(hipStream_t s, const Problem *P, const Grid &g, R *U,
