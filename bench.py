@@ -11,8 +11,13 @@ reference's main loop (source/mainHD.f90:74-99): two fused kernels.
 
 Prints one JSON line: zone-updates/s over K timed steps (inputs resident in
 HBM), the HBM roofline of the dominant kernel (stage2), the CPU baseline (the
-FP64 C oracle with OpenMP on a bounded sample of the same workload), and the
-L1 error of the FP32 path against the FP64 path on the same problem.
+FP64 C oracle with OpenMP on a bounded sample of the same workload), the L1
+error of the FP32 path against the FP64 path on the same problem, and the
+cooling paths compared on the same mesh: the reference's one-step formula
+fused into stage 2 (the headline value), the learned surrogate with its
+per-cell fallback (trained in this run on the GPU), and the converged
+integrator (the surrogate's target and fallback), on the workload as given
+and on a strongly cooling variant of it (T = 3e5 K).
 """
 from __future__ import annotations
 
@@ -33,8 +38,81 @@ BYTES_STAGE2 = 60           # per zone: read U^{1/2} and U^n, write U^{n+1}
 BYTES_STEP = BYTES_STAGE1 + BYTES_STAGE2
 
 
+TRAFFIC_REPORT = os.path.join(ROOT, "profiles", "traffic_clumpy512.md")
+COPY_PROBE_BYTES = 2 << 30
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def timed_run(p, u0, table, warmup, steps, theta=None):
+    """Advance a fresh context over `steps` timed steps; returns
+    (zone-updates/s, ms per step, per-kernel ms, fallback fraction, bad cells)."""
+    from tlaloque_amd.hydro import HydroCore
+    with HydroCore(p, cooling_table=table) as h:
+        if theta is not None:
+            h.surrogate_params(theta)
+        h.set_state(u0)
+        h.advance(warmup)
+        h.sync()
+        h.kernel_timing(True)
+        h.timer_start()
+        h.advance(steps)
+        ms = h.timer_stop()
+        ks = h.kernel_stats()
+        h.kernel_timing(False)
+        frac = h.fallback_fraction()
+        bad = h.check_state()
+    kms = {k: v[0] / max(v[1], 1) for k, v in ks.items() if v[1]}
+    return p.zones * steps / (ms / 1e3), ms / steps, kms, frac, bad
+
+
+def heated(p, u0, temp):
+    """The same state with the thermal energy scaled to temperature `temp`
+    (every cell of the clumpy set-up starts at one temperature)."""
+    from tlaloque_amd import params as P
+    u = u0.copy()
+    rho = u[..., 0]
+    ek = 0.5 * (u[..., 1] ** 2 + u[..., 2] ** 2 + u[..., 3] ** 2) / rho
+    eth = u[..., 4] - ek
+    t0 = float(np.median((p.gamma - 1.0) * eth * p.mu0 * P.AMU / (rho * P.KB)))
+    u[..., 4] = ek + eth * (temp / t0)
+    return u
+
+
+def cooling_modes(p, u0, table, warmup, steps):
+    """The three cooling paths on the workload and on its T = 3e5 K variant."""
+    from tlaloque_amd import params as P
+    from tlaloque_amd import surrogate
+    t0 = time.perf_counter()
+    theta, rep = surrogate.train()
+    log(f"bench: surrogate trained in {time.perf_counter() - t0:.1f}s: {rep}")
+    out = {"surrogate_training": {"samples": rep.samples, "steps": rep.steps,
+                                  "batch": rep.batch, "seconds": rep.seconds,
+                                  "heldout_rms_dex": rep.heldout_rms,
+                                  "heldout_rel_median": rep.heldout_rel_median,
+                                  "heldout_rel_p90": rep.heldout_rel_p90}}
+    for label, u in (("T=1e4K", u0), ("T=3e5K", heated(p, u0, 3e5))):
+        res = {}
+        for name, mode in (("reference_one_step", P.COOL_EXACT),
+                           ("surrogate", P.COOL_SURROGATE),
+                           ("converged_integrator", P.COOL_SUBCYCLED)):
+            if label == "T=1e4K" and mode == P.COOL_EXACT:
+                continue  # the headline run
+            zps, ms, kms, frac, bad = timed_run(p.replace(cooling=mode), u, table, warmup, steps,
+                                                theta if mode == P.COOL_SURROGATE else None)
+            res[name] = {"zone_updates_per_s": zps, "ms_per_step": ms,
+                         "cooling_pass_ms": kms.get("cooling"), "nonfinite_cells": bad}
+            if mode == P.COOL_SURROGATE:
+                res[name]["fallback_fraction"] = frac
+            log(f"bench: {label} {name}: {zps:.4g} zone-updates/s, {ms:.3f} ms/step, "
+                f"kernels {kms}" + (f", fallback {frac:.4f}" if mode == P.COOL_SURROGATE else ""))
+        if "surrogate" in res and "converged_integrator" in res:
+            res["surrogate_speedup_vs_converged"] = (res["surrogate"]["zone_updates_per_s"] /
+                                                     res["converged_integrator"]["zone_updates_per_s"])
+        out[label] = res
+    return out
 
 
 def cpu_baseline(n: int, steps: int, threads: int):
@@ -87,6 +165,7 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-l1", action="store_true")
+    ap.add_argument("--no-modes", action="store_true", help="skip the cooling-path comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -104,7 +183,6 @@ def main():
     table = P.load_cooling_table(os.path.join(ROOT, "tlaloque_amd", "data", "Z1.0.dat"))
     h = HydroCore(p, cooling_table=table)
     h.set_state(u0)
-    del u0
     h.advance(args.warmup)
     h.sync()
     log(f"bench: {args.warmup} warm-up steps done")
@@ -116,14 +194,24 @@ def main():
     h.kernel_timing(False)
     bad = h.check_state()
     dt, t, it = h.time()
+    copy_ms = h.copy_probe(COPY_PROBE_BYTES, 10)
     h.close()
     zones = n ** 3
     value = zones * args.steps / (ms / 1e3)
     s2_ms = ks["stage2"][0] / max(ks["stage2"][1], 1)
     s1_ms = ks["stage1"][0] / max(ks["stage1"][1], 1)
     achieved = BYTES_STAGE2 * zones / (s2_ms / 1e3) / 1e9
+    copy_gbs = 2 * COPY_PROBE_BYTES / (copy_ms / 1e3) / 1e9
     log(f"bench: {args.steps} steps in {ms:.1f} ms; stage1 {s1_ms:.3f} ms, stage2 {s2_ms:.3f} ms; "
-        f"bad cells {bad}; t={t:.4e} it={it}")
+        f"bad cells {bad}; t={t:.4e} it={it}; copy probe {copy_gbs:.0f} GB/s")
+    traffic, traffic_note = None, None
+    if n == 512 and os.path.exists(TRAFFIC_REPORT):
+        from tlaloque_amd.profiling import read_traffic
+        tr = read_traffic(TRAFFIC_REPORT)
+        if tr.get("zones") == zones and "k_stage2" in tr["kernels"]:
+            traffic = tr["kernels"]["k_stage2"]["bytes"]
+            traffic_note = (f"rocprofv3 FETCH_SIZE + WRITE_SIZE of k_stage2, calibrated on a "
+                            f"streaming copy: profiles/{os.path.basename(TRAFFIC_REPORT)}")
 
     result = {
         "metric": "zone-updates/s at 512^3 HLLC+cooling (exact cooling path)",
@@ -142,13 +230,19 @@ def main():
                                "Z1.0 cooling, 1e4 Msun softened point mass, periodic",
                    "mesh": [n, n, n], "precision": "fp32+fp64-guard", "cooling": "exact"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_note,
                      "kernel": "stage2", "bytes_per_zone": BYTES_STAGE2,
+                     "algorithmic_bytes_per_launch": BYTES_STAGE2 * zones,
                      "avg_launch_ms": s2_ms},
+        "copy_probe_gbs": copy_gbs,
         "step_roofline_frac": value * BYTES_STEP / 1e9 / HBM_PEAK_GBS,
         "kernels_ms": {"stage1": s1_ms, "stage2": s2_ms},
         "nonfinite_cells": bad,
     }
+    if not args.no_modes:
+        result["cooling_paths"] = cooling_modes(p, u0, table, args.warmup, args.steps)
+    del u0
     if not args.no_l1:
         t0 = time.perf_counter()
         result["l1_vs_fp64"] = l1_fp32_vs_fp64(128, 10)
@@ -156,12 +250,12 @@ def main():
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         t0 = time.perf_counter()
-        zps, el = cpu_baseline(160, 3, threads)
+        zps, el = cpu_baseline(256, 20, threads)
         log(f"bench: CPU baseline done in {time.perf_counter() - t0:.1f}s")
         result["cpu_baseline"] = {"value": zps, "unit": "zone-updates/s", "cores": threads,
                                   "kind": "port",
                                   "sample": f"FP64 C oracle (OpenMP, {threads} threads), same "
-                                            f"clumpy workload at 160^3, 3 steps ({el:.1f} s)"}
+                                            f"clumpy workload at 256^3, 20 steps ({el:.1f} s)"}
     print(json.dumps(result), flush=True)
 
 
