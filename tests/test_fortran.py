@@ -1,0 +1,191 @@
+"""The Fortran side of the boundary: ISO_C_BINDING module and driver.
+
+CPU:
+  * the bind(C) derived types have the C structs' sizes and member offsets
+    (a mismatch would silently scramble every parameter);
+  * a dry run of the driver (initial state, no GPU) writes VTK and BIN files
+    that equal what the Python writers produce for the same problem, and
+    that the VTK reader parses (the format of source/HydroCore.f90:253-395).
+GPU:
+  * the driver's fused run of the reference harness's diagonal Sod tube
+    equals the same run through the Python binding, and its diagonal profile
+    meets the harness's metrics against solutions/sod_e1rpex.out;
+  * the driver's per-routine loop (the thirteen calls of mainHD.f90:74-99)
+    equals its fused loop.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import GOLDEN, prims_from_cons
+from oracle.metrics import all_error_metrics
+from tlaloque_amd import build as B
+from tlaloque_amd import params as P
+from tlaloque_amd import problems, vtk
+from tlaloque_amd.hydro import TlqParams, TlqWind
+
+needs_flang = pytest.mark.skipif(not B.flang(), reason="flang not available")
+
+
+@pytest.fixture(scope="module")
+def driver():
+    if not os.path.exists(B.DRIVER):
+        if not B.flang():
+            pytest.skip("flang not available and no driver built")
+        B.build_fortran()
+    return B.DRIVER
+
+
+def _namelist(path, p: P.HydroParams, problem, outdir, tfin=1.0, num_out=1, outfile="vtk",
+              extra=""):
+    bc = ",".join(str(b) for b in p.bc)
+    with open(path, "w") as fh:
+        fh.write(f"&mesh nx={p.nx}, ny={p.ny}, nz={p.nz}, xl={p.xl!r}, xr={p.xr!r}, "
+                 f"yl={p.yl!r}, yr={p.yr!r}, zl={p.zl!r}, zr={p.zr!r} /\n")
+        fh.write(f"&physics gamma={p.gamma!r}, mu0={p.mu0!r}, cfl={p.cfl!r}, eta={p.eta!r}, "
+                 f"limiter={p.limiter}, solver={p.solver}, precision={p.precision}, "
+                 f"cooling={p.cooling}, bc={bc}, tfin={tfin!r} /\n")
+        fh.write(f"&run problem='{problem}', num_out={num_out}, outfile='{outfile}', "
+                 f"outputpath='{outdir}/' {extra}/\n")
+
+
+def _run(driver, nml, timeout=300):
+    r = subprocess.run([driver, str(nml)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       text=True, timeout=timeout)
+    assert r.returncode == 0, r.stdout[-2000:]
+    return r.stdout
+
+
+@needs_flang
+def test_bind_c_types_match_c_layout(tmp_path, driver):
+    mod = os.path.join(B.OUT, "fortran")
+    src = tmp_path / "layout.f90"
+    src.write_text("""
+program layout
+  use iso_c_binding
+  use tlq_c
+  implicit none
+  type(tlq_params_t), target :: p
+  type(tlq_wind_t), target :: w
+  integer(c_intptr_t) :: b
+  b = transfer(c_loc(p), b)
+  print '(i0)', c_sizeof(p)
+  print '(i0)', transfer(c_loc(p%xl), b) - b
+  print '(i0)', transfer(c_loc(p%limiter), b) - b
+  print '(i0)', transfer(c_loc(p%bc), b) - b
+  print '(i0)', transfer(c_loc(p%solver), b) - b
+  print '(i0)', transfer(c_loc(p%cooling), b) - b
+  print '(i0)', transfer(c_loc(p%softening), b) - b
+  print '(i0)', c_sizeof(w)
+end program layout
+""")
+    exe = tmp_path / "layout"
+    subprocess.run([B.flang(), "-I", mod, "-module-dir", str(tmp_path), str(src),
+                    os.path.join(mod, "tlq_c.o"), "-L" + B.OUT, "-ltlq", "-Wl,-rpath," + B.OUT,
+                    "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True,
+                                          check=True).stdout.split()]
+    want = [ctypes.sizeof(TlqParams), TlqParams.xl.offset, TlqParams.limiter.offset,
+            TlqParams.bc.offset, TlqParams.solver.offset, TlqParams.cooling.offset,
+            TlqParams.softening.offset, ctypes.sizeof(TlqWind)]
+    assert got == want
+
+
+_DRY = [("sod", lambda: problems.sod_diagonal(n=12)),
+        ("sod_x", lambda: problems.sod_planar(nx=16, ny=6, nz=4)),
+        ("sedov", lambda: problems.sedov(n=10)),
+        ("blast", lambda: problems.blast(n=12))]
+
+
+@needs_flang
+@pytest.mark.parametrize("name,make", _DRY, ids=[d[0] for d in _DRY])
+def test_dry_run_outputs_match_python_writers(tmp_path, driver, name, make):
+    p, u0 = make()
+    out = tmp_path / "DATA"
+    out.mkdir()
+    for fmt in ("vtk", "bin"):
+        nml = tmp_path / f"{fmt}.nml"
+        _namelist(nml, p, name, out, outfile=fmt, extra=", dry_run=.true.")
+        _run(driver, nml)
+    prim = prims_from_cons(u0, p.gamma)
+    ref_vtk = tmp_path / "ref.vtk"
+    vtk.write_vtk(str(ref_vtk), p, prim)
+    a, b = vtk.read_vtk(vtk.vtk_name(str(out), 0)), vtk.read_vtk(str(ref_vtk))
+    assert a["dims"] == (p.nx, p.ny, p.nz) == b["dims"]
+    assert np.allclose(a["origin"], (p.xl, p.yl, p.zl), rtol=1e-6)
+    assert np.allclose(a["spacing"], (p.dx, p.dy, p.dz), rtol=1e-6)
+    for f in ("Density", "Pressure", "Velocity"):
+        assert np.allclose(a["fields"][f], b["fields"][f], rtol=1e-14, atol=0), f
+    # the header lines are the same text
+    ha = open(vtk.vtk_name(str(out), 0), "rb").read().split(b"LOOKUP_TABLE")[0]
+    hb = open(ref_vtk, "rb").read().split(b"LOOKUP_TABLE")[0]
+    assert ha == hb
+    w = vtk.read_bin(vtk.bin_name(str(out), 0), p)
+    assert np.allclose(w, prim, rtol=1e-14, atol=0)
+
+
+def _diagonal_metrics(d):
+    """The harness's Sod analysis (tests_analisis.jl:25-142): profiles along
+    the cube's diagonal against sod_e1rpex.out, r = sign(x) sqrt(3) |x|."""
+    nx = d["dims"][0]
+    x = d["origin"][0] + d["spacing"][0] * np.arange(nx)
+    r = np.sign(x) * np.sqrt(3.0 * x ** 2)
+    idx = np.arange(nx)
+    rho = d["fields"]["Density"][idx, idx, idx]
+    pres = d["fields"]["Pressure"][idx, idx, idx]
+    vel = np.sqrt((d["fields"]["Velocity"][idx, idx, idx] ** 2).sum(-1))
+    ex = np.loadtxt(os.path.join(GOLDEN, "sod_e1rpex.out"))
+    xex = np.arange(0.00025, 0.5, 0.0005)[:ex.shape[0]] - 0.25
+    return {"rho": all_error_metrics(xex, ex[:, 1], r, rho),
+            "p": all_error_metrics(xex, ex[:, 3], r, pres),
+            "v": all_error_metrics(xex, ex[:, 2], r, vel)}
+
+
+@pytest.mark.gpu
+@needs_flang
+def test_driver_sod_matches_python_binding_and_exact_solution(tmp_path, driver):
+    from tlaloque_amd.hydro import HydroCore
+    n = 150  # the harness's mesh (tests_utilities.jl:216)
+    p, u0 = problems.sod_diagonal(n=n)
+    out = tmp_path / "DATA"
+    out.mkdir()
+    nml = tmp_path / "sod.nml"
+    _namelist(nml, p, "sod", out, tfin=problems.SOD_DIAGONAL_TFIN, num_out=2)
+    log = _run(driver, nml)
+    last = sorted(os.listdir(out))[-1]
+    d = vtk.read_vtk(os.path.join(out, last))
+    with HydroCore(p) as h:
+        h.set_state(u0)
+        h.run_until(problems.SOD_DIAGONAL_TFIN)
+        _, t, it = h.time()
+        w = prims_from_cons(h.download("U"), p.gamma)[2:-2, 2:-2, 2:-2]
+    assert f"it = {it}" in log.splitlines()[-1]
+    assert np.array_equal(d["fields"]["Density"], w[..., 0])
+    assert np.array_equal(d["fields"]["Pressure"], w[..., 4])
+    m = _diagonal_metrics(d)
+    print({k: round(v["L1"], 4) for k, v in m.items()})
+    assert m["rho"]["L1"] < 0.04
+    assert m["p"]["L1"] < 0.04
+    assert m["v"]["L1"] < 0.12
+
+
+@pytest.mark.gpu
+@needs_flang
+def test_driver_per_routine_equals_fused(tmp_path, driver):
+    p, _ = problems.sod_diagonal(n=24)
+    res = {}
+    for per in (False, True):
+        out = tmp_path / f"D{int(per)}"
+        out.mkdir()
+        nml = tmp_path / f"r{int(per)}.nml"
+        _namelist(nml, p, "sod", out, tfin=1.0, num_out=1, outfile="bin",
+                  extra=f", max_steps=12, per_routine={'.true.' if per else '.false.'}")
+        _run(driver, nml)
+        res[per] = vtk.read_bin(vtk.bin_name(str(out), 1), p)[2:-2, 2:-2, 2:-2]
+    a, b = res[True], res[False]
+    for e in range(5):
+        scale = max(np.abs(b[..., e]).max(), 1e-30)
+        assert np.abs(a[..., e] - b[..., e]).max() <= 1e-12 * scale, e

@@ -1,0 +1,136 @@
+"""GPU runs of the benchmark's problem configurations at their full sizes,
+checked through what the mathematics fixes (no CPU run of these sizes):
+
+  * planar shock tube, 128x32x32, FP64 HLLC, gamma 1.4, t = 0.2: the (y, z)
+    columns agree to 1e-3 (not bitwise: the reference's ghost lag --
+    U's ghosts are the pre-viscosity UP's, HydroCore.f90:195-206 -- leaves a
+    transverse Laplacian of order eta^2 at the boundary columns; the oracle
+    shows the same spread), L1 of rho, p and u of the middle column against
+    the exact Riemann solution (oracle/riemann.py, pinned on
+    solutions/sod_e1rpex.out);
+  * point explosion, 256^3, HLL and HLLC in FP64 and in FP32 with FP64
+    guards, run to a shock radius of 0.4: mass and energy conserved to
+    round-off while nothing has left the box, the symmetries (x <-> z
+    exchange to round-off and mirror x -> -x exact for HLL, but not for HLLC,
+    whose star flux takes the left formula at S* = 0 with the cell-centred U
+    of HLLC.f90:299-300, so the two formulas differ at the tie -- the oracle
+    breaks the mirror the same way), the shock
+    radius and the radial density profile against the Sedov similarity
+    solution (solutions/sedov_solution_53.txt), and the FP32 path's L1 of
+    rho and p against the FP64 path on the same problem (target 1e-3).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import GOLDEN, l1_rel, prims_from_cons
+from oracle import riemann
+from tlaloque_amd import params as P
+from tlaloque_amd import problems
+from tlaloque_amd.hydro import HydroCore
+
+pytestmark = pytest.mark.gpu
+
+
+def test_planar_sod_matches_exact_riemann_solution():
+    p, u0 = problems.sod_planar()
+    with HydroCore(p) as h:
+        h.set_state(u0)
+        h.run_until(problems.SOD_TFIN)
+        _, t, it = h.time()
+        w = prims_from_cons(h.download("U"), p.gamma)[2:-2, 2:-2, 2:-2]
+        assert h.check_state() == 0
+    mid = w[p.nz // 2, p.ny // 2]
+    assert np.abs(w[..., 0] - mid[None, None, :, 0]).max() <= 1e-3 * mid[:, 0].max()
+    assert np.abs(w[..., 2:4]).max() <= 1e-3 * np.abs(mid[:, 1]).max()
+    x = P.xcoord(p, np.arange(1, p.nx + 1))
+    rho, u, pr = riemann.sample(x, t, 0.5, (1.0, 0.0, 1.0), (0.125, 0.0, 0.1), p.gamma)
+    col = mid
+    l1 = {"rho": l1_rel(col[:, 0], rho), "p": l1_rel(col[:, 4], pr),
+          "u": l1_rel(col[:, 1], u)}
+    print(f"planar Sod 128x32x32 t={t:.4f} it={it}: L1 {l1}")
+    # second-order scheme, 128 cells: the contact and the shock are smeared
+    # over a few cells (observed 0.021, 0.021, 0.046)
+    assert l1["rho"] < 0.03 and l1["p"] < 0.03 and l1["u"] < 0.06
+
+
+def _sedov_table():
+    a = np.loadtxt(os.path.join(GOLDEN, "sedov_solution_53.txt"))
+    return a[::-1]  # xi increasing
+
+
+def _blast_run(solver, precision):
+    p, u0 = problems.blast(n=256, solver=solver, precision=precision)
+    tfin = problems.blast_time(0.4)
+    with HydroCore(p) as h:
+        h.set_state(u0)
+        h.run_until(tfin)
+        _, t, it = h.time()
+        u = h.download("U")
+        assert h.check_state() == 0
+    return p, u0, u, t, it
+
+
+@pytest.fixture(scope="module")
+def blasts():
+    return {(s, q): _blast_run(s, q) for s in (P.SOLVER_HLLC, P.SOLVER_HLL)
+            for q in (P.PREC_FP64, P.PREC_FP32)}
+
+
+@pytest.mark.parametrize("solver", [P.SOLVER_HLLC, P.SOLVER_HLL], ids=["hllc", "hll"])
+@pytest.mark.parametrize("precision", [P.PREC_FP64, P.PREC_FP32], ids=["fp64", "fp32"])
+def test_blast_against_sedov(blasts, solver, precision):
+    p, u0, u, t, it = blasts[(solver, precision)]
+    ui, u0i = u[2:-2, 2:-2, 2:-2], u0[2:-2, 2:-2, 2:-2]
+    vol = p.dx * p.dy * p.dz
+    # the shock has not reached the boundary: mass and energy are conserved
+    # (the telescoping flux sum; viscosity conserves too)
+    tol = 1e-12 if precision == P.PREC_FP64 else 1e-5
+    assert abs(ui[..., 0].sum() / u0i[..., 0].sum() - 1.0) < tol
+    assert abs(ui[..., 4].sum() / u0i[..., 4].sum() - 1.0) < tol * 10
+    rho = ui[..., 0]
+    swap = np.abs(rho - rho.transpose(2, 1, 0)).max() / rho.max()
+    mirror = np.abs(rho - rho[::-1, :, :]).max() / rho.max()
+    if solver == P.SOLVER_HLL:
+        assert mirror <= (1e-12 if precision == P.PREC_FP64 else 1e-3)
+        assert swap <= (1e-9 if precision == P.PREC_FP64 else 1e-3)
+    else:
+        # the tie at S* = 0 seeds an asymmetry that the blast's centre grows
+        # over the 1700 steps (6e-4 for the x <-> z exchange, 4e-3 mirror)
+        assert mirror <= 2e-2 and swap <= 2e-2
+    # radial profile against the similarity solution
+    x = P.xcoord(p, np.arange(1, p.nx + 1))
+    r = np.sqrt(x[None, None, :] ** 2 + x[None, :, None] ** 2 + x[:, None, None] ** 2)
+    tab = _sedov_table()
+    xi0 = 1.151666417967
+    R = xi0 * (1.0 * t ** 2 / 1.0) ** 0.2
+    rho_s = np.interp(r / R * xi0, tab[:, 0], tab[:, 1], right=0.0) * 4.0  # rho1 = 4 rho0
+    inside = r < R
+    rho_s = np.where(inside, rho_s, 1.0)
+    shell = (r > 0.2 * R) & (r < 1.3 * R)
+    l1 = float(np.abs(rho[shell] - rho_s[shell]).sum() / np.abs(rho_s[shell]).sum())
+    # shock radius: radius of the density maximum, averaged over the sphere
+    nb = 200
+    bins = np.linspace(0.0, 0.6, nb + 1)
+    idx = np.digitize(r.ravel(), bins)
+    prof = np.bincount(idx, weights=rho.ravel(), minlength=nb + 2)[1:nb + 1] / \
+        np.maximum(np.bincount(idx, minlength=nb + 2)[1:nb + 1], 1)
+    r_peak = 0.5 * (bins[np.argmax(prof)] + bins[np.argmax(prof) + 1])
+    print(f"blast solver={solver} prec={precision} t={t:.5f} it={it}: R_sedov={R:.4f} "
+          f"r_peak={r_peak:.4f} shell L1(rho)={l1:.4f} peak rho={rho.max():.3f}")
+    assert abs(r_peak / R - 1.0) < 0.05
+    assert l1 < 0.3
+    assert vol > 0
+
+
+@pytest.mark.parametrize("solver", [P.SOLVER_HLLC, P.SOLVER_HLL], ids=["hllc", "hll"])
+def test_blast_fp32_l1_against_fp64(blasts, solver):
+    p, _, u64, t64, it64 = blasts[(solver, P.PREC_FP64)]
+    _, _, u32, t32, it32 = blasts[(solver, P.PREC_FP32)]
+    w64 = prims_from_cons(u64, p.gamma)[2:-2, 2:-2, 2:-2]
+    w32 = prims_from_cons(u32, p.gamma)[2:-2, 2:-2, 2:-2]
+    l1r, l1p = l1_rel(w32[..., 0], w64[..., 0]), l1_rel(w32[..., 4], w64[..., 4])
+    print(f"blast solver={solver}: FP32 vs FP64 L1 rho {l1r:.3e} p {l1p:.3e} "
+          f"(steps {it32} vs {it64})")
+    assert l1r <= 1e-3 and l1p <= 1e-3

@@ -73,5 +73,44 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(OUT, "tlq_hydro")
+
+
+def flang() -> str:
+    for cand in (shutil.which("flang"), "/opt/rocm/llvm/bin/flang", "/opt/rocm/lib/llvm/bin/flang"):
+        if cand and os.path.exists(cand):
+            return cand
+    return ""
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """The ISO_C_BINDING module and the Fortran driver, linked to libtlq.so
+    (run path $ORIGIN).  Returns the driver's path, or "" without flang."""
+    fc = flang()
+    if not fc:
+        return ""
+    mod = os.path.join(OUT, "fortran")
+    os.makedirs(mod, exist_ok=True)
+    objs = []
+    for src in ("tlq_c.f90", "tlq_hydro.f90"):
+        obj = os.path.join(mod, src.replace(".f90", ".o"))
+        cmd = [fc, "-O2", "-module-dir", mod, "-c", os.path.join(FORTRAN_DIR, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"flang failed on {src}:\n{r.stdout}")
+        objs.append(obj)
+    tmp = DRIVER + ".tmp"
+    cmd = [fc, *objs, "-L" + OUT, "-ltlq", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("flang link failed:\n" + r.stdout)
+    os.replace(tmp, DRIVER)
+    return DRIVER
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_fortran(verbose="-v" in sys.argv) or "flang not found: Fortran driver not built")

@@ -6,6 +6,15 @@ included), the form ``user_initconds`` fills (user/user.f90:22-52).
 
   sod_planar   3D planar shock tube along x, 128x32x32, gamma 1.4, FP64, HLLC,
                outflow, CFL 0.4, t = 0.2 (exact solution: oracle/riemann.py).
+  sod_diagonal the shock tube of the reference's test harness
+               (tests_utilities.jl:214-224, 407-445): the cube [-0.25, 0.25]^3,
+               rho, p = 4, 1 where x + y + z < 0 and 1, 0.1795 elsewhere,
+               gamma 5/3, CFL 0.8, eta 0.05, outflow, t = 0.12 (exact solution
+               solutions/sod_e1rpex.out along the diagonal).
+  sedov        the point explosion of the harness (tests_utilities.jl:225-234,
+               455-495): the cube [-5, 5]^3, rho = 1, E0 = 1e5 as uniform
+               pressure in r < 0.15, p = 1.5e-20 (gamma - 1) elsewhere,
+               gamma 5/3, CFL 0.8, eta 0.05, outflow, t = 0.1.
   blast        point explosion, gamma 5/3, rho = 1, p = 1e-5, E = 1 spread
                uniformly over the cells within 3 cells of the centre, outflow.
   wind         one stellar wind (radius 5 cells, 1000 km/s, 1e-6 Msun/yr) in a
@@ -59,6 +68,34 @@ def sod_planar(nx=128, ny=32, nz=32, precision=P.PREC_FP64) -> Tuple[P.HydroPara
 
 
 SOD_TFIN = 0.2
+
+
+def sod_diagonal(n=150, precision=P.PREC_FP64) -> Tuple[P.HydroParams, np.ndarray]:
+    p = P.HydroParams(nx=n, ny=n, nz=n, xl=-0.25, xr=0.25, yl=-0.25, yr=0.25, zl=-0.25,
+                      zr=0.25, gamma=5.0 / 3.0, cfl=0.8, eta=0.05, bc=(P.BC_OUTFLOW,) * 6,
+                      precision=precision)
+    x, y, z = centres(p)
+    left = (x[None, None, :] + y[None, :, None] + z[:, None, None]) < 0.0
+    rho = np.where(left, 4.0, 1.0)
+    pres = np.where(left, 1.0, 0.1795)
+    return p, cons_from_prims(p, rho, np.zeros(p.shape + (3,)), pres)
+
+
+SOD_DIAGONAL_TFIN = 0.12
+
+
+def sedov(n=200, precision=P.PREC_FP64) -> Tuple[P.HydroParams, np.ndarray]:
+    p = P.HydroParams(nx=n, ny=n, nz=n, xl=-5.0, xr=5.0, yl=-5.0, yr=5.0, zl=-5.0, zr=5.0,
+                      gamma=5.0 / 3.0, cfl=0.8, eta=0.05, bc=(P.BC_OUTFLOW,) * 6,
+                      precision=precision)
+    x, y, z = centres(p)
+    r = np.sqrt(x[None, None, :] ** 2 + y[None, :, None] ** 2 + z[:, None, None] ** 2)
+    pin = (1e5 / ((4.0 / 3.0) * P.PI * 0.15 ** 3)) * (p.gamma - 1.0)
+    pres = np.where(r < 0.15, pin, 1.5e-20 * (p.gamma - 1.0))
+    return p, cons_from_prims(p, np.ones(p.shape), np.zeros(p.shape + (3,)), pres)
+
+
+SEDOV_TFIN = 0.1
 
 
 def blast(n=256, precision=P.PREC_FP64, solver=P.SOLVER_HLLC, energy=1.0, rho0=1.0, p0=1e-5,
