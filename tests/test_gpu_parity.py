@@ -144,3 +144,100 @@ def test_fp32_fluxes_with_subnormal_differences():
     for f32, f64 in zip(out[P.PREC_FP32], out[P.PREC_FP64]):
         assert np.isfinite(f32).all()
         assert max_rel(f32, f64) <= 1e-5
+
+
+def _oracle_steps(p, u0, n):
+    import oracle
+    o = oracle.Oracle(p, table=cooling_table() if p.cooling else None)
+    o.set_cons(u0)
+    for _ in range(n):
+        o.advance()
+    return o
+
+
+@pytest.mark.parametrize("case", ["hydro_outflow", "bc_mixed", "periodic", "limiter3"])
+def test_fused_hll_fp64_matches_oracle(case):
+    # HLL.f90 is not part of the reference build; the HLL oracle is pinned by
+    # the Riemann-problem properties of tests/test_oracle_pins.py
+    p, u0, _, _ = load_ref(case)
+    p = p.replace(solver=P.SOLVER_HLL)
+    o = _oracle_steps(p, u0, 3)
+    with _core(p) as h:
+        h.set_state(u0)
+        h.advance(3)
+        w = prims_from_cons(h.download("U"), p.gamma)
+    assert max_rel(interior(w), interior(o.prim)) <= 1e-12, case
+
+
+def test_routines_hll_fp64_match_oracle():
+    p, u0, _, _ = load_ref("hydro_outflow")
+    p = p.replace(solver=P.SOLVER_HLL)
+    o = _oracle_steps(p, u0, 2)
+    with _core(p) as h:
+        h.upload("U", u0)
+        h.borders("U", 1)
+        h.calc_prims("U")
+        t, it = 0.0, 0
+        for _ in range(2):
+            dt = h.time_step()
+            h.fluxes("U", 1)
+            h.godunov("UPP", dt / 2.0)
+            h.borders("UPP", 2)
+            h.calc_prims("UPP")
+            h.fluxes("UPP", 2)
+            h.godunov("UP", dt)
+            h.borders("UP", 1)
+            t, it = h.step(t, it, dt)
+            h.calc_prims("U")
+        w = h.download("UPrim")
+    assert max_rel(interior(w), interior(o.prim)) <= 1e-12
+
+
+def _gravity_problem(precision):
+    from tlaloque_amd import problems
+    return problems.clumpy(n=24, precision=precision, cooling=P.COOL_OFF)
+
+
+def test_point_mass_gravity_fp64_matches_oracle():
+    p, u0 = _gravity_problem(P.PREC_FP64)
+    assert p.masses and p.softening > 0
+    o = _oracle_steps(p, u0, 3)
+    with _core(p) as h:
+        h.set_state(u0)
+        h.advance(3)
+        w = prims_from_cons(h.download("U"), p.gamma)
+    assert max_rel(interior(w), interior(o.prim)) <= 1e-12
+    # per routine: godunov adds dt S from the device's own source kernel
+    with _core(p) as h:
+        h.upload("U", u0)
+        h.borders("U", 1)
+        h.calc_prims("U")
+        t, it = 0.0, 0
+        for _ in range(3):
+            dt = h.time_step()
+            h.fluxes("U", 1)
+            h.godunov("UPP", dt / 2.0)
+            h.borders("UPP", 2)
+            h.calc_prims("UPP")
+            h.fluxes("UPP", 2)
+            h.godunov("UP", dt)
+            h.borders("UP", 1)
+            t, it = h.step(t, it, dt)
+            h.calc_prims("U")
+        w2 = h.download("UPrim")
+    assert max_rel(interior(w2), interior(o.prim)) <= 1e-12
+
+
+def test_point_mass_gravity_fp32_close_to_oracle():
+    p, u0 = _gravity_problem(P.PREC_FP32)
+    o = _oracle_steps(p.replace(precision=P.PREC_FP64), u0, 3)
+    with _core(p) as h:
+        h.set_state(u0)
+        h.advance(3)
+        w = prims_from_cons(h.download("U"), p.gamma)
+    ref = interior(o.prim)
+    got = interior(w)
+    assert l1_rel(got[..., 0], ref[..., 0]) <= 1e-3
+    assert l1_rel(got[..., 4], ref[..., 4]) <= 1e-3
+    for e in (1, 2, 3):
+        assert l1_rel(got[..., e], ref[..., e]) <= 1e-3
