@@ -511,6 +511,19 @@ __device__ __forceinline__ bool winds_at_mask(const Problem &P, unsigned long lo
     return false;
 }
 
+// Replaces v by the state of the last wind of `mask` that covers (i, j, k).
+// The wind state passes through a private array only inside the branch, so
+// v itself stays in registers (no scratch traffic when no wind is near).
+__device__ __forceinline__ bool wind_override(const Problem &P, unsigned long long mask, int i,
+                                              int j, int k, double v[5]) {
+    if (!mask) return false;
+    double o[5];
+    if (!winds_at_mask(P, mask, i, j, k, o)) return false;
+#pragma unroll
+    for (int e = 0; e < 5; ++e) v[e] = o[e];
+    return true;
+}
+
 // Order-preserving map of a non-negative double onto uint64 for atomicMax.
 __device__ __forceinline__ unsigned long long ord64(double x) {
     return (unsigned long long)__double_as_longlong(x);

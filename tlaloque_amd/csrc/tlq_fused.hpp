@@ -91,11 +91,9 @@ __device__ __forceinline__ void write_face_ghosts(const Problem &P, unsigned lon
     const Grid &g = P.g;
     auto put = [&](int gi, int gj, int gk, int comp, bool neg) {
         double w[5];
-        if (!winds_at_mask(P, wmask, gi, gj, gk, w)) {
 #pragma unroll
-            for (int e = 0; e < 5; ++e) w[e] = v[e];
-            if (neg) w[comp] = -w[comp];
-        }
+        for (int e = 0; e < 5; ++e) w[e] = (neg && e == comp) ? -v[e] : v[e];
+        wind_override(P, wmask, gi, gj, gk, w);
         const long long c = g.idx(gi, gj, gk);
 #pragma unroll
         for (int e = 0; e < 5; ++e) out[e * g.nc + c] = (R)w[e];
@@ -365,10 +363,9 @@ __global__ __launch_bounds__(TX *TY) void k_stage1(const Problem *__restrict__ P
             }
             const unsigned long long wmask = s_wmask;
             double o[5];
-            if (!winds_at_mask(P, wmask, i, j, k, o)) {
 #pragma unroll
-                for (int e = 0; e < 5; ++e) o[e] = v[e];
-            }
+            for (int e = 0; e < 5; ++e) o[e] = v[e];
+            wind_override(P, wmask, i, j, k, o);
             const long long c = g.idx(i, j, k);
 #pragma unroll
             for (int e = 0; e < 5; ++e) Uh[e * g.nc + c] = (R)o[e];
@@ -606,10 +603,9 @@ __global__ __launch_bounds__(TX *TY) void k_stage2(const Problem *__restrict__ P
             // ghosts of U^{n+1} = boundary fill of the pre-wind UP, then winds
             write_face_ghosts(P, wmask, Unew, i, j, k, v, false);
             double o[5];
-            if (!winds_at_mask(P, wmask, i, j, k, o)) {
 #pragma unroll
-                for (int e = 0; e < 5; ++e) o[e] = v[e];
-            }
+            for (int e = 0; e < 5; ++e) o[e] = v[e];
+            wind_override(P, wmask, i, j, k, o);
             // step: U^{n+1} = UP + eta * Lap(U^n)  (HydroCore.f90:184-193)
             R oq[5];
 #pragma unroll
