@@ -62,7 +62,7 @@ struct tlq_ctx {
     unsigned long long *scratch_max = nullptr;  // per-routine time_step maxima
     unsigned long long *scratch_cnt = nullptr;
     bool fused_ready = false;
-    Tiling tiling{};
+    Tiling tiling1{}, tiling2{};  // tiles of the stage-1 and stage-2 kernels
     // timing
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool ktiming = false;
@@ -218,12 +218,12 @@ template <class R> int do_advance(tlq_ctx *c, int nsteps) {
         const int par = c->cur;
         {
             KTimer kt(c, 0);
-            HIPCHK(Launch<R>::stage1(c->stream, c->dprob, c->tiling, cp<R>(c->u[par]), mp<R>(c->uh),
+            HIPCHK(Launch<R>::stage1(c->stream, c->dprob, c->tiling1, cp<R>(c->u[par]), mp<R>(c->uh),
                                      c->st, par, c->host.solver));
         }
         {
             KTimer kt(c, 1);
-            HIPCHK(Launch<R>::stage2(c->stream, c->dprob, c->tiling, cp<R>(c->u[par]),
+            HIPCHK(Launch<R>::stage2(c->stream, c->dprob, c->tiling2, cp<R>(c->u[par]),
                                      cp<R>(c->uh), mp<R>(c->u[par ^ 1]), c->st, par,
                                      c->host.limiter, c->host.solver));
         }
@@ -469,22 +469,17 @@ int tlq_create(const tlq_params *p, tlq_ctx **out) {
     hipEventCreate(&c->ev0);
     hipEventCreate(&c->ev1);
 
-    int tx, ty, zc;
     if (c->prec == TLQ_FP32) {
-        tx = TileShape<float>::TX;
-        ty = TileShape<float>::TY;
-        zc = TileShape<float>::ZC;
+        using S1 = TileShape<float>::S1;
+        using S2 = TileShape<float>::S2;
+        c->tiling1 = make_tiling(p->nx, p->ny, p->nz, S1::TX, S1::TY, S1::ZC);
+        c->tiling2 = make_tiling(p->nx, p->ny, p->nz, S2::TX, S2::TY, S2::ZC);
     } else {
-        tx = TileShape<double>::TX;
-        ty = TileShape<double>::TY;
-        zc = TileShape<double>::ZC;
+        using S1 = TileShape<double>::S1;
+        using S2 = TileShape<double>::S2;
+        c->tiling1 = make_tiling(p->nx, p->ny, p->nz, S1::TX, S1::TY, S1::ZC);
+        c->tiling2 = make_tiling(p->nx, p->ny, p->nz, S2::TX, S2::TY, S2::ZC);
     }
-    Tiling &T = c->tiling;
-    T.nbx = (p->nx + tx - 1) / tx;
-    T.nby = (p->ny + ty - 1) / ty;
-    T.nbz = (p->nz + zc - 1) / zc;
-    T.ntiles = T.nbx * T.nby * T.nbz;
-    T.per_xcd = (T.ntiles + 7) / 8;
     int rc = push_problem(c);
     if (rc) {
         tlq_destroy(c);

@@ -243,8 +243,8 @@ __device__ __forceinline__ HaloTask halo_task2(int h) {
 }
 
 // ---------------------------------------------------------------------------
-template <class R, int TX, int TY, int ZC, int SOL>
-__global__ __launch_bounds__(TX *TY) void k_stage1(const Problem *__restrict__ Pp, Tiling T,
+template <class R, int TX, int TY, int ZC, int SOL, int WAVES>
+__global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restrict__ Pp, Tiling T,
                                                     const R *__restrict__ U,
                                                     R *__restrict__ Uh,
                                                     StepState *__restrict__ st, int parity) {
@@ -381,8 +381,8 @@ __global__ __launch_bounds__(TX *TY) void k_stage1(const Problem *__restrict__ P
 }
 
 // ---------------------------------------------------------------------------
-template <class R, int TX, int TY, int ZC, int LIM, int SOL>
-__global__ __launch_bounds__(TX *TY) void k_stage2(const Problem *__restrict__ Pp, Tiling T,
+template <class R, int TX, int TY, int ZC, int LIM, int SOL, int WAVES>
+__global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restrict__ Pp, Tiling T,
                                                     const R *__restrict__ U,
                                                     const R *__restrict__ Uh,
                                                     R *__restrict__ Unew,

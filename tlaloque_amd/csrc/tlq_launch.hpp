@@ -14,18 +14,38 @@ struct Tiling;
 
 // Tile shapes of the fused kernels per precision.
 template <class R> struct TileShape;
-#ifndef TLQ_F32_TY
-#define TLQ_F32_TY 8
+// Tile shape of each fused kernel: TX x TY columns, ZC planes, and the
+// minimum number of waves per SIMD the kernel is compiled for (its register
+// budget follows from it).  Stage 1 (first-order fluxes, fewer live values)
+// runs small blocks at three waves per SIMD; stage 2 is register-bound at two.
+template <int TX_, int TY_, int ZC_, int WAVES_> struct Tile {
+    static constexpr int TX = TX_, TY = TY_, ZC = ZC_, WAVES = WAVES_;
+};
+#ifndef TLQ_F32_S1
+#define TLQ_F32_S1 32, 8, 64, 3
 #endif
-#ifndef TLQ_F64_TY
-#define TLQ_F64_TY 8
+#ifndef TLQ_F32_S2
+#define TLQ_F32_S2 64, 8, 64, 1
 #endif
 template <> struct TileShape<float> {
-    static constexpr int TX = 64, TY = TLQ_F32_TY, ZC = 64;
+    using S1 = Tile<TLQ_F32_S1>;
+    using S2 = Tile<TLQ_F32_S2>;
 };
 template <> struct TileShape<double> {
-    static constexpr int TX = 32, TY = TLQ_F64_TY, ZC = 64;
+    using S1 = Tile<32, 8, 64, 1>;
+    using S2 = Tile<32, 8, 64, 1>;
 };
+
+// Tiles of a mesh for one tile shape.
+inline Tiling make_tiling(int nx, int ny, int nz, int tx, int ty, int zc) {
+    Tiling T;
+    T.nbx = (nx + tx - 1) / tx;
+    T.nby = (ny + ty - 1) / ty;
+    T.nbz = (nz + zc - 1) / zc;
+    T.ntiles = T.nbx * T.nby * T.nbz;
+    T.per_xcd = (T.ntiles + 7) / 8;
+    return T;
+}
 
 template <class R> struct Launch {
     static hipError_t calc_prims(hipStream_t s, const Problem *P, const R *U, R *W, long long ncell);

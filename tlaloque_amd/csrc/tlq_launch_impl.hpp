@@ -100,13 +100,13 @@ hipError_t Launch<R>::check(hipStream_t s, const Problem *P, const R *U, unsigne
 template <class R>
 hipError_t Launch<R>::stage1(hipStream_t s, const Problem *P, const Tiling &T, const R *U, R *Uh,
                              StepState *st, int parity, int solver) {
-    using TS = TileShape<R>;
+    using TS = typename TileShape<R>::S1;
     const dim3 grid(8 * T.per_xcd), block(TS::TX * TS::TY);
     if (solver == SOLVER_HLL)
-        hipLaunchKernelGGL((k_stage1<R, TS::TX, TS::TY, TS::ZC, SOLVER_HLL>), grid, block, 0, s, P,
+        hipLaunchKernelGGL((k_stage1<R, TS::TX, TS::TY, TS::ZC, SOLVER_HLL, TS::WAVES>), grid, block, 0, s, P,
                            T, U, Uh, st, parity);
     else
-        hipLaunchKernelGGL((k_stage1<R, TS::TX, TS::TY, TS::ZC, SOLVER_HLLC>), grid, block, 0, s,
+        hipLaunchKernelGGL((k_stage1<R, TS::TX, TS::TY, TS::ZC, SOLVER_HLLC, TS::WAVES>), grid, block, 0, s,
                            P, T, U, Uh, st, parity);
     return hipGetLastError();
 }
@@ -114,24 +114,24 @@ template <class R>
 hipError_t Launch<R>::stage2(hipStream_t s, const Problem *P, const Tiling &T, const R *U,
                              const R *Uh, R *Unew, StepState *st, int parity, int limiter,
                              int solver) {
-    using TS = TileShape<R>;
+    using TS = typename TileShape<R>::S2;
     const dim3 grid(8 * T.per_xcd), block(TS::TX * TS::TY);
     // van Leer (the reference default) is compiled in; other limiters are
     // selected at run time inside one kernel.
     const bool vl = limiter == LIM_VANLEER;
     if (solver == SOLVER_HLL) {
         if (vl)
-            hipLaunchKernelGGL((k_stage2<R, TS::TX, TS::TY, TS::ZC, LIM_VANLEER, SOLVER_HLL>), grid,
+            hipLaunchKernelGGL((k_stage2<R, TS::TX, TS::TY, TS::ZC, LIM_VANLEER, SOLVER_HLL, TS::WAVES>), grid,
                                block, 0, s, P, T, U, Uh, Unew, st, parity);
         else
-            hipLaunchKernelGGL((k_stage2<R, TS::TX, TS::TY, TS::ZC, 0, SOLVER_HLL>), grid, block, 0,
+            hipLaunchKernelGGL((k_stage2<R, TS::TX, TS::TY, TS::ZC, 0, SOLVER_HLL, TS::WAVES>), grid, block, 0,
                                s, P, T, U, Uh, Unew, st, parity);
     } else {
         if (vl)
-            hipLaunchKernelGGL((k_stage2<R, TS::TX, TS::TY, TS::ZC, LIM_VANLEER, SOLVER_HLLC>),
+            hipLaunchKernelGGL((k_stage2<R, TS::TX, TS::TY, TS::ZC, LIM_VANLEER, SOLVER_HLLC, TS::WAVES>),
                                grid, block, 0, s, P, T, U, Uh, Unew, st, parity);
         else
-            hipLaunchKernelGGL((k_stage2<R, TS::TX, TS::TY, TS::ZC, 0, SOLVER_HLLC>), grid, block, 0,
+            hipLaunchKernelGGL((k_stage2<R, TS::TX, TS::TY, TS::ZC, 0, SOLVER_HLLC, TS::WAVES>), grid, block, 0,
                                s, P, T, U, Uh, Unew, st, parity);
     }
     return hipGetLastError();
