@@ -17,8 +17,10 @@ included), the form ``user_initconds`` fills (user/user.f90:22-52).
                gamma 5/3, CFL 0.8, eta 0.05, outflow, t = 0.1.
   blast        point explosion, gamma 5/3, rho = 1, p = 1e-5, E = 1 spread
                uniformly over the cells within 3 cells of the centre, outflow.
-  wind         one stellar wind (radius 5 cells, 1000 km/s, 1e-6 Msun/yr) in a
-               uniform n = 1 cm^-3, T = 1e4 K medium, cgs, tabulated cooling.
+  wind         one stellar wind (radius 5 cells, 1000 km/s, 1e-6 Msun/yr, wind
+               T = 1e4 K, mu = 1) at the box centre (a cell corner for even n,
+               so no cell sits at r = 0) in a uniform n = 1 cm^-3, T = 1e4 K
+               medium, cgs, tabulated cooling.
   clumpy       lognormal density (mean n = 1 cm^-3, sigma_ln rho = 1) from a
                seeded Gaussian random field with P(k) ~ k^-4, velocities of the
                same spectrum at RMS Mach 2, T = 1e4 K, a central 1e4 Msun point
@@ -43,6 +45,16 @@ def cons_from_prims(p: P.HydroParams, rho, vel, pres) -> np.ndarray:
     u[..., 0] = rho
     u[..., 1:4] = rho[..., None] * vel
     u[..., 4] = 0.5 * rho * (vel ** 2).sum(-1) + pres / (p.gamma - 1.0)
+    return u
+
+
+def uniform_state(p: P.HydroParams, rho: float, pres: float) -> np.ndarray:
+    """A uniform medium at rest, built in place (no temporaries of the state's
+    size: the 1024^3 state alone is 43 GB)."""
+    u = np.empty(p.shape + (5,), dtype=np.float64)
+    u[..., 0] = rho
+    u[..., 1:4] = 0.0
+    u[..., 4] = pres / (p.gamma - 1.0)
     return u
 
 
@@ -172,14 +184,15 @@ def clumpy(n=512, precision=P.PREC_FP32, seed=2024, box_pc=10.0, cooling=P.COOL_
 def wind(n=256, precision=P.PREC_FP32, box_pc=1.0, cooling=P.COOL_EXACT, temp_wind=1e4):
     L = box_pc * P.PC
     dx = L / n
-    w = P.Wind(xc=0.5 * dx, yc=0.5 * dx, zc=0.5 * dx, vx=0.0, vy=0.0, vz=0.0, radius=5.0 * dx,
+    # the wind state is singular at r = 0 (winds.f90:72): the centre must not
+    # be a cell centre -- the box centre is a cell corner for even n
+    c = 0.0 if n % 2 == 0 else 0.5 * dx
+    w = P.Wind(xc=c, yc=c, zc=c, vx=0.0, vy=0.0, vz=0.0, radius=5.0 * dx,
                mdot=1e-6 * P.MSUN / P.YEAR, vinf=1000.0 * P.KMS, temp=temp_wind, mu=1.0)
     p = P.HydroParams(nx=n, ny=n, nz=n, xl=-L / 2, xr=L / 2, yl=-L / 2, yr=L / 2, zl=-L / 2,
                       zr=L / 2, gamma=5.0 / 3.0, cfl=0.4, bc=(P.BC_OUTFLOW,) * 6,
                       precision=precision, cooling=cooling, winds=[w])
-    rho = np.full(p.shape, 1.0 * p.mu0 * P.AMU)
-    pres = np.full(p.shape, 1.0 * P.KB * 1e4)
-    return p, cons_from_prims(p, rho, np.zeros(p.shape + (3,)), pres)
+    return p, uniform_state(p, 1.0 * p.mu0 * P.AMU, 1.0 * P.KB * 1e4)
 
 
 def cluster(n=1024, precision=P.PREC_FP32, seed=7, box_pc=4.0, cooling=P.COOL_SURROGATE,
@@ -203,6 +216,4 @@ def cluster(n=1024, precision=P.PREC_FP32, seed=7, box_pc=4.0, cooling=P.COOL_SU
                       zr=L / 2, gamma=5.0 / 3.0, cfl=0.4, bc=(P.BC_OUTFLOW,) * 6,
                       precision=precision, cooling=cooling, winds=winds, masses=masses,
                       softening=2.0 * dx)
-    rho = np.full(p.shape, 10.0 * p.mu0 * P.AMU)
-    pres = np.full(p.shape, 10.0 * P.KB * 100.0)
-    return p, cons_from_prims(p, rho, np.zeros(p.shape + (3,)), pres)
+    return p, uniform_state(p, 10.0 * p.mu0 * P.AMU, 10.0 * P.KB * 100.0)
