@@ -16,8 +16,9 @@ error of the FP32 path against the FP64 path on the same problem, and the
 cooling paths compared on the same mesh: the reference's one-step formula
 fused into stage 2 (the headline value), the learned surrogate with its
 per-cell fallback (trained in this run on the GPU), and the converged
-integrator (the surrogate's target and fallback), on the workload as given
-and on a strongly cooling variant of it (T = 3e5 K).
+integrator (the surrogate's target and fallback), on the workload as given,
+at T = 3e5 K, and at T = 3e5 K with every density times 100 (strongly
+cooling: dt/t_cool of order 1-10, hundreds of integrator substeps per cell).
 """
 from __future__ import annotations
 
@@ -48,7 +49,8 @@ def log(*a):
 
 def timed_run(p, u0, table, warmup, steps, theta=None):
     """Advance a fresh context over `steps` timed steps; returns
-    (zone-updates/s, ms per step, per-kernel ms, fallback fraction, bad cells)."""
+    (zone-updates/s, ms per step, per-kernel ms, (queued cells, fallbacks) of
+    the last step, bad cells)."""
     from tlaloque_amd.hydro import HydroCore
     with HydroCore(p, cooling_table=table) as h:
         if theta is not None:
@@ -62,22 +64,24 @@ def timed_run(p, u0, table, warmup, steps, theta=None):
         ms = h.timer_stop()
         ks = h.kernel_stats()
         h.kernel_timing(False)
-        frac = h.fallback_fraction()
+        counts = h.cooling_counts()
         bad = h.check_state()
     kms = {k: v[0] / max(v[1], 1) for k, v in ks.items() if v[1]}
-    return p.zones * steps / (ms / 1e3), ms / steps, kms, frac, bad
+    return p.zones * steps / (ms / 1e3), ms / steps, kms, counts, bad
 
 
-def heated(p, u0, temp):
-    """The same state with the thermal energy scaled to temperature `temp`
-    (every cell of the clumpy set-up starts at one temperature)."""
+def heated(p, u0, temp, dens=1.0):
+    """The same state at temperature `temp` (every cell of the clumpy set-up
+    starts at one temperature) with all densities scaled by `dens` (the
+    velocities, hence the hydro time step, unchanged)."""
     from tlaloque_amd import params as P
     u = u0.copy()
     rho = u[..., 0]
     ek = 0.5 * (u[..., 1] ** 2 + u[..., 2] ** 2 + u[..., 3] ** 2) / rho
     eth = u[..., 4] - ek
     t0 = float(np.median((p.gamma - 1.0) * eth * p.mu0 * P.AMU / (rho * P.KB)))
-    u[..., 4] = ek + eth * (temp / t0)
+    u[..., 4] = (ek + eth * (temp / t0)) * dens
+    u[..., :4] *= dens
     return u
 
 
@@ -93,25 +97,35 @@ def cooling_modes(p, u0, table, warmup, steps):
                                   "heldout_rms_dex": rep.heldout_rms,
                                   "heldout_rel_median": rep.heldout_rel_median,
                                   "heldout_rel_p90": rep.heldout_rel_p90}}
-    for label, u in (("T=1e4K", u0), ("T=3e5K", heated(p, u0, 3e5))):
+    variants = (("T=1e4K", lambda: u0), ("T=3e5K", lambda: heated(p, u0, 3e5)),
+                ("T=3e5K,n*100", lambda: heated(p, u0, 3e5, 100.0)))
+    for label, make in variants:
+        u = make()
         res = {}
         for name, mode in (("reference_one_step", P.COOL_EXACT),
                            ("surrogate", P.COOL_SURROGATE),
                            ("converged_integrator", P.COOL_SUBCYCLED)):
             if label == "T=1e4K" and mode == P.COOL_EXACT:
                 continue  # the headline run
-            zps, ms, kms, frac, bad = timed_run(p.replace(cooling=mode), u, table, warmup, steps,
-                                                theta if mode == P.COOL_SURROGATE else None)
+            zps, ms, kms, (nq, nf), bad = timed_run(p.replace(cooling=mode), u, table, warmup,
+                                                     steps,
+                                                     theta if mode == P.COOL_SURROGATE else None)
             res[name] = {"zone_updates_per_s": zps, "ms_per_step": ms,
                          "cooling_pass_ms": kms.get("cooling"), "nonfinite_cells": bad}
+            if mode != P.COOL_EXACT:
+                res[name]["queued_fraction"] = nq / p.zones
             if mode == P.COOL_SURROGATE:
-                res[name]["fallback_fraction"] = frac
+                res[name]["fallback_fraction"] = nf / nq if nq else 0.0
             log(f"bench: {label} {name}: {zps:.4g} zone-updates/s, {ms:.3f} ms/step, "
-                f"kernels {kms}" + (f", fallback {frac:.4f}" if mode == P.COOL_SURROGATE else ""))
+                f"kernels {kms}, queued {nq}, fallbacks {nf}")
         if "surrogate" in res and "converged_integrator" in res:
             res["surrogate_speedup_vs_converged"] = (res["surrogate"]["zone_updates_per_s"] /
                                                      res["converged_integrator"]["zone_updates_per_s"])
+            cs, ci = res["surrogate"]["cooling_pass_ms"], res["converged_integrator"]["cooling_pass_ms"]
+            if cs and ci:
+                res["cooling_pass_speedup"] = ci / cs
         out[label] = res
+        del u
     return out
 
 

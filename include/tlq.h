@@ -146,6 +146,10 @@ int tlq_check_state(tlq_ctx *ctx, long long *bad);
  * one substep of cooling.f90:67-112 is the converged answer), the fraction
  * that took the exact-integrator fallback. */
 int tlq_fallback_fraction(tlq_ctx *ctx, double *frac);
+/* The same step's counts: cells handed to the network (mode 2) or to the
+ * converged integrator (mode 3) -- those at T >= T_min with dt > 0.01 t_cool --
+ * and, of them, cells that took the fallback (mode 2). */
+int tlq_cooling_counts(tlq_ctx *ctx, long long *queued, long long *fallbacks);
 
 /* Learned cooling surrogate (MLP 3 -> 64 -> 64 -> 1, GELU, bf16 weights,
  * FP32 accumulation; TLQ_MLP_NPARAMS parameters laid out as W1[64][3], b1[64],

@@ -77,7 +77,10 @@ struct tlq_ctx {
     long long sur_n = 0;
     int sur_t = 0;
     bool sur_ready = false;
-    unsigned long long *cool_counts = nullptr;  // [cooled cells, fallback cells]
+    unsigned *cool_queue = nullptr;  // cells queued for the dense cooling pass
+    unsigned *cool_fqueue = nullptr; // surrogate cells queued for the fallback
+    unsigned *cool_seg = nullptr;    // per segment: cells queued, cells that fell back
+    unsigned *cool_fall = nullptr;
 };
 
 namespace {
@@ -229,9 +232,9 @@ template <class R> int do_advance(tlq_ctx *c, int nsteps) {
         }
         if (c->host.cooling >= TLQ_COOL_SURROGATE) {
             KTimer kt(c, 3);
-            HIPCHK(hipMemsetAsync(c->cool_counts, 0, 2 * sizeof(unsigned long long), c->stream));
             HIPCHK(sur_cool_pass<R>(c->stream, c->dprob, c->g, mp<R>(c->u[par ^ 1]), c->sur_thq,
-                                    c->st, par, c->cool_counts, c->host.cooling));
+                                    c->st, par, c->cool_queue, c->cool_seg, c->cool_fqueue,
+                                    c->cool_fall, c->host.cooling));
         }
         c->cur ^= 1;
     }
@@ -453,8 +456,7 @@ int tlq_create(const tlq_params *p, tlq_ctx **out) {
     if (hipMalloc(&c->dprob, sizeof(Problem)) != hipSuccess ||
         hipMalloc(&c->st, sizeof(StepState)) != hipSuccess ||
         hipMalloc(&c->scratch_max, 8 * sizeof(unsigned long long)) != hipSuccess ||
-        hipMalloc(&c->scratch_cnt, sizeof(unsigned long long)) != hipSuccess ||
-        hipMalloc(&c->cool_counts, 2 * sizeof(unsigned long long)) != hipSuccess)
+        hipMalloc(&c->scratch_cnt, sizeof(unsigned long long)) != hipSuccess)
         return bail(TLQ_ERR_NOMEM, "hipMalloc failed");
     const size_t bytes = (size_t)5 * g.nc * c->esize;
     for (int q = 0; q < 2; ++q) {
@@ -464,6 +466,21 @@ int tlq_create(const tlq_params *p, tlq_ctx **out) {
     if (hipMalloc(&c->uh, bytes) != hipSuccess) return bail(TLQ_ERR_NOMEM, "hipMalloc failed (U1/2)");
     hipMemsetAsync(c->uh, 0, bytes, c->stream);
     c->bufs[TLQ_BUF_UPP] = c->uh;
+    if (p->cooling >= TLQ_COOL_SURROGATE) {
+        // queue of the dense cooling pass: one 32-bit cell offset per interior cell
+        if (g.nc >= (1ll << 32)) return bail(TLQ_ERR_UNSUPPORTED, "mesh too large for the cooling queue");
+        const long long ncell = (long long)p->nx * p->ny * p->nz;
+        const size_t nb = (size_t)cool_blocks(ncell);
+        if (hipMalloc(&c->cool_queue, (size_t)cool_queue_len(ncell) * sizeof(unsigned)) != hipSuccess ||
+            hipMalloc(&c->cool_seg, nb * sizeof(unsigned)) != hipSuccess ||
+            hipMalloc(&c->cool_fall, nb * sizeof(unsigned)) != hipSuccess)
+            return bail(TLQ_ERR_NOMEM, "hipMalloc failed (cooling queue)");
+        if (p->cooling == TLQ_COOL_SURROGATE &&
+            hipMalloc(&c->cool_fqueue, (size_t)cool_queue_len(ncell) * sizeof(unsigned)) != hipSuccess)
+            return bail(TLQ_ERR_NOMEM, "hipMalloc failed (fallback queue)");
+        hipMemsetAsync(c->cool_seg, 0, nb * sizeof(unsigned), c->stream);
+        hipMemsetAsync(c->cool_fall, 0, nb * sizeof(unsigned), c->stream);
+    }
     StepState s0{};
     hipMemcpyAsync(c->st, &s0, sizeof s0, hipMemcpyHostToDevice, c->stream);
     hipEventCreate(&c->ev0);
@@ -506,7 +523,10 @@ int tlq_destroy(tlq_ctx *c) {
     if (c->st) hipFree(c->st);
     if (c->scratch_max) hipFree(c->scratch_max);
     if (c->scratch_cnt) hipFree(c->scratch_cnt);
-    if (c->cool_counts) hipFree(c->cool_counts);
+    if (c->cool_queue) hipFree(c->cool_queue);
+    if (c->cool_fqueue) hipFree(c->cool_fqueue);
+    if (c->cool_seg) hipFree(c->cool_seg);
+    if (c->cool_fall) hipFree(c->cool_fall);
     for (void *q : {(void *)c->sur_theta, (void *)c->sur_m, (void *)c->sur_v, (void *)c->sur_gout,
                     (void *)c->sur_thq, (void *)c->sur_X, (void *)c->sur_Y,
                     (void *)c->sur_partial, (void *)c->sur_loss})
@@ -715,13 +735,30 @@ int tlq_check_state(tlq_ctx *c, long long *bad) {
     return DISPATCH(c, do_check, bad);
 }
 
-int tlq_fallback_fraction(tlq_ctx *c, double *frac) {
+int tlq_cooling_counts(tlq_ctx *c, long long *queued, long long *fallbacks) {
     if (int rc = check_ctx(c)) return rc;
-    if (!frac) return fail(TLQ_ERR_ARG, "null output");
-    unsigned long long n[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(n, c->cool_counts, sizeof n, hipMemcpyDeviceToHost, c->stream));
+    if (!queued || !fallbacks) return fail(TLQ_ERR_ARG, "null output");
+    *queued = *fallbacks = 0;
+    if (!c->cool_seg) return TLQ_OK;
+    const size_t nb = (size_t)cool_blocks((long long)c->g.nx * c->g.ny * c->g.nz);
+    std::vector<unsigned> q(nb), f(nb);
+    HIPCHK(hipMemcpyAsync(q.data(), c->cool_seg, nb * sizeof(unsigned), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipMemcpyAsync(f.data(), c->cool_fall, nb * sizeof(unsigned), hipMemcpyDeviceToHost,
+                          c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    *frac = n[0] ? (double)n[1] / (double)n[0] : 0.0;
+    for (size_t b = 0; b < nb; ++b) {
+        *queued += q[b];
+        *fallbacks += f[b];
+    }
+    return TLQ_OK;
+}
+
+int tlq_fallback_fraction(tlq_ctx *c, double *frac) {
+    if (!frac) return fail(TLQ_ERR_ARG, "null output");
+    long long nq = 0, nf = 0;
+    if (int rc = tlq_cooling_counts(c, &nq, &nf)) return rc;
+    *frac = nq ? (double)nf / (double)nq : 0.0;
     return TLQ_OK;
 }
 

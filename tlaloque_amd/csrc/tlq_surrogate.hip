@@ -18,10 +18,21 @@ __device__ __forceinline__ unsigned short f2bf(float f) {  // round to nearest e
 }
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
-// GELU, tanh form: 0.5 x (1 + tanh(u/2)) = x sigmoid(u), u = 2 sqrt(2/pi) (x + 0.044715 x^3).
+// GELU, tanh form: 0.5 x (1 + tanh(u/2)) = x sigmoid(u), u = 2 sqrt(2/pi) (x + 0.044715 x^3);
+// exp(-u) = 2^(x (A + B x^2)) with the constants folded.
 __device__ __forceinline__ float gelu(float x) {
-    const float u = 1.5957691216057308f * (x + 0.044715f * x * x * x);
-    return x * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-1.4426950408889634f * u));
+    constexpr float A = -1.5957691216057308f * 1.4426950408889634f;
+    constexpr float B = A * 0.044715f;
+    const float e = __builtin_amdgcn_exp2f(x * __builtin_fmaf(B, x * x, A));
+    return x * __builtin_amdgcn_rcpf(1.f + e);
+}
+
+// Four floats to bf16, rounded to nearest even (v_cvt_pk_bf16_f32).
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 b16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ short2 bf16_pair(float a, float b) {
+    const b16x2 v = __builtin_convertvector(f32x2{a, b}, b16x2);
+    return __builtin_bit_cast(short2, v);
 }
 __device__ __forceinline__ float gelu_d(float x) {
     const float x2 = x * x;
@@ -247,6 +258,75 @@ __device__ __forceinline__ void normalise(const float x[3], float xn[3]) {
     xn[2] = (x[2] - C2) * (1.f / S2);
 }
 
+// FP32 copies of the small parameter blocks (bf16-rounded), for inference.
+struct MlpSmall {
+    float w1[3 * H], b1[H], b2[H], w3[H];
+};
+
+__device__ __forceinline__ void load_small_f(MlpSmall &sm, const unsigned short *__restrict__ thq,
+                                             int tid, int nthreads) {
+    for (int q = tid; q < 3 * H; q += nthreads) sm.w1[q] = bf2f(thq[OW1 + q]);
+    for (int q = tid; q < H; q += nthreads) {
+        sm.b1[q] = bf2f(thq[OB1 + q]);
+        sm.b2[q] = bf2f(thq[OB2 + q]);
+        sm.w3[q] = bf2f(thq[OW3 + q]);
+    }
+}
+
+// The network on the 64 samples of a wavefront, one per lane (xn: this
+// lane's normalised inputs); returns this lane's output.  Layer 2 runs
+// transposed, z2^T = W2 h1^T, so that the layer-1 activations each lane
+// computes -- units 16 kt + 4 (lane/16) + v of sample 16 st + lane%16 -- are
+// the MFMA B operand as they stand, and the layer-2 outputs -- units
+// 16 jt + 4 (lane/16) + v of the same samples -- feed layer 3 in registers:
+// no activation goes through LDS.  The four lanes that share a sample hold
+// disjoint units of it; two cross-lane sums finish layer 3.  Same roundings
+// as the training forward pass (bf16 h1 and weights, FP32 accumulation).
+// Every lane of the wavefront must call it (the MFMA and shuffles are
+// wavefront-wide).
+__device__ __forceinline__ float mlp_wave(const MlpSmall &sm, const bf16x4 w2a[4][4], float b3,
+                                          int lane, const float xn[3]) {
+    const int q = lane >> 4;
+    float r = 0.f;
+    // one sample tile at a time keeps only its four activation fragments live
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+        float xs[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xs[c] = __shfl(xn[c], 16 * st + (lane & 15));
+        bf16x4 hb[4];  // [kt]
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) {
+            float h[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = 16 * kt + 4 * q + v;
+                const float z = sm.b1[i] + sm.w1[3 * i] * xs[0] + sm.w1[3 * i + 1] * xs[1] +
+                                sm.w1[3 * i + 2] * xs[2];
+                h[v] = gelu(z);
+            }
+            const short2 lo = bf16_pair(h[0], h[1]), hi = bf16_pair(h[2], h[3]);
+            hb[kt] = bf16x4{lo.x, lo.y, hi.x, hi.y};
+        }
+        float o = 0.f;
+#pragma unroll
+        for (int jt = 0; jt < 4; ++jt) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kt = 0; kt < 4; ++kt) acc = mfma16(w2a[kt][jt], hb[kt], acc);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int j = 16 * jt + 4 * q + v;
+                o += sm.w3[j] * gelu(acc[v] + sm.b2[j]);
+            }
+        }
+        o += __shfl_xor(o, 16);
+        o += __shfl_xor(o, 32);
+        if (st == q) r = o;
+    }
+    return b3 + r;
+}
+
 // ---------------------------------------------------------------------------
 // Training gradient of the mean squared error over B samples starting at
 // `off`: every block (one wavefront) handles 64-sample chunks and writes its
@@ -435,12 +515,12 @@ __global__ void k_quantize(const float *__restrict__ theta, unsigned short *__re
 __global__ __launch_bounds__(64) void k_mlp_eval(const unsigned short *__restrict__ thq,
                                                  const float *__restrict__ X, long long n,
                                                  float *__restrict__ out) {
-    __shared__ MlpSharedInfer sh;
+    __shared__ MlpSmall sm;
     const int lane = threadIdx.x;
-    load_small(sh, thq, lane);
+    load_small_f(sm, thq, lane, 64);
     const float b3 = bf2f(thq[OB3]);
-    bf16x4 wf[4][4];
-    load_w2_fwd(thq, lane, wf);
+    bf16x4 w2a[4][4];
+    load_w2_fwd(thq, lane, w2a);
     __syncthreads();
     for (long long base = (long long)blockIdx.x * 64; base < n; base += (long long)gridDim.x * 64) {
         const long long s = base + lane;
@@ -452,166 +532,283 @@ __global__ __launch_bounds__(64) void k_mlp_eval(const unsigned short *__restric
         }
         float xn[3];
         normalise(x, xn);
-        layer1(sh, lane, xn, false);
-        __syncthreads();
-        layer2(sh, lane, wf);
-        __syncthreads();
-        float o = b3;
-        for (int j = 0; j < H; ++j) o += sh.w3[j] * gelu(sh.f[lane][j] + sh.b2[j]);
+        const float o = mlp_wave(sm, w2a, b3, lane, xn);
         if (s < n) out[s] = o;
-        __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
-// Cooling pass of the fused step (modes 2 and 3): one wavefront per 64 cells
-// of an x row.  Cells at T >= T_min with dt <= eps t_cool take one substep
-// (the converged integrator takes no more).  For the others mode 3 runs the
-// converged integrator and mode 2 evaluates the surrogate (skipped by a
-// wavefront with no such cell) and keeps its answer T1 when
+// Cooling of the fused step in modes 2 and 3, after stage 2, in two passes.
+//
+// k_cool_classify visits every interior cell.  Cells below T_min are left
+// alone; cells with dt <= eps t_cool take one substep (all the converged
+// integrator would take); their CFL maxima are formed here.  Every other
+// cell goes to a queue (one atomic per wavefront).
+//
+// k_cool_dense runs the queued cells in full wavefronts: mode 3 integrates
+// them to convergence; mode 2 evaluates the surrogate and keeps its answer T1
+// when
 //   * the inputs lie in the training box,
 //   * 10 K <= T1 <= T0 (cooling only lowers T, never below the floor), and
 //   * the exact cooling time from T0 to T1, t = k_B (Y(T0) - Y(T1)) / ((gamma-1) n),
 //     is within 0.05 min(dt, t_cool(T1)) of dt, so that ln T1 is within 5% of
-//     its change (at most 0.05) of the exact solution -- or, for T1 below the
-//     gate T_min (where the converged integrator stops), the gas reaches T_min
-//     within that time;
-// other cells take the exact converged integrator (the fallback).  Then the
-// CFL maxima of the cooled state (time_step of the next step).
+//     its change (at most 0.05) of the exact solution -- or, for T1 within 10%
+//     of the gate T_min, the gas reaches T_min within that time, and the
+//     answer is 0.995 T_min (the converged integrator stops within 1% below
+//     the gate);
+// other cells go to a fallback queue that a mode-3 dense pass integrates.
+// Then the CFL maxima of the cells finished here.
+//
 constexpr float COOL_EPS = 0.01f;
 constexpr int COOL_MAXSUB = 100000;
+// A surrogate answer at the gate is set just below it, in the 1% band where
+// the converged integrator's last substep ends (so the cell stays gated).
+constexpr double SNAP = 0.995;
 
-template <class R, int MODE>
-__global__ __launch_bounds__(64) void k_cool_pass(const Problem *__restrict__ Pp, R *__restrict__ U,
-                                                  const unsigned short *__restrict__ thq,
-                                                  StepState *__restrict__ st, int parity,
-                                                  unsigned long long *__restrict__ counts) {
-    __shared__ MlpSharedInfer sh;
+struct CellState {
+    double rho, ek, p, nd, T;
+    float rinv;  // 1/rho (FP32 state)
+};
+
+// Primitive state of one cell for cooling: FP64 arithmetic for an FP64
+// state; for an FP32 state the guarded fast path of calc_prims (pressure
+// recovery in FP64 with a refined reciprocal, the rest in FP32).
+template <class R>
+__device__ __forceinline__ CellState cell_state(const Problem &P, const R *__restrict__ U,
+                                                long long nc, long long c, double u[5]) {
+#pragma unroll
+    for (int e = 0; e < 5; ++e) u[e] = (double)U[e * nc + c];
+    CellState s;
+    s.rho = u[0];
+    if constexpr (sizeof(R) == 8) {
+        s.ek = 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) / s.rho;
+        s.p = (P.gamma - 1.0) * (u[4] - s.ek);
+        if (!(s.p > 1e-20)) s.p = 1e-20;
+        s.nd = s.rho / (P.mu0 * AMU);
+        s.T = s.p * P.mu0 * AMU / (s.rho * KB);
+        s.rinv = 0.f;
+    } else {
+        s.rinv = __builtin_amdgcn_rcpf((float)u[0]);
+        s.ek = 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) * inv64((float)u[0]);
+        s.p = (P.gamma - 1.0) * (u[4] - s.ek);
+        if (!(s.p > 1e-20)) s.p = 1e-20;
+        s.nd = (double)((float)u[0] * P.f_inv_mu_amu);
+        s.T = (double)((float)s.p * P.f_cT * s.rinv);
+    }
+    return s;
+}
+
+// dt / t_cool of a cell at T >= T_min
+template <class R>
+__device__ __forceinline__ double dt_over_tcool(const Problem &P, const CellState &s, double dt) {
+    if constexpr (sizeof(R) == 8) {
+        return find_coef(P, s.T) * s.nd * s.nd / (s.p / (P.gamma - 1.0)) * dt;
+    } else {
+        const float ce = lambda_f(P, (float)s.T) * (float)s.nd * (float)s.nd *
+                         (float)(P.gamma - 1.0) * __builtin_amdgcn_rcpf((float)s.p);
+        return (double)(ce * (float)dt);
+    }
+}
+
+template <class R>
+__device__ __forceinline__ double integrate(const Problem &P, const CellState &s, double dt,
+                                            int max_sub) {
+    if constexpr (sizeof(R) == 8) return cool_sub_d(P, s.nd, s.T, dt, COOL_EPS, max_sub);
+    else return cool_sub_f(P, (float)s.nd, (float)s.T, (float)dt, COOL_EPS, max_sub);
+}
+
+template <class R>
+__device__ __forceinline__ void finish_cell(const Problem &P, R *__restrict__ U, long long nc,
+                                            long long c, const double u[5], const CellState &s,
+                                            double Tn, double &mx, double &my, double &mz) {
+    if constexpr (sizeof(R) == 8) {
+        const double pn = s.p * (Tn / s.T);
+        U[4 * nc + c] = (R)(s.ek + pn / (P.gamma - 1.0));
+        const double cs = sqrt(P.gamma * pn / s.rho);
+        mx = fmax(mx, fabs(u[1] / s.rho) + cs);
+        my = fmax(my, fabs(u[2] / s.rho) + cs);
+        mz = fmax(mz, fabs(u[3] / s.rho) + cs);
+    } else {
+        const float pn = (float)s.p * ((float)Tn * __builtin_amdgcn_rcpf((float)s.T));
+        U[4 * nc + c] = (R)(s.ek + (double)pn * (double)P.f_inv_gm1);
+        const float cs = __builtin_amdgcn_sqrtf((float)P.gamma * pn * s.rinv);
+        mx = fmax(mx, (double)(fabsf((float)u[1] * s.rinv) + cs));
+        my = fmax(my, (double)(fabsf((float)u[2] * s.rinv) + cs));
+        mz = fmax(mz, (double)(fabsf((float)u[3] * s.rinv) + cs));
+    }
+}
+
+// CFL maxima of a cell left as it is (below T_min)
+template <class R>
+__device__ __forceinline__ void cell_maxima(const Problem &P, const double u[5],
+                                            const CellState &s, double &mx, double &my,
+                                            double &mz) {
+    if constexpr (sizeof(R) == 8) {
+        const double cs = sqrt(P.gamma * s.p / s.rho);
+        mx = fmax(mx, fabs(u[1] / s.rho) + cs);
+        my = fmax(my, fabs(u[2] / s.rho) + cs);
+        mz = fmax(mz, fabs(u[3] / s.rho) + cs);
+    } else {
+        const float cs = __builtin_amdgcn_sqrtf((float)P.gamma * (float)s.p * s.rinv);
+        mx = fmax(mx, (double)(fabsf((float)u[1] * s.rinv) + cs));
+        my = fmax(my, (double)(fabsf((float)u[2] * s.rinv) + cs));
+        mz = fmax(mz, (double)(fabsf((float)u[3] * s.rinv) + cs));
+    }
+}
+
+// log10 through the hardware log2 (FP32 inputs of the surrogate and its check)
+__device__ __forceinline__ float lg10(float x) {
+    return __builtin_amdgcn_logf(x) * 0.301029995663981f;
+}
+
+// Block b of the classifying pass owns the contiguous interior cells
+// [b C, (b+1) C) (flat index, x fastest) and the queue segment of the same
+// range; it fills the segment with an LDS counter and leaves its length in
+// seg[b].  No global atomics besides the CFL maxima.
+template <class R>
+__global__ __launch_bounds__(256) void k_cool_classify(const Problem *__restrict__ Pp,
+                                                       R *__restrict__ U,
+                                                       StepState *__restrict__ st, int parity,
+                                                       unsigned *__restrict__ queue,
+                                                       unsigned *__restrict__ seg, long long C) {
+    __shared__ unsigned s_n;
     const Problem &P = *Pp;
     const Grid &g = P.g;
-    const int lane = threadIdx.x;
-    const int nxw = (g.nx + 63) / 64;  // wavefronts per row
     const double dt = dt_from_maxima(P, st->maxima[parity]);
-    bf16x4 wf[4][4];
-    float b3 = 0.f;
-    if (MODE == 2) {
-        load_small(sh, thq, lane);
-        b3 = bf2f(thq[OB3]);
-        load_w2_fwd(thq, lane, wf);
-        __syncthreads();
-    }
-    double mxd = 0.0, myd = 0.0, mzd = 0.0;
-    unsigned long long ncool = 0, nfall = 0;
-    const long long nrows = (long long)g.ny * g.nz * nxw;
-    for (long long w = blockIdx.x; w < nrows; w += gridDim.x) {
-        const int wx = (int)(w % nxw);
-        const long long r = w / nxw;
+    const int lane = threadIdx.x & 63;
+    const long long ncell = (long long)g.nx * g.ny * g.nz;
+    const long long first = (long long)blockIdx.x * C;
+    unsigned *qseg = queue + first;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    double mx = 0.0, my = 0.0, mz = 0.0;
+    // C is a multiple of the block size: every wavefront runs whole
+    // iterations, so its ballot sees all 64 lanes
+    for (long long t = first + threadIdx.x; t < first + C; t += blockDim.x) {
+        const bool in = t < ncell;
+        const long long tt = in ? t : ncell - 1;
+        const int i = (int)(tt % g.nx) + 1;
+        const long long r = tt / g.nx;
         const int j = (int)(r % g.ny) + 1, k = (int)(r / g.ny) + 1;
-        const int i = wx * 64 + lane + 1;
-        const bool in = i <= g.nx;
-        const long long c = g.idx(in ? i : g.nx, j, k);
+        const long long c = g.idx(i, j, k);
         double u[5];
-#pragma unroll
-        for (int e = 0; e < 5; ++e) u[e] = (double)U[e * g.nc + c];
-        const double rho = u[0];
-        const double ek = 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) / rho;
-        double p = (P.gamma - 1.0) * (u[4] - ek);
-        if (!(p > 1e-20)) p = 1e-20;
-        const double nd = rho / (P.mu0 * AMU);
-        const double T = p * P.mu0 * AMU / (rho * KB);
-        const bool hot = in && T >= P.tmin;
-        double Tn = T;
-        // dt <= eps t_cool: the converged integrator takes exactly one
-        // substep, i.e. the reference's formula -- no network, no fallback
-        bool big = hot;
-        if (hot) {
-            if constexpr (sizeof(R) == 8) {
-                big = find_coef(P, T) * nd * nd / (p / (P.gamma - 1.0)) * dt > (double)COOL_EPS;
-            } else {
-                const float ce = lambda_f(P, (float)T) * (float)nd * (float)nd *
-                                 (float)(P.gamma - 1.0) / (float)p;
-                big = ce * (float)dt > COOL_EPS;
-            }
+        const CellState s = cell_state(P, U, g.nc, c, u);
+        const bool hot = in && s.T >= P.tmin;
+        const bool big = hot && dt_over_tcool<R>(P, s, dt) > (double)COOL_EPS;
+        if (in && !big) {
+            if (hot) finish_cell(P, U, g.nc, c, u, s, integrate<R>(P, s, dt, 1), mx, my, mz);
+            else cell_maxima<R>(P, u, s, mx, my, mz);
         }
-        bool need_exact = false;
-        if (MODE == 2 && __any(big)) {
-            float x[3] = {0.f, 0.f, 0.f};
-            if (big) {
-                const float lam = lambda_f(P, (float)T);
-                const float ce = lam * (float)nd * (float)nd * (float)(P.gamma - 1.0) / (float)p;
-                x[0] = log10f((float)nd);
-                x[1] = log10f((float)T);
-                x[2] = log10f(ce * (float)dt);
-            }
+        const unsigned long long mask = __ballot(big);
+        if (mask) {
+            const int leader = __ffsll((long long)mask) - 1;
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(&s_n, (unsigned)__popcll(mask));
+            base = __shfl(base, leader);
+            if (big) qseg[base + __popcll(mask & ((1ull << lane) - 1ull))] = (unsigned)c;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) seg[blockIdx.x] = s_n;
+    block_max3<256>(mx, my, mz, st->maxima[parity ^ 1]);
+}
+
+// Block b of the dense pass runs queue segment b, 64 cells per wavefront
+// step; fall[b] receives the segment's fallback count.
+template <class R, int MODE>
+__global__ __launch_bounds__(128) void k_cool_dense(const Problem *__restrict__ Pp,
+                                                    R *__restrict__ U,
+                                                    const unsigned short *__restrict__ thq,
+                                                    StepState *__restrict__ st, int parity,
+                                                    const unsigned *__restrict__ queue,
+                                                    const unsigned *__restrict__ seg,
+                                                    unsigned *__restrict__ fqueue,
+                                                    unsigned *__restrict__ fall, long long C) {
+    __shared__ MlpSmall sm;
+    __shared__ unsigned s_fall;
+    unsigned *fseg = MODE == 2 ? fqueue + (long long)blockIdx.x * C : nullptr;
+    const Problem &P = *Pp;
+    const Grid &g = P.g;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const long long nq = seg[blockIdx.x];
+    const unsigned *qseg = queue + (long long)blockIdx.x * C;
+    const double dt = dt_from_maxima(P, st->maxima[parity]);
+    bf16x4 w2a[4][4];
+    float b3 = 0.f;
+    if (threadIdx.x == 0) s_fall = 0;
+    if (MODE == 2 && nq > 0) {
+        load_small_f(sm, thq, threadIdx.x, blockDim.x);
+        b3 = bf2f(thq[OB3]);
+        load_w2_fwd(thq, lane, w2a);
+    }
+    __syncthreads();
+    double mx = 0.0, my = 0.0, mz = 0.0;
+    for (long long base = (long long)wv * 64; base < nq; base += 128) {
+        const long long qi = base + lane;
+        const bool valid = qi < nq;
+        const long long c = (long long)qseg[valid ? qi : base];
+        double u[5];
+        const CellState s = cell_state(P, U, g.nc, c, u);
+        bool need_exact = valid;
+        double Tn = s.T;
+        if constexpr (MODE == 2) {
+            const float lam = lambda_f(P, (float)s.T);
+            const float ce = lam * (float)s.nd * (float)s.nd * (float)(P.gamma - 1.0) *
+                             __builtin_amdgcn_rcpf((float)s.p);
+            float x[3] = {lg10((float)s.nd), lg10((float)s.T), lg10(ce * (float)dt)};
             float xn[3];
             normalise(x, xn);
-            layer1(sh, lane, xn, false);
-            __syncthreads();
-            layer2(sh, lane, wf);
-            __syncthreads();
-            float o = b3;
-            for (int q = 0; q < H; ++q) o += sh.w3[q] * gelu(sh.f[lane][q] + sh.b2[q]);
-            __syncthreads();
-            if (big) {
-                const bool inbox = x[0] >= X0LO && x[0] <= X0HI && x[1] >= X1LO && x[1] <= X1HI &&
-                                   x[2] <= X2HI;
-                const float d = __builtin_amdgcn_exp2f(x[2] * 3.32192809488736f);
-                const float rh = -0.4342944819032518f * d / (1.f + d) * o;  // log10(T1/T0)
-                const float rmin = log10f(10.f / (float)T);
-                bool ok = inbox && rh <= 1e-3f && rh >= rmin - 1e-3f;
-                const float T0 = (float)T;
-                const float T1 = T0 * __builtin_amdgcn_exp2f(fminf(rh, 0.f) * 3.32192809488736f);
-                if (ok) {
-                    const float kn = (float)(KB / (P.gamma - 1.0)) / (float)nd;
-                    const float fdt = (float)dt;
-                    // a time error dt' shifts ln T1 by dt'/t_cool(T1): allow 5% of
-                    // the change in ln T, at most 0.05
-                    const float Te = fmaxf(T1, P.f_tmin);
-                    const float tol = 0.05f * fminf(fdt, kn * Te / lambda_f(P, Te));
-                    if (T1 <= P.f_tmin)
-                        ok = T1 >= 0.97f * P.f_tmin && kn * cool_dY(P, T0, P.f_tmin) <= fdt + tol;
-                    else
-                        ok = fabsf(kn * cool_dY(P, T0, T1) - fdt) <= tol;
+            const float o = mlp_wave(sm, w2a, b3, lane, xn);
+            const bool inbox = x[0] >= X0LO && x[0] <= X0HI && x[1] >= X1LO && x[1] <= X1HI &&
+                               x[2] <= X2HI;
+            const float d = __builtin_amdgcn_exp2f(x[2] * 3.32192809488736f);
+            const float rh = -0.4342944819032518f * d / (1.f + d) * o;  // log10(T1/T0)
+            const float rmin = lg10(10.f * __builtin_amdgcn_rcpf((float)s.T));
+            bool ok = valid && inbox && rh <= 1e-3f && rh >= rmin - 1e-3f;
+            bool snap = false;
+            const float T0 = (float)s.T;
+            const float T1 = T0 * __builtin_amdgcn_exp2f(fminf(rh, 0.f) * 3.32192809488736f);
+            if (ok) {
+                const float kn = (float)(KB / (P.gamma - 1.0)) / (float)s.nd;
+                const float fdt = (float)dt;
+                // a time error dt' shifts ln T1 by dt'/t_cool(T1): allow 5% of
+                // the change in ln T, at most 0.05
+                const float Te = fmaxf(T1, P.f_tmin);
+                const float tol = 0.05f * fminf(fdt, kn * Te / lambda_f(P, Te));
+                if (T1 <= 1.1f * P.f_tmin) {
+                    // the network says the gas reaches the gate: accepted when
+                    // the exact cooling time to T_min fits in dt, and the answer
+                    // is the gate itself (where the integrator stops)
+                    ok = kn * cool_dY(P, T0, P.f_tmin) <= fdt + tol;
+                    snap = true;
+                } else {
+                    ok = fabsf(kn * cool_dY(P, T0, T1) - fdt) <= tol;
                 }
-                if (ok) Tn = (double)T1;
-                else need_exact = true;
             }
-        } else if (MODE != 2) {
-            need_exact = big;
-        }
-        if (hot && !big) {  // one substep: apply_cooling's formula
-            if constexpr (sizeof(R) == 8) Tn = cool_sub_d(P, nd, T, dt, COOL_EPS, 1);
-            else Tn = cool_sub_f(P, (float)nd, (float)T, (float)dt, COOL_EPS, 1);
-        }
-        if (need_exact) {  // the converged integrator (mode 3, and mode 2's fallback)
-            if constexpr (sizeof(R) == 8) Tn = cool_sub_d(P, nd, T, dt, COOL_EPS, COOL_MAXSUB);
-            else Tn = cool_sub_f(P, (float)nd, (float)T, (float)dt, COOL_EPS, COOL_MAXSUB);
-        }
-        if (hot) {
-            ncool += big;
-            if (MODE == 2 && need_exact) ++nfall;
-            const double pn = p * (Tn / T);
-            const R e5 = (R)(ek + pn / (P.gamma - 1.0));
-            U[4 * g.nc + c] = e5;
-            p = pn;
-        }
-        if (in) {
-            const double cs = sqrt(P.gamma * p / rho);
-            mxd = fmax(mxd, fabs(u[1] / rho) + cs);
-            myd = fmax(myd, fabs(u[2] / rho) + cs);
-            mzd = fmax(mzd, fabs(u[3] / rho) + cs);
+            if (ok) {
+                Tn = snap ? (double)SNAP * P.tmin : (double)T1;
+                need_exact = false;
+            }
+            // fallbacks go to a queue of their own, integrated by a dense
+            // mode-3 pass: a lane that needs the integrator does not hold up
+            // the other 63
+            const unsigned long long fm = __ballot(need_exact);
+            if (fm) {
+                const int leader = __ffsll((long long)fm) - 1;
+                unsigned fb = 0;
+                if (lane == leader) fb = atomicAdd(&s_fall, (unsigned)__popcll(fm));
+                fb = __shfl(fb, leader);
+                if (need_exact) fseg[fb + __popcll(fm & ((1ull << lane) - 1ull))] = (unsigned)c;
+            }
+            if (valid && !need_exact) finish_cell(P, U, g.nc, c, u, s, Tn, mx, my, mz);
+        } else {
+            if (valid) finish_cell(P, U, g.nc, c, u, s, integrate<R>(P, s, dt, COOL_MAXSUB), mx,
+                                   my, mz);
         }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        ncool += __shfl_xor(ncool, o);
-        nfall += __shfl_xor(nfall, o);
-    }
-    if (lane == 0 && counts) {
-        atomicAdd(counts, ncool);
-        atomicAdd(counts + 1, nfall);
-    }
-    block_max3<64>(mxd, myd, mzd, st->maxima[parity ^ 1]);
+    __syncthreads();
+    if (MODE == 2 && threadIdx.x == 0) fall[blockIdx.x] = s_fall;
+    if (nq > 0) block_max3<128>(mx, my, mz, st->maxima[parity ^ 1]);
 }
 
 // ---------------------------------------------------------------------------
@@ -654,24 +851,30 @@ hipError_t sur_eval(hipStream_t s, const unsigned short *thq, const float *X, lo
 
 template <class R>
 hipError_t sur_cool_pass(hipStream_t s, const Problem *P, const Grid &g, R *U,
-                         const unsigned short *thq, StepState *st, int parity,
-                         unsigned long long *counts, int mode) {
-    const long long nrows = (long long)g.ny * g.nz * ((g.nx + 63) / 64);
-    long long nb = nrows < 256 * 64 ? nrows : 256 * 64;
-    if (mode == 2)
-        hipLaunchKernelGGL((k_cool_pass<R, 2>), dim3((unsigned)nb), dim3(64), 0, s, P, U, thq, st,
-                           parity, counts);
-    else
-        hipLaunchKernelGGL((k_cool_pass<R, 3>), dim3((unsigned)nb), dim3(64), 0, s, P, U, thq, st,
-                           parity, counts);
+                         const unsigned short *thq, StepState *st, int parity, unsigned *queue,
+                         unsigned *seg, unsigned *fqueue, unsigned *fall, int mode) {
+    const long long ncell = (long long)g.nx * g.ny * g.nz;
+    const long long C = cool_segment(ncell);
+    const unsigned nb = (unsigned)((ncell + C - 1) / C);
+    hipLaunchKernelGGL(k_cool_classify<R>, dim3(nb), dim3(256), 0, s, P, U, st, parity, queue,
+                       seg, C);
+    if (mode == 2) {
+        hipLaunchKernelGGL((k_cool_dense<R, 2>), dim3(nb), dim3(128), 0, s, P, U, thq, st, parity,
+                           queue, seg, fqueue, fall, C);
+        hipLaunchKernelGGL((k_cool_dense<R, 3>), dim3(nb), dim3(128), 0, s, P, U, thq, st, parity,
+                           fqueue, fall, nullptr, nullptr, C);
+    } else {
+        hipLaunchKernelGGL((k_cool_dense<R, 3>), dim3(nb), dim3(128), 0, s, P, U, thq, st, parity,
+                           queue, seg, nullptr, nullptr, C);
+    }
     return hipGetLastError();
 }
 
 template hipError_t sur_cool_pass<float>(hipStream_t, const Problem *, const Grid &, float *,
-                                         const unsigned short *, StepState *, int,
-                                         unsigned long long *, int);
+                                         const unsigned short *, StepState *, int, unsigned *,
+                                         unsigned *, unsigned *, unsigned *, int);
 template hipError_t sur_cool_pass<double>(hipStream_t, const Problem *, const Grid &, double *,
-                                          const unsigned short *, StepState *, int,
-                                          unsigned long long *, int);
+                                          const unsigned short *, StepState *, int, unsigned *,
+                                          unsigned *, unsigned *, unsigned *, int);
 
 }  // namespace tlq

@@ -52,15 +52,32 @@ hipError_t sur_quantize(hipStream_t s, const float *theta, unsigned short *thq);
 hipError_t sur_eval(hipStream_t s, const unsigned short *thq, const float *X, long long n,
                     float *out);
 
-// Cooling pass over the interior after the fused corrector: mode 2 = surrogate
-// with per-cell fallback, mode 3 = the converged exact integrator.  Also
-// produces the next step's CFL maxima.  counts[0] += cells at T >= T_min with
-// dt > eps t_cool (those handed to the network in mode 2), counts[1] += cells
-// among them that took the exact fallback.
+// Cooling after the fused corrector in modes 2 (surrogate with per-cell
+// fallback) and 3 (the converged integrator): a classifying pass over the
+// interior that queues the cells with dt > 0.01 t_cool, then a dense pass over
+// the queue (mode 2: the network, whose rejected cells go to fqueue for a
+// second dense pass with the integrator).  Also produces the next step's CFL
+// maxima.  The interior (flat index) is cut into segments of
+// cool_segment(ncell) cells, one block of each pass per segment; queue and
+// fqueue hold one 32-bit cell offset per segment slot (cool_queue_len
+// entries), seg[b] the number queued in segment b and fall[b] the number of
+// those that took the fallback.
+inline long long cool_segment(long long ncell) {
+    const long long nb = 2048;  // blocks per pass
+    long long C = (ncell + nb - 1) / nb;
+    C = (C + 255) / 256 * 256;
+    return C < 256 ? 256 : C;
+}
+inline long long cool_blocks(long long ncell) {
+    const long long C = cool_segment(ncell);
+    return (ncell + C - 1) / C;
+}
+inline long long cool_queue_len(long long ncell) { return cool_blocks(ncell) * cool_segment(ncell); }
+
 struct StepState;
 template <class R>
 hipError_t sur_cool_pass(hipStream_t s, const Problem *P, const Grid &g, R *U,
-                         const unsigned short *thq, StepState *st, int parity,
-                         unsigned long long *counts, int mode);
+                         const unsigned short *thq, StepState *st, int parity, unsigned *queue,
+                         unsigned *seg, unsigned *fqueue, unsigned *fall, int mode);
 
 }  // namespace tlq

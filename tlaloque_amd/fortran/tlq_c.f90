@@ -191,6 +191,12 @@ module tlq_c
        real(c_double), intent(out) :: frac
      end function tlq_fallback_fraction
 
+     integer(c_int) function tlq_cooling_counts(ctx, queued, fallbacks) bind(C, name="tlq_cooling_counts")
+       import :: c_int, c_ptr, c_long_long
+       type(c_ptr), value :: ctx
+       integer(c_long_long), intent(out) :: queued, fallbacks
+     end function tlq_cooling_counts
+
      integer(c_int) function tlq_surrogate_init(ctx, seed) bind(C, name="tlq_surrogate_init")
        import :: c_int, c_ptr, c_long_long
        type(c_ptr), value :: ctx

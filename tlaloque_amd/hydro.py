@@ -43,7 +43,7 @@ EXPORTS = (
     "tlq_set_cooling_table", "tlq_upload", "tlq_download", "tlq_calc_prims", "tlq_time_step",
     "tlq_fluxes", "tlq_godunov", "tlq_borders", "tlq_step", "tlq_cooling", "tlq_init_state",
     "tlq_advance", "tlq_advance_n", "tlq_get_time", "tlq_sync", "tlq_check_state",
-    "tlq_fallback_fraction", "tlq_timer_start", "tlq_timer_stop", "tlq_kernel_timing",
+    "tlq_fallback_fraction", "tlq_cooling_counts", "tlq_timer_start", "tlq_timer_stop", "tlq_kernel_timing",
     "tlq_kernel_stats", "tlq_layout", "tlq_copy_probe", "tlq_surrogate_init",
     "tlq_surrogate_set_params", "tlq_surrogate_get_params", "tlq_surrogate_generate",
     "tlq_surrogate_samples", "tlq_surrogate_train", "tlq_surrogate_eval", "tlq_surrogate_grad",
@@ -110,6 +110,7 @@ def load_library(path: Optional[str] = None):
         "tlq_sync": [vp],
         "tlq_check_state": [vp, ctypes.POINTER(ctypes.c_longlong)],
         "tlq_fallback_fraction": [vp, dp],
+        "tlq_cooling_counts": [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
         "tlq_timer_start": [vp],
         "tlq_timer_stop": [vp, dp],
         "tlq_kernel_timing": [vp, ip],
@@ -276,6 +277,13 @@ class HydroCore:
         bad = ctypes.c_longlong()
         _check(_lib.tlq_check_state(self._ctx, ctypes.byref(bad)))
         return bad.value
+
+    def cooling_counts(self) -> Tuple[int, int]:
+        """(cells queued for the network or the converged integrator, fallbacks)
+        of the last step in cooling modes 2 and 3."""
+        q, f = ctypes.c_longlong(), ctypes.c_longlong()
+        _check(_lib.tlq_cooling_counts(self._ctx, ctypes.byref(q), ctypes.byref(f)))
+        return q.value, f.value
 
     def fallback_fraction(self) -> float:
         f = ctypes.c_double()
