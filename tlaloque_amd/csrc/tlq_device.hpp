@@ -1,9 +1,11 @@
 // tlq_device.hpp -- device math of the HydroCore step for gfx950.
 //
 // Every routine is templated on the sweep type R: double (FP64 mode) or float
-// (FP32 mode).  In FP32 mode the pressure recovery E - |m|^2/(2 rho), the
-// conserved update, viscosity accumulation, cooling, winds and gravity run in
-// double; only the reconstruction and Riemann fluxes run in float.
+// (FP32 mode).  In FP32 mode the pressure recovery E - |m|^2/(2 rho) and the
+// accumulation of the conserved update and of the viscosity term run in
+// double, and the wind states are computed in double; the reconstruction,
+// the Riemann fluxes and the cooling and point-gravity source terms run in
+// float (the *_fast routines).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -408,6 +410,62 @@ __device__ __forceinline__ void cons2prim_fast(double gm1, const float u[5], flo
 // apply_cooling for one cell in FP32 (cooling.f90:67-112): the table row is
 // found by direct indexing when the table is equally spaced in log10 T
 // (Z1.0.dat is: 0.01 dex), else by bisection; 10^x and exp via exp2/log2.
+// The FP32 cooling table as one float4 per row (T, log10 T, log10 Lambda,
+// slope of the segment ending at the row), the form a kernel stages in LDS.
+struct TabRow {
+    float T, lgT, L, slope;
+};
+
+// Fills tab[0..ntable) from the Problem's FP32 table (all threads of a block
+// call it; the caller synchronises before use).
+__device__ __forceinline__ void stage_table(const Problem &P, TabRow *tab, int tid, int nthreads) {
+    for (int q = tid; q < P.ntable; q += nthreads)
+        tab[q] = TabRow{P.tab_Tf[q], P.tab_lgTf[q], P.tab_Lf[q], P.tab_slopef[q]};
+}
+
+// apply_cooling in FP32 with the table rows in LDS (see cool_fast).
+__device__ __forceinline__ float cool_fast_tab(const Problem &P, const TabRow *tab, float rho,
+                                               float p, float dt, bool *cooled) {
+    const float rinv = __builtin_amdgcn_rcpf(rho);
+    const float temp = p * P.f_cT * rinv;
+    if (!(temp >= P.f_tmin)) {
+        *cooled = false;
+        return p;
+    }
+    *cooled = true;
+    const float lg = __builtin_amdgcn_logf(temp) * 0.301029995663981f;  // log10 T
+    float lgl;
+    if (!(temp < P.f_tmax)) {
+        lgl = tab[P.ntable - 1].L;
+    } else {
+        int hi;
+        if (P.tab_uniform) {
+            hi = (int)((lg - P.f_lgT0) * P.f_inv_dlg) + 1;
+            hi = hi < 1 ? 1 : (hi > P.ntable - 1 ? P.ntable - 1 : hi);
+            if (!(tab[hi].T > temp) && hi < P.ntable - 1) ++hi;
+            if (hi > 1 && tab[hi - 1].T > temp) --hi;
+        } else {
+            int lo = 0;
+            hi = P.ntable - 1;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (tab[mid].T > temp) hi = mid;
+                else lo = mid;
+            }
+        }
+        const TabRow a = tab[hi - 1];
+        lgl = a.L + tab[hi].slope * (lg - a.lgT);
+    }
+    const float lam = __builtin_amdgcn_exp2f(lgl * 3.32192809488736f);  // 10^lgl
+    const float n = rho * P.f_inv_mu_amu;
+    const float eth = p * P.f_inv_gm1;
+    const float ce = lam * n * n * __builtin_amdgcn_rcpf(eth);
+    float f = __builtin_amdgcn_exp2f(-ce * dt * 1.44269504088896f);  // exp(-ce dt)
+    if (f < P.f_floor_f) f = P.f_floor_f;
+    if (temp * f < 10.f) f = 10.f * __builtin_amdgcn_rcpf(temp);
+    return p * f;
+}
+
 __device__ __forceinline__ float cool_fast(const Problem &P, float rho, float p, float dt,
                                            bool *cooled) {
     const float rinv = __builtin_amdgcn_rcpf(rho);

@@ -409,6 +409,8 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
     __shared__ R sXm[5][TY][TX + 2], sXp[5][TY][TX + 2];
     __shared__ R sYm[5][TY + 2][TX], sYp[5][TY + 2][TX];
     __shared__ unsigned long long s_wmask;
+    // FP32 cooling table (one-step cooling in FP32 mode)
+    __shared__ TabRow sTab[sizeof(R) == 4 ? MAX_TABLE : 1];
     auto &sW = sPF.W;
     auto &sF = sPF.f.F;
     auto &sG = sPF.f.G;
@@ -444,6 +446,9 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
     if (tid == 0)
         s_wmask = P.nwinds ? winds_mask(P, x0 - 2, x0 + TX + 1, y0 - 2, y0 + TY + 1, z0 - 2, z1 + 2)
                            : 0ull;
+    if constexpr (sizeof(R) == 4) {
+        if (cool) stage_table(P, sTab, tid, NT);  // visible after the plane's first barrier
+    }
 
     // z pipeline of U^{1/2}: primitives at k, k+1, conserved at k, k+1, the
     // half slope of cell k (limiter, HLLC.f90:574-582); U^n at k-1, k; the z
@@ -642,7 +647,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
                 float p = pd > 1e-20 ? (float)pd : 1e-20f;
                 if (cool) {
                     bool cooled;
-                    const float pn = cool_fast(P, oq[0], p, (float)dt, &cooled);
+                    const float pn = cool_fast_tab(P, sTab, oq[0], p, (float)dt, &cooled);
                     if (cooled) {
                         oq[4] = (float)(ek + (double)pn * (double)P.f_inv_gm1);
                         p = pn;
