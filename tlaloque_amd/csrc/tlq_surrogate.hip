@@ -191,14 +191,9 @@ struct MlpShared {
     float w1[3 * H], b1[H], b2[H], w3[H];
     float dout[64];
 };
-struct MlpSharedInfer {
-    unsigned short h1[64][72];
-    float f[64][65];
-    float w1[3 * H], b1[H], b2[H], w3[H];
-};
-
-// B-operand fragments of W2 for z2 = h1 W2^T: lane holds W2[j][i..i+3] with
-// j = 16 nt + lane%16, i = 16 kt + 4 (lane/16).
+// Fragments of W2: lane holds W2[j][i..i+3] with j = 16 nt + lane%16,
+// i = 16 kt + 4 (lane/16) -- the B operand of z2 = h1 W2^T (training) and the
+// A operand of z2^T = W2 h1^T (inference, mlp_wave).
 __device__ __forceinline__ void load_w2_fwd(const unsigned short *__restrict__ thq, int lane,
                                             bf16x4 w[4][4]) {
 #pragma unroll
@@ -211,23 +206,19 @@ __device__ __forceinline__ void load_w2_fwd(const unsigned short *__restrict__ t
         }
 }
 
-// layer 1 for the lane's own sample -> h1 (bf16) in shared memory
-template <class S>
-__device__ __forceinline__ void layer1(S &sh, int lane, const float xn[3], bool transposed) {
+// layer 1 for the lane's own sample -> h1 (bf16) and its transpose in LDS
+__device__ __forceinline__ void layer1(MlpShared &sh, int lane, const float xn[3]) {
     for (int i = 0; i < H; ++i) {
         const float z = sh.b1[i] + sh.w1[3 * i] * xn[0] + sh.w1[3 * i + 1] * xn[1] +
                         sh.w1[3 * i + 2] * xn[2];
         const unsigned short q = f2bf(gelu(z));
         sh.h1[lane][i] = q;
-        if constexpr (sizeof(S) == sizeof(MlpShared)) {
-            if (transposed) sh.h1t[i][lane] = q;
-        }
+        sh.h1t[i][lane] = q;
     }
 }
 
 // layer 2 (MFMA): f[sample][j] = sum_i h1[sample][i] W2[j][i]
-template <class S>
-__device__ __forceinline__ void layer2(S &sh, int lane, const bf16x4 w[4][4]) {
+__device__ __forceinline__ void layer2(MlpShared &sh, int lane, const bf16x4 w[4][4]) {
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
@@ -244,8 +235,8 @@ __device__ __forceinline__ void layer2(S &sh, int lane, const bf16x4 w[4][4]) {
         }
 }
 
-template <class S>
-__device__ __forceinline__ void load_small(S &sh, const unsigned short *__restrict__ thq, int lane) {
+__device__ __forceinline__ void load_small(MlpShared &sh, const unsigned short *__restrict__ thq,
+                                           int lane) {
     for (int q = lane; q < 3 * H; q += 64) sh.w1[q] = bf2f(thq[OW1 + q]);
     sh.b1[lane] = bf2f(thq[OB1 + lane]);
     sh.b2[lane] = bf2f(thq[OB2 + lane]);
@@ -373,7 +364,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const unsigned short *__restric
         sh.x[lane][0] = xn[0];
         sh.x[lane][1] = xn[1];
         sh.x[lane][2] = xn[2];
-        layer1(sh, lane, xn, true);
+        layer1(sh, lane, xn);
         __syncthreads();
         layer2(sh, lane, wf);
         __syncthreads();
