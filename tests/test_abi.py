@@ -46,3 +46,40 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(hydro, "_lib", None)
     with pytest.raises(hydro.TlqError):
         hydro.load_library(str(tmp_path / "nope.so"))
+
+
+_BAD = [
+    ("mesh", dict(nx=1), -1, "nx, ny, nz"),
+    ("inflow", dict(bc=(1, 4, 1, 1, 1, 1)), -4, "BC_INFLOW"),
+    ("bc id", dict(bc=(1, 9, 1, 1, 1, 1)), -1, "boundary"),
+    ("limiter", dict(limiter=7), -1, "limiter"),
+    ("solver", dict(solver=2), -1, "solver"),
+    ("precision", dict(precision=2), -1, "precision"),
+    ("cooling", dict(cooling=4), -1, "cooling"),
+    ("gamma", dict(gamma=1.0), -1, "physical"),
+    ("domain", dict(xr=-2.0), -1, "physical"),
+]
+
+
+@pytest.mark.parametrize("name,kw,code,text", _BAD, ids=[b[0] for b in _BAD])
+def test_create_rejects_bad_parameters(name, kw, code, text):
+    # validation precedes every HIP call, so this runs without a GPU
+    if not os.path.exists(hydro.LIB_PATH):
+        pytest.skip("libtlq.so not built")
+    from tlaloque_amd import params as P
+    p = P.HydroParams(nx=8, ny=8, nz=8).replace(**kw)
+    with pytest.raises(hydro.TlqError) as e:
+        hydro.HydroCore(p)
+    assert e.value.code == code
+    assert text in str(e.value)
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
+def test_create_without_gpu_reports_a_hip_error():
+    # the product has no CPU path: without a GPU, creating a context fails
+    if not os.path.exists(hydro.LIB_PATH):
+        pytest.skip("libtlq.so not built")
+    from tlaloque_amd import params as P
+    with pytest.raises(hydro.TlqError) as e:
+        hydro.HydroCore(P.HydroParams(nx=8, ny=8, nz=8))
+    assert e.value.code == -2

@@ -189,3 +189,50 @@ def test_driver_per_routine_equals_fused(tmp_path, driver):
     for e in range(5):
         scale = max(np.abs(b[..., e]).max(), 1e-30)
         assert np.abs(a[..., e] - b[..., e]).max() <= 1e-12 * scale, e
+
+
+@pytest.mark.gpu
+@needs_flang
+def test_driver_sources_from_namelist_match_python_binding(tmp_path, driver):
+    # winds, a point mass and tabulated cooling set through the namelist and
+    # the Fortran bindings (tlq_set_winds / tlq_set_point_masses /
+    # tlq_set_cooling_table), the state read from a file: bit-equal to the
+    # same run through the Python binding, fused and per routine
+    from helpers import TABLE_PATH, cooling_table, load_ref
+    from tlaloque_amd.hydro import HydroCore
+    p, u0, _, _ = load_ref("winds_cooling")
+    L = p.xr - p.xl
+    p = p.replace(masses=[P.PointMass(0.1 * L, -0.05 * L, 0.02 * L, 50.0 * P.MSUN)],
+                  softening=0.5 * p.dx)
+    init = tmp_path / "u0.bin"
+    np.ascontiguousarray(u0).astype("<f8").tofile(init)
+    lines = [f"initfile='{init}'", f"nwinds={len(p.winds)}", f"nmasses={len(p.masses)}",
+             "max_steps=5"]
+    for q, w in enumerate(p.winds, start=1):
+        for f, v in zip(("xc", "yc", "zc", "vx", "vy", "vz", "radius", "mdot", "vinf", "temp", "mu"),
+                        w.as_tuple()):
+            lines.append(f"winds({q})%{f}={v!r}")
+    m = p.masses[0]
+    lines.append(f"masses(1:4,1)={m.x!r},{m.y!r},{m.z!r},{m.mass!r}")
+    res = {}
+    for per in (False, True):
+        out = tmp_path / f"D{int(per)}"
+        out.mkdir()
+        nml = tmp_path / f"s{int(per)}.nml"
+        extra = ", " + ", ".join(lines + [f"per_routine={'.true.' if per else '.false.'}"])
+        _namelist(nml, p, "file", out, tfin=1e30, num_out=1, outfile="bin", extra=extra)
+        text = nml.read_text().replace(
+            "/\n&run", f", softening={p.softening!r}, cooling_limit={p.cooling_limit!r}, "
+                       f"cooling_file='{TABLE_PATH}' /\n&run", 1)
+        nml.write_text(text)
+        _run(driver, nml)
+        res[per] = vtk.read_bin(vtk.bin_name(str(out), 1), p)[2:-2, 2:-2, 2:-2]
+    with HydroCore(p, cooling_table=cooling_table()) as h:
+        h.set_state(u0)
+        h.advance(5)
+        h.calc_prims("U")
+        w = h.download("UPrim")[2:-2, 2:-2, 2:-2]
+    assert np.array_equal(res[False], w)
+    for e in range(5):
+        scale = max(np.abs(w[..., e]).max(), 1e-300)
+        assert np.abs(res[True][..., e] - w[..., e]).max() <= 1e-10 * scale, e
