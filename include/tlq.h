@@ -155,17 +155,31 @@ int tlq_cooling_counts(tlq_ctx *ctx, long long *queued, long long *fallbacks);
  * FP32 accumulation; TLQ_MLP_NPARAMS parameters laid out as W1[64][3], b1[64],
  * W2[64][64], b2[64], W3[64], b3).  No counterpart in the reference, which
  * has only the exact formula; the training targets come from that formula
- * (TLQ_COOL_SUBCYCLED).  Inputs per sample: log10 n [cm^-3], log10 T [K],
- * log10(dt / t_cool); output log10(T_new / T). */
+ * integrated to convergence (TLQ_COOL_SUBCYCLED).  Inputs per sample:
+ * x = (log10 n [cm^-3], log10 T [K], log10(dt / t_cool)) in the training box
+ * [-3,4] x [4,9] x [-2,2]; output g = r / r1 with r = log10(T_new / T) and
+ * r1 = -log10(e) d / (1 + d), d = dt / t_cool. */
 #define TLQ_MLP_NPARAMS 4481
+/* Random initial parameters (seeded) and fresh Adam moments. */
 int tlq_surrogate_init(tlq_ctx *ctx, unsigned long long seed);
+/* FP32 master parameters in / out (n = TLQ_MLP_NPARAMS); setting them also
+ * readies the context for TLQ_COOL_SURROGATE. */
 int tlq_surrogate_set_params(tlq_ctx *ctx, const float *theta, int n);
 int tlq_surrogate_get_params(tlq_ctx *ctx, float *theta, int n);
+/* nsamples uniform draws of the training box with their targets g, made on
+ * the GPU by the FP64 converged integrator (kept on the device). */
 int tlq_surrogate_generate(tlq_ctx *ctx, long long nsamples, unsigned long long seed);
+/* Copies samples [first, first + n) out: x as n triples, y as n targets. */
 int tlq_surrogate_samples(tlq_ctx *ctx, long long first, long long n, float *x, float *y);
+/* steps of Adam on the mean squared error, each over the next contiguous batch
+ * of the (independently drawn) generated samples, cycling through the set;
+ * learning rate on a cosine from lr_max to lr_min. */
 int tlq_surrogate_train(tlq_ctx *ctx, int steps, int batch, double lr_max, double lr_min,
                         double *final_loss);
+/* Network output g for n input triples. */
 int tlq_surrogate_eval(tlq_ctx *ctx, const float *x, long long n, float *out);
+/* Gradient (TLQ_MLP_NPARAMS values) and value of the mean squared error of
+ * the network on the n samples (x, y) given. */
 int tlq_surrogate_grad(tlq_ctx *ctx, const float *x, const float *y, int n, double *grad,
                        double *loss);
 
