@@ -136,6 +136,14 @@ int tlq_init_state(tlq_ctx *ctx);
 int tlq_advance(tlq_ctx *ctx, double *dt, double *t, int *it);
 int tlq_advance_n(tlq_ctx *ctx, int n);
 int tlq_get_time(tlq_ctx *ctx, double *dt, double *t, int *it);
+/* Restart: continue a run from a state uploaded with tlq_upload(U), at time
+ * t after `it` steps; in place of tlq_init_state.  fill_ghosts = 0 keeps the
+ * uploaded ghosts (a BIN dump holds them, and U's ghosts are images of UP,
+ * not of U's interior, HydroCore.f90:195-206), so the run continues to
+ * rounding; fill_ghosts = 1 fills them from the interior first (a VTK dump
+ * holds the interior only).  Replaces the unfinished warmstart
+ * (warmstart.f90:1-10; its autosave #warmstart.f90#:11-13 reads a VTK). */
+int tlq_resume(tlq_ctx *ctx, double t, int it, int fill_ghosts);
 int tlq_sync(tlq_ctx *ctx);
 
 /* Diagnostics: device-side check that every interior density is finite and

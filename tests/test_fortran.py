@@ -155,7 +155,7 @@ def test_driver_sod_matches_python_binding_and_exact_solution(tmp_path, driver):
     nml = tmp_path / "sod.nml"
     _namelist(nml, p, "sod", out, tfin=problems.SOD_DIAGONAL_TFIN, num_out=2)
     log = _run(driver, nml)
-    last = sorted(os.listdir(out))[-1]
+    last = sorted(f for f in os.listdir(out) if f.endswith(".vtk"))[-1]
     d = vtk.read_vtk(os.path.join(out, last))
     with HydroCore(p) as h:
         h.set_state(u0)

@@ -705,6 +705,24 @@ int tlq_advance_n(tlq_ctx *c, int n) {
     return DISPATCH(c, do_advance, n);
 }
 
+int tlq_resume(tlq_ctx *c, double t, int it, int fill_ghosts) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!(t >= 0.0 && t < HUGE_VAL) || it < 0) return fail(TLQ_ERR_ARG, "bad time or step count");
+    if (fill_ghosts != 0 && fill_ghosts != 1) return fail(TLQ_ERR_ARG, "fill_ghosts must be 0 or 1");
+    // A dump with ghosts keeps them: U's ghosts are images of UP, not of U's
+    // interior (HydroCore.f90:195-206), and a boundary fill would lose that.
+    if (fill_ghosts) {
+        int rc = DISPATCH(c, do_borders, c->u[c->cur], 1);
+        if (rc) return rc;
+    }
+    StepState s0{};
+    s0.time[c->cur] = t;
+    s0.it = it;
+    HIPCHK(hipMemcpyAsync(c->st, &s0, sizeof s0, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return DISPATCH(c, do_init_maxima);
+}
+
 int tlq_get_time(tlq_ctx *c, double *dt, double *t, int *it) {
     if (int rc = check_ctx(c)) return rc;
     StepState s;

@@ -174,6 +174,13 @@ module tlq_c
        integer(c_int), intent(out) :: it
      end function tlq_get_time
 
+     integer(c_int) function tlq_resume(ctx, t, it, fill_ghosts) bind(C, name="tlq_resume")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: t
+       integer(c_int), value :: it, fill_ghosts
+     end function tlq_resume
+
      integer(c_int) function tlq_sync(ctx) bind(C, name="tlq_sync")
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

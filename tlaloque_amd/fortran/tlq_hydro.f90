@@ -22,10 +22,22 @@
 !                   r < 3 dx, rho = 1, p = 1e-5 outside
 !          'file'   U read from initfile (little-endian real(8) stream of
 !                   U(5,-1:nx+2,-1:ny+2,-1:nz+2))
+!          'restart' continue a run from one of its outputs, restart_file:
+!                   a BIN dump (UPrim with ghosts; the ghosts are kept, so
+!                   the run continues to rounding) or a VTK file (interior
+!                   cells; the ghosts come from the boundary conditions, as
+!                   at a fresh start).  U is rebuilt from the primitives;
+!                   t, it and the output number come from the dump's sidecar
+!                   <restart_file>.time unless restart_t, restart_it,
+!                   restart_nout are given.
+!                   This is what source/warmstart.f90:1-10 set out to do
+!                   (read a VTK output back), which the reference never
+!                   finished.
 ! outfile: 'vtk', 'bin' or 'dat', written as source/HydroCore.f90:253-395
 ! does (file names EPV000Nnnn.vtk, points000Nnnn.bin / .dat), so the
 ! reference's analysis scripts read them unchanged.  VTK and BIN always hold
 ! real(8) values under the 'double' header.
+! Every output also writes <file>.time, one line "t it nout", for restarts.
 ! dry_run = .true. writes the initial state's output without touching the GPU.
 module driver_setup
   use iso_c_binding
@@ -50,7 +62,10 @@ module driver_setup
   character(len=32) :: problem = 'sod'
   integer :: num_out = 1, max_steps = 0, nwinds = 0, nmasses = 0
   character(len=8) :: outfile = 'vtk'
-  character(len=512) :: outputpath = './DATA/', initfile = ''
+  character(len=512) :: outputpath = './DATA/', initfile = '', restart_file = ''
+  real(8) :: restart_t = -1d0
+  integer :: restart_it = -1, restart_nout = -1
+  logical :: restart_fill = .true.   ! the dump has no ghosts (VTK)
   logical :: per_routine = .false., dry_run = .false.
   type(tlq_wind_t) :: winds(MAXW)
   real(8) :: masses(4, MAXM) = 0d0
@@ -59,7 +74,7 @@ module driver_setup
   namelist /physics/ gamma, mu0, cfl, eta, cooling_limit, limiter, solver, precision, cooling, &
        bc, softening, tfin, cooling_file
   namelist /run/ problem, num_out, outfile, outputpath, max_steps, per_routine, initfile, &
-       dry_run, nwinds, winds, nmasses, masses
+       dry_run, nwinds, winds, nmasses, masses, restart_file, restart_t, restart_it, restart_nout
 
   real(8) :: dx, dy, dz
 
@@ -218,11 +233,137 @@ contains
             form='unformatted', convert='LITTLE_ENDIAN', action='read')
        read(u_in) U
        close(u_in)
+    case ('restart')
+       call read_restart(U)
     case default
        write(*, '(a,a)') 'unknown problem ', trim(problem)
        error stop 2
     end select
   end subroutine initflow
+
+  ! U from the primitives of a dump (the inverse of calc_prims,
+  ! source/HydroCore.f90:14-21).  A VTK dump holds the interior cells only:
+  ! its ghosts start as copies of the nearest interior cell and the
+  ! boundary fill at start-up (tlq_init_state) sets them.
+  subroutine read_restart(U)
+    real(8), intent(inout) :: U(5, -1:nx+2, -1:ny+2, -1:nz+2)
+    real(8), allocatable :: W(:, :, :, :)
+    integer :: u_in, n, i, j, k
+    allocate(W(5, -1:nx+2, -1:ny+2, -1:nz+2))
+    n = len_trim(restart_file)
+    restart_fill = n > 4 .and. restart_file(max(n-3, 1):n) == '.vtk'
+    if (restart_fill) then
+       call read_vtk_prims(trim(restart_file), W)
+       do k = -1, nz + 2
+          do j = -1, ny + 2
+             do i = -1, nx + 2
+                W(:, i, j, k) = W(:, min(max(i, 1), nx), min(max(j, 1), ny), min(max(k, 1), nz))
+             end do
+          end do
+       end do
+    else
+       open(newunit=u_in, file=trim(restart_file), status='old', access='stream', &
+            form='unformatted', convert='LITTLE_ENDIAN', action='read')
+       read(u_in) W
+       close(u_in)
+    end if
+    do k = -1, nz + 2
+       do j = -1, ny + 2
+          do i = -1, nx + 2
+             call set_cell(U, i, j, k, W(1, i, j, k), W(2, i, j, k), W(3, i, j, k), &
+                  W(4, i, j, k), W(5, i, j, k))
+          end do
+       end do
+    end do
+  end subroutine read_restart
+
+  ! The legacy binary VTK this driver writes (DIMENSIONS nx ny nz, then the
+  ! interior Density, Pressure and Velocity as big-endian real(8)).
+  subroutine read_vtk_prims(path, W)
+    character(*), intent(in) :: path
+    real(8), intent(inout) :: W(5, -1:nx+2, -1:ny+2, -1:nz+2)
+    character(len=200) :: line
+    integer :: u_in, i, j, k, dims(3)
+    character(len=1) :: c
+    open(newunit=u_in, file=path, status='old', access='stream', form='unformatted', &
+         convert='BIG_ENDIAN', action='read')
+    dims = 0
+    do
+       call next_line()
+       if (line(1:10) == 'DIMENSIONS') then
+          read(line(11:), *) dims
+          if (any(dims /= [nx, ny, nz])) then
+             write(*, '(a,3i6)') 'restart file has DIMENSIONS ', dims
+             error stop 2
+          end if
+       else if (line(1:15) == 'SCALARS Density') then
+          call next_line()
+          read(u_in) (((W(1, i, j, k), i = 1, nx), j = 1, ny), k = 1, nz)
+       else if (line(1:16) == 'SCALARS Pressure') then
+          call next_line()
+          read(u_in) (((W(5, i, j, k), i = 1, nx), j = 1, ny), k = 1, nz)
+       else if (line(1:16) == 'VECTORS Velocity') then
+          read(u_in) (((W(2, i, j, k), W(3, i, j, k), W(4, i, j, k), i = 1, nx), j = 1, ny), &
+               k = 1, nz)
+          exit
+       end if
+    end do
+    close(u_in)
+    if (dims(1) == 0) then
+       write(*, '(a)') 'restart file has no DIMENSIONS line'
+       error stop 2
+    end if
+  contains
+    subroutine next_line()
+      integer :: m, ios
+      line = ''
+      m = 0
+      do
+         read(u_in, iostat=ios) c
+         if (ios /= 0) then
+            write(*, '(a,a)') 'restart file ends early: ', path
+            error stop 2
+         end if
+         if (c == char(10)) then
+            if (m > 0) exit
+            cycle
+         end if
+         m = m + 1
+         if (m <= len(line)) line(m:m) = c
+      end do
+    end subroutine next_line
+  end subroutine read_vtk_prims
+
+  ! t, it and the output number of a restart: the namelist's values where
+  ! given, else the dump's sidecar.
+  subroutine restart_clock(t, it, nout)
+    real(8), intent(out) :: t
+    integer(c_int), intent(out) :: it
+    integer, intent(out) :: nout
+    integer :: u_in, ios, it_f, nout_f
+    real(8) :: t_f
+    t_f = 0d0
+    it_f = 0
+    nout_f = 0
+    open(newunit=u_in, file=trim(restart_file)//'.time', status='old', action='read', iostat=ios)
+    if (ios == 0) then
+       read(u_in, *, iostat=ios) t_f, it_f, nout_f
+       close(u_in)
+       if (ios /= 0) then
+          write(*, '(a)') 'bad restart sidecar ', trim(restart_file)//'.time'
+          error stop 2
+       end if
+    else if (restart_t < 0d0) then
+       write(*, '(a,a)') 'no sidecar and no restart_t for ', trim(restart_file)
+       error stop 2
+    end if
+    t = t_f
+    it = it_f
+    nout = nout_f
+    if (restart_t >= 0d0) t = restart_t
+    if (restart_it >= 0) it = restart_it
+    if (restart_nout >= 0) nout = restart_nout
+  end subroutine restart_clock
 
   ! calc_prims on the host (source/HydroCore.f90:14-21), for dry runs.
   subroutine host_prims(UPrim, U)
@@ -267,13 +408,14 @@ module driver_output
 contains
 
   ! Output number nout of UPrim, in the reference's formats
-  ! (source/HydroCore.f90:253-395, single rank); then nout += 1 and
-  ! tout = dtout * nout.
-  subroutine output(UPrim, nout, tout, dtout)
+  ! (source/HydroCore.f90:253-395, single rank), and its sidecar
+  ! <file>.time holding t, it and nout; then nout += 1 and tout = dtout * nout.
+  subroutine output(UPrim, nout, tout, dtout, t, it)
     real(8), intent(in) :: UPrim(5, -1:nx+2, -1:ny+2, -1:nz+2)
     integer, intent(inout) :: nout
     real(8), intent(inout) :: tout
-    real(8), intent(in) :: dtout
+    real(8), intent(in) :: dtout, t
+    integer(c_int), intent(in) :: it
     character(len=600) :: fname
     character(len=200) :: line
     character(len=1), parameter :: lf = char(10)
@@ -329,6 +471,9 @@ contains
        write(*, '(a,a)') 'unknown outfile ', trim(outfile)
        error stop 2
     end select
+    open(newunit=u, file=trim(fname)//'.time', status='replace')
+    write(u, '(es25.17,1x,i0,1x,i0)') t, it, nout
+    close(u)
     write(*, '(a,i0,a)') ' File ', nout, ' written'
     nout = nout + 1
     tout = dtout * nout
@@ -370,10 +515,16 @@ program tlq_hydro
   nout = 0
   tout = 0d0
   dtout = tfin / max(num_out, 1)
+  if (trim(problem) == 'restart') then
+     call restart_clock(t, it, nout)
+     nout = nout + 1
+     tout = dtout * nout
+     write(*, '(a,a,a,es14.6,a,i0)') ' Restart from ', trim(restart_file), ' t = ', t, ' it = ', it
+  end if
 
   if (dry_run) then
      call host_prims(UPrim, U)
-     call output(UPrim, nout, tout, dtout)
+     call output(UPrim, nout, tout, dtout, t, it)
   else
      call run_gpu()
   end if
@@ -393,8 +544,12 @@ contains
          'tlq_set_point_masses')
     call tlq_require(tlq_upload(ctx, TLQ_BUF_U, U), 'tlq_upload')
     if (per_routine) then
-       call tlq_require(tlq_borders(ctx, TLQ_BUF_U, 1_c_int), 'borders')          ! mainHD.f90:38
+       if (trim(problem) /= 'restart' .or. restart_fill) &
+            call tlq_require(tlq_borders(ctx, TLQ_BUF_U, 1_c_int), 'borders')     ! mainHD.f90:38
        call tlq_require(tlq_calc_prims(ctx, TLQ_BUF_U), 'calc_prims')             ! mainHD.f90:44
+    else if (trim(problem) == 'restart') then
+       call tlq_require(tlq_resume(ctx, t, it, merge(1_c_int, 0_c_int, restart_fill)), &
+            'tlq_resume')
     else
        call tlq_require(tlq_init_state(ctx), 'tlq_init_state')
     end if
@@ -430,7 +585,7 @@ contains
   subroutine write_output()
     if (.not. per_routine) call tlq_require(tlq_calc_prims(ctx, TLQ_BUF_U), 'calc_prims')
     call tlq_require(tlq_download(ctx, TLQ_BUF_UPRIM, UPrim), 'tlq_download')
-    call output(UPrim, nout, tout, dtout)
+    call output(UPrim, nout, tout, dtout, t, it)
   end subroutine write_output
 
 end program tlq_hydro

@@ -43,7 +43,7 @@ EXPORTS = (
     "tlq_set_cooling_table", "tlq_upload", "tlq_download", "tlq_calc_prims", "tlq_time_step",
     "tlq_fluxes", "tlq_godunov", "tlq_borders", "tlq_step", "tlq_cooling", "tlq_init_state",
     "tlq_advance", "tlq_advance_n", "tlq_get_time", "tlq_sync", "tlq_check_state",
-    "tlq_fallback_fraction", "tlq_cooling_counts", "tlq_timer_start", "tlq_timer_stop", "tlq_kernel_timing",
+    "tlq_fallback_fraction", "tlq_cooling_counts", "tlq_resume", "tlq_timer_start", "tlq_timer_stop", "tlq_kernel_timing",
     "tlq_kernel_stats", "tlq_layout", "tlq_copy_probe", "tlq_surrogate_init",
     "tlq_surrogate_set_params", "tlq_surrogate_get_params", "tlq_surrogate_generate",
     "tlq_surrogate_samples", "tlq_surrogate_train", "tlq_surrogate_eval", "tlq_surrogate_grad",
@@ -111,6 +111,7 @@ def load_library(path: Optional[str] = None):
         "tlq_check_state": [vp, ctypes.POINTER(ctypes.c_longlong)],
         "tlq_fallback_fraction": [vp, dp],
         "tlq_cooling_counts": [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
+        "tlq_resume": [vp, ctypes.c_double, ip, ip],
         "tlq_timer_start": [vp],
         "tlq_timer_stop": [vp, dp],
         "tlq_kernel_timing": [vp, ip],
@@ -259,6 +260,13 @@ class HydroCore:
         dt, t, it = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
         _check(_lib.tlq_get_time(self._ctx, ctypes.byref(dt), ctypes.byref(t), ctypes.byref(it)))
         return dt.value, t.value, it.value
+
+    def resume(self, u: np.ndarray, t: float, it: int, fill_ghosts: bool = False) -> None:
+        """Restart from U (reference layout, with ghosts) at time t after `it`
+        steps (tlq_upload + tlq_resume).  Keep the ghosts of a BIN dump
+        (fill_ghosts=False); fill them for a VTK dump, which has none."""
+        self.upload("U", u)
+        _check(_lib.tlq_resume(self._ctx, float(t), int(it), int(fill_ghosts)))
 
     def run_until(self, tfin: float, max_steps: int = 10 ** 9) -> int:
         """mainHD.f90:61 loop: step while t < tfin (no clamp of the last dt)."""

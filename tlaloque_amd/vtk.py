@@ -12,6 +12,12 @@ one rank, so its analysis scripts read the GPU path's results unchanged:
 
 ``read_vtk`` parses such a file (the reference's harness uses VTKLegacy.jl
 for this, LoadVTK in tests_analisis.jl).
+
+Restarts: the reference's warmstart (source/warmstart.f90:1-10) was left
+unfinished; ``read_restart`` does what it set out to do, for BIN and VTK
+dumps.  Each dump may carry a sidecar ``<dump>.time`` with one line
+``t it nout`` (written by ``write_time`` and by the Fortran driver), so a
+restarted run continues the same clock and output numbering.
 """
 from __future__ import annotations
 
@@ -22,6 +28,7 @@ from typing import Dict
 import numpy as np
 
 from . import params as P
+from . import problems
 
 
 def vtk_name(outputpath: str, nout: int, rank: int = 0) -> str:
@@ -130,3 +137,41 @@ def read_vtk(path: str) -> Dict[str, object]:
         else:
             raise ValueError(f"unexpected VTK line {s!r}")
     return out
+
+
+def write_time(dump_path: str, t: float, it: int, nout: int) -> None:
+    """The sidecar ``<dump>.time``: t (17 significant digits), it, nout."""
+    with open(dump_path + ".time", "w") as fh:
+        fh.write(f"{t:25.17e} {int(it)} {int(nout)}\n")
+
+
+def read_time(dump_path: str):
+    """(t, it, nout) from the sidecar ``<dump>.time``."""
+    with open(dump_path + ".time") as fh:
+        f = fh.read().split()
+    return float(f[0].replace("D", "E").replace("d", "e")), int(f[1]), int(f[2])
+
+
+def read_restart(path: str, p: P.HydroParams):
+    """U (reference layout, with ghosts) rebuilt from a dump, and the dump's
+    (t, it, nout) when its sidecar exists (else None).
+
+    A BIN dump holds every cell; a VTK dump holds the interior only, and the
+    ghosts start as copies of the nearest interior cell (the boundary fill at
+    start-up, ``HydroCore.set_state``, overwrites them).
+    """
+    if path.endswith(".vtk"):
+        d = read_vtk(path)
+        if tuple(d["dims"]) != (p.nx, p.ny, p.nz):
+            raise ValueError(f"{path}: DIMENSIONS {d['dims']} != {(p.nx, p.ny, p.nz)}")
+        f = d["fields"]
+        w = np.empty((p.nz, p.ny, p.nx, 5))
+        w[..., 0] = f["Density"]
+        w[..., 1:4] = f["Velocity"]
+        w[..., 4] = f["Pressure"]
+        w = np.pad(w, ((2, 2), (2, 2), (2, 2), (0, 0)), mode="edge")
+    else:
+        w = read_bin(path, p)
+    u = problems.cons_from_prims(p, w[..., 0], w[..., 1:4], w[..., 4])
+    clock = read_time(path) if os.path.exists(path + ".time") else None
+    return u, clock
