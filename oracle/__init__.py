@@ -59,6 +59,12 @@ class OrcParams(ctypes.Structure):
     ]
 
 
+class OrcSG(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_int), ("boundary", ctypes.c_int), ("gconst", ctypes.c_double),
+                ("phi", ctypes.POINTER(ctypes.c_double)), ("work", ctypes.POINTER(ctypes.c_double)),
+                ("it", ctypes.c_int), ("cycles", ctypes.c_int), ("dif", ctypes.c_double)]
+
+
 def build(force: bool = False) -> str:
     """Compile the C oracle with its Makefile (gcc); returns the library path."""
     if force or not os.path.exists(_LIB_PATH):
@@ -92,6 +98,17 @@ def lib():
         L.orc_hll_face.argtypes = [pp, dp, dp, dp, dp, ctypes.c_int, dp]
         L.orc_advance.argtypes = [pp, dp, dp, dp, dp, dp, dp, dp, dp]
         L.orc_advance.restype = ctypes.c_double
+        sp = ctypes.POINTER(OrcSG)
+        L.orc_sg_work_len.argtypes = [pp, ctypes.c_int]
+        L.orc_sg_work_len.restype = ctypes.c_size_t
+        L.orc_multigrid.argtypes = [pp, sp, dp]
+        L.orc_multigrid.restype = ctypes.c_int
+        L.orc_moments.argtypes = [pp, dp, dp, dp, dp]
+        L.orc_phi_boundaries.argtypes = [pp, ctypes.c_double, dp, dp, dp, dp]
+        L.orc_density.argtypes = [pp, dp, dp]
+        L.orc_gravity_source_phi.argtypes = [pp, dp, dp, dp]
+        L.orc_advance_sg.argtypes = [pp, dp, dp, dp, dp, dp, dp, dp, dp, sp, dp]
+        L.orc_advance_sg.restype = ctypes.c_double
         L.orc_cool_subcycled.argtypes = [pp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_int)]
@@ -160,28 +177,78 @@ class Oracle:
         z = lambda: np.zeros(shape, dtype=np.float64)  # noqa: E731
         self.u, self.up, self.upp, self.prim = z(), z(), z(), z()
         self.ff, self.gg, self.hh = z(), z(), z()
-        self.src = z() if masses else None
+        self.src = z() if (masses or getattr(params, "self_gravity", 0)) else None
         self.t = 0.0
         self.it = 0
         self.dt = 0.0
+        self.sg = None
+        if getattr(params, "self_gravity", 0):
+            from tlaloque_amd.params import mg_levels
+            lv = mg_levels(params)
+            n = int(lib().orc_sg_work_len(self.P, lv))
+            self.phi = np.zeros((params.nz + 2, params.ny + 2, params.nx + 2))
+            self.work = np.zeros(n)
+            self.dens = np.zeros_like(self.phi)
+            sg = OrcSG()
+            sg.levels, sg.boundary, sg.gconst = lv, int(params.self_gravity), float(params.gconst)
+            sg.phi, sg.work, sg.it = _ptr(self.phi), _ptr(self.work), 0
+            self.sg = sg
 
     @property
     def P(self):
         return ctypes.byref(self.op)
 
     def set_cons(self, u: np.ndarray) -> None:
-        """initflow + borders(U,1) + calc_prims (source/mainHD.f90:32-44)."""
+        """initflow + borders(U,1) + calc_prims (source/mainHD.f90:32-44), and
+        with self-gravity the first MultiGrid (mainHD.f90:50-52)."""
         self.u[...] = u
         L = lib()
         L.orc_boundaries1(self.P, _ptr(self.u))
         L.orc_winds(self.P, _ptr(self.u))
         L.orc_calc_prims(self.P, _ptr(self.prim), _ptr(self.u))
+        if self.sg is not None:
+            L.orc_density(self.P, _ptr(self.u), _ptr(self.dens))
+            L.orc_multigrid(self.P, ctypes.byref(self.sg), _ptr(self.dens))
+
+    def set_phi(self, phi: np.ndarray) -> None:
+        """PHI as the user's initial conditions set it (its ghost faces are the
+        boundary values), shape (nz+2, ny+2, nx+2)."""
+        self.phi[...] = phi
+
+    def multigrid(self, dens: np.ndarray) -> int:
+        """One MultiGrid(dens, PHI) call at the current step count; returns the
+        V-cycle count."""
+        self.sg.it = self.it
+        d = np.ascontiguousarray(dens, dtype=np.float64)
+        return int(lib().orc_multigrid(self.P, ctypes.byref(self.sg), _ptr(d)))
+
+    def moments(self, rho: np.ndarray):
+        """Moments (sources.f90:599-676) with the MPI build's sums:
+        c0[0..4], cm[l-1, m-1], sm[l-1, m-1]."""
+        c0, cm, sm = np.zeros(5), np.zeros((4, 4)), np.zeros((4, 4))
+        lib().orc_moments(self.P, _ptr(np.ascontiguousarray(rho, dtype=np.float64)), _ptr(c0),
+                          _ptr(cm), _ptr(sm))
+        return c0, cm, sm
+
+    def phi_boundaries(self, gconst: float, c0, cm, sm, phi: np.ndarray) -> np.ndarray:
+        """phi_[xyz]boundaries (sources.f90:678-787) on the six faces, in place."""
+        lib().orc_phi_boundaries(self.P, float(gconst), _ptr(np.ascontiguousarray(c0)),
+                                 _ptr(np.ascontiguousarray(cm)), _ptr(np.ascontiguousarray(sm)),
+                                 _ptr(phi))
+        return phi
 
     def advance(self) -> float:
         """One iteration of source/mainHD.f90:74-99; returns dt."""
-        dt = lib().orc_advance(self.P, _ptr(self.u), _ptr(self.up), _ptr(self.upp),
-                               _ptr(self.prim), _ptr(self.ff), _ptr(self.gg), _ptr(self.hh),
-                               _ptr(self.src))
+        if self.sg is not None:
+            self.sg.it = self.it
+            dt = lib().orc_advance_sg(self.P, _ptr(self.u), _ptr(self.up), _ptr(self.upp),
+                                      _ptr(self.prim), _ptr(self.ff), _ptr(self.gg),
+                                      _ptr(self.hh), _ptr(self.src), ctypes.byref(self.sg),
+                                      _ptr(self.dens))
+        else:
+            dt = lib().orc_advance(self.P, _ptr(self.u), _ptr(self.up), _ptr(self.upp),
+                                   _ptr(self.prim), _ptr(self.ff), _ptr(self.gg), _ptr(self.hh),
+                                   _ptr(self.src))
         self.dt = dt
         self.t += dt
         self.it += 1

@@ -15,6 +15,17 @@ Makefile:15-30) with ``-O2 -cpp -DDOUBLEP -fdefault-real-8`` (Makefile:46-54),
 runs the program with BIN output (source/HydroCore.f90:277-283: UPrim with
 ghosts, little-endian) after every step, and reads the snapshots back.
 
+With ``gravity=True`` the build adds ``-DGRAV`` (the self-gravity multigrid
+inside Godunov, HLLC.f90:15-24, and the point-mass potential,
+pointsource_gravity.f90:12-39).  The user module then sets PHI from
+``phi0.bin`` (its ghost faces are the potential's boundary values), zeroes
+PHIP and, for point masses, calls pointmass_potential with the masses as
+the stars (8 Msun each, as hard-coded there).  The run writes PHIT per
+output to ``phi`` + the BIN name (HydroCore.f90:284-290), i.e. under the
+directory ``phi./DATA``, which the recipe creates.  In the serial build the
+neighbour ranks stay 0, so MultiGrid never applies its multipole boundary
+values (sources.f90:114-136 test ``left<0`` ...): the faces keep phi0's.
+
 The initial state comes from a file (``ic.bin``) written here, so the
 reference and the C oracle start from the same bits.  Only this container
 runs this script: nothing on the GPU side reads /root/reference.
@@ -54,7 +65,7 @@ def _r(x: float) -> str:
     return repr(float(x)) + "_8"
 
 
-def _parameters_module(p, tfin: float, cooling_file: Optional[str]) -> str:
+def _parameters_module(p, tfin: float, cooling_file: Optional[str], levels: int = 0) -> str:
     lb, rb, db, tb, fb, bb = (BC_NAME[int(b)] for b in p.bc)
     lines = [
         "module parameters",
@@ -62,7 +73,7 @@ def _parameters_module(p, tfin: float, cooling_file: Optional[str]) -> str:
         "  implicit none",
         f"  integer, parameter :: LB = {lb}, RB = {rb}, TB = {tb}, DB = {db}, FB = {fb}, BB = {bb}",
         f"  integer, parameter :: limtr = {int(p.limiter)}",
-        "  integer, parameter :: ndim = 3, neq = 5, size = 0",
+        f"  integer, parameter :: ndim = 3, neq = 5, size = {int(levels)}",
         f"  integer, parameter :: nxtot = {p.nx}, nytot = {p.ny}, nztot = {p.nz}",
         "  integer, parameter :: choice = 2",
         "  integer, parameter :: num_out = 100000",
@@ -91,6 +102,7 @@ def _parameters_module(p, tfin: float, cooling_file: Optional[str]) -> str:
         "  integer, parameter :: nxmax = nx+nghost, nymax = ny+nghost, nzmax = nz+nghost",
         "  integer, parameter :: nxmin = 1-nghost, nymin = 1-nghost, nzmin = 1-nghost",
         "  real, parameter :: dx = (xr-xl)/nxtot, dy = (yr-yl)/nytot, dz = (zr-zl)/nztot",
+        "  integer, parameter :: lvlm = size",
         "  logical, parameter :: logged = .false.",
         "  integer, parameter :: logu = 12",
         "  character(*), parameter :: logfile = './log.dat'",
@@ -100,12 +112,16 @@ def _parameters_module(p, tfin: float, cooling_file: Optional[str]) -> str:
     return "\n".join(lines) + "\n"
 
 
-def _user_module(p) -> str:
+def _user_module(p, gravity: bool = False) -> str:
     winds = list(p.winds)
+    masses = list(p.masses) if gravity else []
+    if masses and winds:
+        raise ValueError("the generated user module takes winds or point masses, not both")
+    nstars = len(masses) if masses else len(winds)
     lines = [
         "module user",
         "  use parameters",
-        "  use globals, only: posstars, Nstrs",
+        "  use globals, only: posstars, Nstrs" + (", PHI, PHIP" if gravity else ""),
         "  use winds",
         "  use pgravity",
         "  implicit none",
@@ -117,8 +133,8 @@ def _user_module(p) -> str:
         "         convert='LITTLE_ENDIAN')",
         "    read(77) U",
         "    close(77)",
-        f"    Nstrs = {len(winds)}",
-        "    allocate(spherical_wind(0:Nstrs-1))",
+        f"    Nstrs = {nstars}",
+        f"    allocate(spherical_wind(0:{len(winds)}-1))",
         "    allocate(posstars(0:2,0:Nstrs-1))",
     ]
     for i, w in enumerate(winds):
@@ -126,12 +142,24 @@ def _user_module(p) -> str:
                               "temp", "mu"), w.as_tuple()):
             lines.append(f"    spherical_wind({i})%{name} = {_r(val)}")
         lines.append(f"    posstars(:,{i}) = (/ {_r(w.xc)}, {_r(w.yc)}, {_r(w.zc)} /)")
+    for i, m in enumerate(masses):
+        lines.append(f"    posstars(:,{i}) = (/ {_r(m.x)}, {_r(m.y)}, {_r(m.z)} /)")
+    if gravity:
+        lines += [
+            "    open(unit=78, file='phi0.bin', access='stream', form='unformatted', status='old', &",
+            "         convert='LITTLE_ENDIAN')",
+            "    read(78) PHI",
+            "    close(78)",
+            "    PHIP = 0.0",
+        ]
+        if masses:
+            lines.append("    call pointmass_potential(PHIP)")
     lines += [
         "  end subroutine user_initconds",
         "  subroutine user_boundaries(U)",
         "    real, dimension(neq, nxmin:nxmax, nymin:nymax, nzmin:nzmax), intent(inout) :: U",
         "    integer :: i",
-        "    do i = 0, Nstrs-1",
+        f"    do i = 0, {len(winds)}-1",
         "       call imposeSphericalWind(spherical_wind(i), U)",
         "    enddo",
         "  end subroutine user_boundaries",
@@ -141,11 +169,15 @@ def _user_module(p) -> str:
 
 
 def build_and_run(name: str, p, u0: np.ndarray, tfin: float, cooling: bool = False,
-                  timeout: float = 600.0) -> Dict[str, object]:
+                  timeout: float = 600.0, gravity: bool = False,
+                  phi0: Optional[np.ndarray] = None) -> Dict[str, object]:
     """Compile the reference for problem ``p`` and run it from ``u0`` (Fortran
-    layout, C shape (nz+4, ny+4, nx+4, 5)) until t >= tfin.
+    layout, C shape (nz+4, ny+4, nx+4, 5)) until t >= tfin.  With ``gravity``,
+    ``phi0`` (C shape (nz+2, ny+2, nx+2)) is the initial PHI; the levels are
+    ``tlaloque_amd.params.mg_levels(p)``.
 
-    Returns {"prims": [UPrim after 0, 1, ... steps], "log": stdout}."""
+    Returns {"prims": [UPrim after 0, 1, ... steps], "log": stdout} and, with
+    gravity, "phit": [PHIT after 1, 2, ... steps]."""
     if not available():
         raise RuntimeError("reference sources or flang not available")
     case = os.path.join(OUT, name)
@@ -155,11 +187,19 @@ def build_and_run(name: str, p, u0: np.ndarray, tfin: float, cooling: bool = Fal
     for d in (src, obj, run, os.path.join(run, "DATA")):
         os.makedirs(d)
     cfile = os.path.join(REF, "Z1.0.dat") if cooling else None
+    levels = 0
+    if gravity:
+        import sys
+        sys.path.insert(0, os.path.dirname(HERE))
+        from tlaloque_amd.params import MSUN, mg_levels
+        levels = mg_levels(p)
+        if any(m.mass != 8.0 * MSUN for m in p.masses) or (p.masses and p.softening != 0.0):
+            raise ValueError("pointsource_gravity.f90:23,32 fixes 8 Msun per star, no softening")
     with open(os.path.join(src, "parameters.f90"), "w") as fh:
-        fh.write(_parameters_module(p, tfin, cfile))
+        fh.write(_parameters_module(p, tfin, cfile, levels))
     with open(os.path.join(src, "user.f90"), "w") as fh:
-        fh.write(_user_module(p))
-    flags = FFLAGS + (["-DCOOL"] if cooling else [])
+        fh.write(_user_module(p, gravity))
+    flags = FFLAGS + (["-DCOOL"] if cooling else []) + (["-DGRAV"] if gravity else [])
     objs: List[str] = []
     order: Sequence[str] = REF_SOURCES_BEFORE_USER + ["user"] + REF_SOURCES_AFTER_USER
     for s in order:
@@ -176,12 +216,21 @@ def build_and_run(name: str, p, u0: np.ndarray, tfin: float, cooling: bool = Fal
     exe = os.path.join(case, "tlaloque_ref")
     subprocess.run([FLANG, *flags, *objs, "-o", exe], check=True, capture_output=True)
     np.ascontiguousarray(u0, dtype="<f8").tofile(os.path.join(run, "ic.bin"))
+    if gravity:
+        ph = np.zeros((p.nz + 2, p.ny + 2, p.nx + 2)) if phi0 is None else phi0
+        np.ascontiguousarray(ph, dtype="<f8").tofile(os.path.join(run, "phi0.bin"))
+        os.makedirs(os.path.join(run, "phi.", "DATA"))
     res = subprocess.run([exe], cwd=run, check=True, capture_output=True, text=True,
                          timeout=timeout)
     files = sorted(glob.glob(os.path.join(run, "DATA", "points000N*.bin")))
     shape = u0.shape
     prims = [np.fromfile(f, dtype="<f8").reshape(shape) for f in files]
-    return {"prims": prims, "log": res.stdout}
+    out = {"prims": prims, "log": res.stdout}
+    if gravity:
+        pf = sorted(glob.glob(os.path.join(run, "phi.", "DATA", "points000N*.bin")))[1:]
+        out["phit"] = [np.fromfile(f, dtype="<f8").reshape(p.nz + 2, p.ny + 2, p.nx + 2)
+                       for f in pf]
+    return out
 
 
 def compile_default() -> Optional[str]:

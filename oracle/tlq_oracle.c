@@ -490,8 +490,21 @@ static inline double potential(const orc_params *p, int i, int j, int k) {
     return phi;
 }
 
-/* S_Solver, source/sources.f90:872-897, with Phi the point-mass potential. */
-void orc_gravity_source(const orc_params *p, const double *prim, double *src) {
+/* PHI(0:nx+1, 0:ny+1, 0:nz+1) of source/globals.f90:19 (i fastest). */
+static inline size_t phc(const orc_params *p, int i, int j, int k) {
+    return (size_t)i + (size_t)(p->nx + 2) * ((size_t)j + (size_t)(p->ny + 2) * (size_t)k);
+}
+
+/* PHIT = PHIP + PHI (HLLC.f90:22) at a cell: the point masses' potential
+ * plus, when phi is given, the self-gravity potential. */
+static inline double phit(const orc_params *p, const double *phi, int i, int j, int k) {
+    const double pp = potential(p, i, j, k);
+    return phi ? pp + phi[phc(p, i, j, k)] : pp;
+}
+
+/* S_Solver, source/sources.f90:872-897, with Phi = PHIT. */
+void orc_gravity_source_phi(const orc_params *p, const double *prim, const double *phi,
+                            double *src) {
     const double dx = (p->xr - p->xl) / p->nx, dy = (p->yr - p->yl) / p->ny,
                  dz = (p->zr - p->zl) / p->nz;
     memset(src, 0, orc_array_len(p) * sizeof(double));
@@ -499,9 +512,9 @@ void orc_gravity_source(const orc_params *p, const double *prim, double *src) {
     for (int k = 1; k <= p->nz; ++k)
         for (int j = 1; j <= p->ny; ++j)
             for (int i = 1; i <= p->nx; ++i) {
-                const double g0 = (potential(p, i + 1, j, k) - potential(p, i - 1, j, k)) / (2.0 * dx);
-                const double g1 = (potential(p, i, j + 1, k) - potential(p, i, j - 1, k)) / (2.0 * dy);
-                const double g2 = (potential(p, i, j, k + 1) - potential(p, i, j, k - 1)) / (2.0 * dz);
+                const double g0 = (phit(p, phi, i + 1, j, k) - phit(p, phi, i - 1, j, k)) / (2.0 * dx);
+                const double g1 = (phit(p, phi, i, j + 1, k) - phit(p, phi, i, j - 1, k)) / (2.0 * dy);
+                const double g2 = (phit(p, phi, i, j, k + 1) - phit(p, phi, i, j, k - 1)) / (2.0 * dz);
                 const size_t c = cell(p, i, j, k);
                 const double *w = prim + c;
                 double *s = src + c;
@@ -511,6 +524,11 @@ void orc_gravity_source(const orc_params *p, const double *prim, double *src) {
                 s[3] = -w[0] * g2;
                 s[4] = -(w[1] * w[0] * g0) - w[2] * w[0] * g1 - w[3] * w[0] * g2;
             }
+}
+
+/* S_Solver with Phi the point-mass potential only. */
+void orc_gravity_source(const orc_params *p, const double *prim, double *src) {
+    orc_gravity_source_phi(p, prim, NULL, src);
 }
 
 /* One iteration of the main loop, source/mainHD.f90:74-99. */
@@ -533,6 +551,382 @@ double orc_advance(const orc_params *p, double *u, double *up, double *upp, doub
     orc_step(p, u, up, upp, dt);                         /* :92 */
     orc_calc_prims(p, prim, u);                          /* :94 */
     if (p->cooling) orc_cooling(p, prim, u, dt);         /* :96-98 */
+    return dt;
+}
+
+/* ------------------------------------------------------------------------
+ * Self-gravity: MultiGrid and its parts, source/sources.f90:99-676, and the
+ * step with the potential solved inside Godunov (HLLC.f90:15-24).
+ *
+ * Level l holds Error(l), Residue(l), ret(l) (globals.f90:22-28), each
+ * (nxl+2)(nyl+2)(nzl+2) with nxl = nx / 2^l (init.f90:94-116), i fastest.
+ * Loops run in the reference's order (i outermost, k innermost), so the
+ * Gauss-Seidel sweeps see the same mix of new and old values. */
+
+typedef struct { int nx, ny, nz; double *e, *r, *t; } mg_level;
+
+static inline size_t lidx(const mg_level *L, int i, int j, int k) {
+    return (size_t)i + (size_t)(L->nx + 2) * ((size_t)j + (size_t)(L->ny + 2) * (size_t)k);
+}
+static inline size_t lsize(int nx, int ny, int nz) {
+    return (size_t)(nx + 2) * (ny + 2) * (nz + 2);
+}
+
+size_t orc_sg_work_len(const orc_params *p, int levels) {
+    size_t n = 0;
+    for (int l = 0; l <= levels; ++l) n += 3 * lsize(p->nx >> l, p->ny >> l, p->nz >> l);
+    return n;
+}
+
+static void mg_levels(const orc_params *p, int levels, double *work, mg_level *L) {
+    for (int l = 0; l <= levels; ++l) {
+        L[l].nx = p->nx >> l;
+        L[l].ny = p->ny >> l;
+        L[l].nz = p->nz >> l;
+        const size_t n = lsize(L[l].nx, L[l].ny, L[l].nz);
+        L[l].e = work;
+        L[l].r = work + n;
+        L[l].t = work + 2 * n;
+        work += 3 * n;
+    }
+}
+
+/* restriction, sources.f90:207-228: 8-cell average onto the coarse level. */
+static void mg_restriction(const mg_level *F, const double *var, const mg_level *C, double *vt) {
+    for (int i = 1; i <= C->nx; ++i)
+        for (int j = 1; j <= C->ny; ++j)
+            for (int k = 1; k <= C->nz; ++k) {
+                const int a = 2 * i - 1, b = 2 * j - 1, c = 2 * k - 1;
+#define V(ii, jj, kk) var[lidx(F, ii, jj, kk)]
+                const double s = V(a, b, c) + V(a + 1, b, c) + V(a, b + 1, c) + V(a, b, c + 1) +
+                                 V(a + 1, b + 1, c) + V(a + 1, b, c + 1) + V(a, b + 1, c + 1) +
+                                 V(a + 1, b + 1, c + 1);
+#undef V
+                vt[lidx(C, i, j, k)] = (1.0 / 8.0) * s;
+            }
+}
+
+/* prolongation, sources.f90:230-268: trilinear 27/9/3/1 weights from the
+ * coarse level (ghosts included) onto the 8 children. */
+static void mg_prolongation(const mg_level *C, const double *var, const mg_level *F, double *vt) {
+    for (int i = 1; i <= C->nx; ++i)
+        for (int j = 1; j <= C->ny; ++j)
+            for (int k = 1; k <= C->nz; ++k) {
+                const int a = 2 * i - 1, b = 2 * j - 1, c = 2 * k - 1;
+#define V(ii, jj, kk) var[lidx(C, ii, jj, kk)]
+                for (int dc = 0; dc < 2; ++dc)
+                    for (int db = 0; db < 2; ++db)
+                        for (int da = 0; da < 2; ++da) {
+                            const int si = da ? 1 : -1, sj = db ? 1 : -1, sk = dc ? 1 : -1;
+                            const double s = 27.0 * V(i, j, k) + 9.0 * V(i + si, j, k) +
+                                             9.0 * V(i, j + sj, k) + 9.0 * V(i, j, k + sk) +
+                                             3.0 * V(i + si, j + sj, k) + 3.0 * V(i + si, j, k + sk) +
+                                             3.0 * V(i, j + sj, k + sk) + V(i + si, j + sj, k + sk);
+                            vt[lidx(F, a + da, b + db, c + dc)] = (1.0 / 64.0) * s;
+                        }
+#undef V
+            }
+}
+
+/* resid, sources.f90:270-300: resi = source - Laplacian(var) (ghosts: the
+ * Laplacian is 0 there, so resi = source). */
+static void mg_resid(const orc_params *p, const mg_level *L, const double *var, const double *src,
+                     double *resi) {
+    const double dxl = (p->xr - p->xl) / (double)L->nx;
+    const size_t n = lsize(L->nx, L->ny, L->nz);
+    for (size_t q = 0; q < n; ++q) resi[q] = src[q];
+    for (int i = 1; i <= L->nx; ++i)
+        for (int j = 1; j <= L->ny; ++j)
+            for (int k = 1; k <= L->nz; ++k) {
+                const size_t c = lidx(L, i, j, k);
+                const size_t sx = 1, sy = (size_t)(L->nx + 2), sz = sy * (size_t)(L->ny + 2);
+                const double g = (var[c - sx] + var[c + sx] + var[c - sy] + var[c + sy] +
+                                  var[c - sz] + var[c + sz] - 6.0 * var[c]) /
+                                 (dxl * dxl);
+                resi[c] = src[c] - g;
+            }
+}
+
+/* The correction levels' boundary (sources.f90:337-344 / 400-407, serial
+ * branch): each ghost face is minus the first interior plane, in the order
+ * x-low, y-low, z-low, x-high, y-high, z-high, whole planes. */
+static void mg_bc(const mg_level *L, double *v) {
+    const int nx = L->nx, ny = L->ny, nz = L->nz;
+    for (int k = 0; k <= nz + 1; ++k)
+        for (int j = 0; j <= ny + 1; ++j) v[lidx(L, 0, j, k)] = -v[lidx(L, 1, j, k)];
+    for (int k = 0; k <= nz + 1; ++k)
+        for (int i = 0; i <= nx + 1; ++i) v[lidx(L, i, 0, k)] = -v[lidx(L, i, 1, k)];
+    for (int j = 0; j <= ny + 1; ++j)
+        for (int i = 0; i <= nx + 1; ++i) v[lidx(L, i, j, 0)] = -v[lidx(L, i, j, 1)];
+    for (int k = 0; k <= nz + 1; ++k)
+        for (int j = 0; j <= ny + 1; ++j) v[lidx(L, nx + 1, j, k)] = -v[lidx(L, nx, j, k)];
+    for (int k = 0; k <= nz + 1; ++k)
+        for (int i = 0; i <= nx + 1; ++i) v[lidx(L, i, ny + 1, k)] = -v[lidx(L, i, ny, k)];
+    for (int j = 0; j <= ny + 1; ++j)
+        for (int i = 0; i <= nx + 1; ++i) v[lidx(L, i, j, nz + 1)] = -v[lidx(L, i, j, nz)];
+}
+
+/* One Gauss-Seidel update (w = 1), sources.f90:362-365; returns |change|. */
+static inline double mg_update(const mg_level *L, double *v, const double *src, int i, int j, int k,
+                               double dxl) {
+    const size_t c = lidx(L, i, j, k);
+    const size_t sy = (size_t)(L->nx + 2), sz = sy * (size_t)(L->ny + 2);
+    const double w = 1.0;
+    const double d1 = v[c];
+    v[c] = ((1.0 / 6.0) * (v[c + 1] + v[c - 1] + v[c + sy] + v[c - sy] + v[c + sz] + v[c - sz] -
+                           src[c] * (dxl * dxl))) * w + (1.0 - w) * v[c];
+    return fabs(v[c] - d1);
+}
+
+/* Iterations cap of the coarsest-level solve: the reference iterates until
+ * Dif <= 0.001 with no cap (sources.f90:347-376). */
+#define ORC_MG_SOLVE_CAP 1000000
+
+/* poisson_solver, sources.f90:302-485, serial branch.  relsolv 1 = Solve
+ * (iterate to Dif <= 0.001), 2 = Relax (it-1 iterations of two sweeps, the
+ * column's start plane depending on (i, j) parity as written). */
+static void mg_poisson(const orc_params *p, const mg_level *L, const double *src, double *v,
+                       int relsolv, int it, double *dif, int level) {
+    const double dxl = (p->xr - p->xl) / (double)L->nx;
+    if (level > 0) mg_bc(L, v);
+    if (relsolv == 1) {
+        const double er = 0.001;
+        *dif = 1000.0;
+        int iter = 0;
+        while (*dif > er && iter < ORC_MG_SOLVE_CAP) {
+            double difp = 0.0;
+            for (int i = 1; i <= L->nx; ++i)
+                for (int j = 1; j <= L->ny; ++j)
+                    for (int k = 1; k <= L->nz; ++k) difp += mg_update(L, v, src, i, j, k, dxl);
+            ++iter;
+            *dif = difp;
+        }
+        return;
+    }
+    for (int iter = 1; iter <= it - 1; ++iter) {
+        double difp = 0.0;
+        for (int i = 1; i <= L->nx; ++i)
+            for (int j = 1; j <= L->ny; ++j) {
+                const int start = ((i % 2 == 0) && (j % 2 == 1)) ? 1 : 2;
+                for (int k = start; k <= L->nz; k += 2) difp += mg_update(L, v, src, i, j, k, dxl);
+            }
+        for (int i = 1; i <= L->nx; ++i)
+            for (int j = 1; j <= L->ny; ++j) {
+                const int start = ((i % 2 == 0) && (j % 2 == 1)) ? 2 : 1;
+                for (int k = start; k <= L->nz; k += 2) difp += mg_update(L, v, src, i, j, k, dxl);
+            }
+        *dif = difp;
+    }
+}
+
+/* Legendre / aLegendre, sources.f90:9-97 (associated functions with the
+ * Condon-Shortley phase). */
+static double legendre(double x, int l) {
+    switch (l) {
+    case 0: return 1.0;
+    case 1: return x;
+    case 2: return 0.5 * (3.0 * (x * x) - 1.0);
+    case 3: return 0.5 * (5.0 * (x * x * x) - 3.0 * x);
+    default: return (1.0 / 8.0) * (35.0 * (x * x * x * x) - 30.0 * (x * x) + 3.0);
+    }
+}
+static double alegendre(double x, int l, int m) {
+    const double s = pow(1.0 - x * x, 0.5);
+    switch (l * 8 + m) {
+    case 0: return 1.0;
+    case 8: return x;
+    case 9: return -s;
+    case 16: return 0.5 * (3.0 * (x * x) - 1.0);
+    case 17: return -3.0 * x * s;
+    case 18: return 3.0 * (1.0 - x * x);
+    case 24: return 0.5 * (5.0 * (x * x * x) - 3.0 * x);
+    case 25: return (3.0 / 2.0) * (1.0 - 5.0 * (x * x)) * s;
+    case 26: return 15.0 * x * (1.0 - x * x);
+    case 27: return -15.0 * pow(1.0 - x * x, 1.5);
+    case 32: return (1.0 / 8.0) * (35.0 * (x * x * x * x) - 30.0 * (x * x) + 3.0);
+    case 33: return -(5.0 / 2.0) * (7.0 * (x * x * x) - 3.0 * x) * s;
+    case 34: return (15.0 / 2.0) * (7.0 * (x * x) - 1.0) * (1.0 - x * x);
+    case 35: return -105.0 * x * pow(1.0 - x * x, 1.5);
+    default: return 105.0 * ((1.0 - x * x) * (1.0 - x * x));
+    }
+}
+
+static inline double ipow(double r, int l) {
+    double v = 1.0;
+    for (int q = 0; q < l; ++q) v *= r;
+    return v;
+}
+
+/* Spherical angles of a cell centre: pangle, aangle, distance
+ * (sources.f90:556-597). */
+static inline void sph(const orc_params *p, int i, int j, int k, double *r, double *ct, double *fi) {
+    const double x = xcoord(p, i), y = ycoord(p, j), z = zcoord(p, k);
+    const double rr = sqrt(x * x + y * y + z * z);
+    const double the = acos(z / rr);
+    double f = atan2(y, x);
+    if (f < 0) f = f + 2. * ORC_PI;
+    *r = rr;
+    *ct = cos(the);
+    *fi = f;
+}
+
+/* Moments, sources.f90:599-676, with the sums the MPI build reduces (one
+ * rank: the local sums).  The serial build reads cmoment, smoment and
+ * c0moment uninitialised (sources.f90:642-652).  c0[0..4]; cm, sm [l-1][m-1]. */
+void orc_moments(const orc_params *p, const double *rho, double *c0, double *cm, double *sm) {
+    const double dx = (p->xr - p->xl) / p->nx, dy = (p->yr - p->yl) / p->ny,
+                 dz = (p->zr - p->zl) / p->nz;
+    for (int q = 0; q < 16; ++q) cm[q] = sm[q] = 0.0;
+    for (int l = 0; l <= 4; ++l) c0[l] = 0.0;
+    for (int l = 1; l <= 4; ++l)
+        for (int m = 1; m <= l; ++m) {
+            double c0mp = 0.0, cmp = 0.0, smp = 0.0;
+            const double c = 2.0 * tgamma(l - m + 1.0) / tgamma(l + m + 1.0);
+            for (int i = 1; i <= p->nx; ++i)
+                for (int j = 1; j <= p->ny; ++j)
+                    for (int k = 1; k <= p->nz; ++k) {
+                        double r, ct, fi;
+                        sph(p, i, j, k, &r, &ct, &fi);
+                        const double rl = ipow(r, l), d = rho[phc(p, i, j, k)];
+                        c0mp = c0mp + legendre(ct, l) * rl * d * dx * dy * dz;
+                        cmp = cmp + alegendre(ct, l, m) * cos(m * fi) * rl * d * dx * dy * dz;
+                        smp = smp + alegendre(ct, l, m) * sin(m * fi) * rl * d * dx * dy * dz;
+                    }
+            cm[(l - 1) * 4 + (m - 1)] = c * cmp;
+            sm[(l - 1) * 4 + (m - 1)] = c * smp;
+            c0[l] = c0mp;
+        }
+    double c0mp = 0.0;
+    for (int i = 1; i <= p->nx; ++i)
+        for (int j = 1; j <= p->ny; ++j)
+            for (int k = 1; k <= p->nz; ++k) {
+                double r, ct, fi;
+                sph(p, i, j, k, &r, &ct, &fi);
+                c0mp = c0mp + legendre(ct, 0) * ipow(r, 0) * rho[phc(p, i, j, k)] * dx * dy * dz;
+            }
+    c0[0] = c0mp;
+}
+
+/* The multipole potential at one ghost cell, phi_[xyz]boundaries
+ * (sources.f90:678-787): the m > 0 terms carry a factor 2 on top of the
+ * one already in cm, sm (as written there). */
+static double multipole(const orc_params *p, double gconst, const double *c0, const double *cm,
+                        const double *sm, int i, int j, int k) {
+    double rb, ct, fi;
+    sph(p, i, j, k, &rb, &ct, &fi);
+    double phi = 0.0;
+    phi = phi - c0[0] / rb;
+    for (int l = 1; l <= 4; ++l) {
+        phi = phi - c0[l] * legendre(ct, l) / ipow(rb, l + 1);
+        for (int m = 1; m <= l; ++m)
+            phi = phi - 2.0 * (cm[(l - 1) * 4 + m - 1] * cos(m * fi) +
+                               sm[(l - 1) * 4 + m - 1] * sin(m * fi)) *
+                            alegendre(ct, l, m) / ipow(rb, l + 1);
+    }
+    return gconst * phi;
+}
+
+/* The six faces of phi (face cells only, edges untouched), in MultiGrid's
+ * order: x-low, x-high, y-low, y-high, z-low, z-high (sources.f90:117-136). */
+void orc_phi_boundaries(const orc_params *p, double gconst, const double *c0, const double *cm,
+                        const double *sm, double *phi) {
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    for (int side = 0; side < 2; ++side) {
+        const int ib = side ? nx + 1 : 0;
+        for (int j = 1; j <= ny; ++j)
+            for (int k = 1; k <= nz; ++k) phi[phc(p, ib, j, k)] = multipole(p, gconst, c0, cm, sm, ib, j, k);
+    }
+    for (int side = 0; side < 2; ++side) {
+        const int jb = side ? ny + 1 : 0;
+        for (int i = 1; i <= nx; ++i)
+            for (int k = 1; k <= nz; ++k) phi[phc(p, i, jb, k)] = multipole(p, gconst, c0, cm, sm, i, jb, k);
+    }
+    for (int side = 0; side < 2; ++side) {
+        const int kb = side ? nz + 1 : 0;
+        for (int i = 1; i <= nx; ++i)
+            for (int j = 1; j <= ny; ++j) phi[phc(p, i, j, kb)] = multipole(p, gconst, c0, cm, sm, i, j, kb);
+    }
+}
+
+/* MultiGrid, sources.f90:99-205.  dens and sg->phi are (0:n+1)^3 arrays.
+ * boundary 2 refreshes phi's faces from the multipole moments when
+ * mod(it, 10) = 0 (the MPI build at the domain edge); boundary 1 keeps
+ * them (the serial build, whose neighbour ranks are never negative).
+ * Returns the number of V-cycles. */
+int orc_multigrid(const orc_params *p, orc_sg *sg, const double *dens) {
+    mg_level L[32];
+    const int lv = sg->levels;
+    mg_levels(p, lv, sg->work, L);
+    if (sg->boundary == 2 && sg->it % 10 == 0) {
+        double c0[5], cm[16], sm[16];
+        orc_moments(p, dens, c0, cm, sm);
+        orc_phi_boundaries(p, sg->gconst, c0, cm, sm, sg->phi);
+    }
+    const size_t n0 = lsize(p->nx, p->ny, p->nz);
+    const double fpg = 4.0 * ORC_PI * sg->gconst;
+    for (size_t q = 0; q < n0; ++q) L[0].r[q] = fpg * dens[q];
+    memcpy(L[0].e, sg->phi, n0 * sizeof(double));
+    double dif = 1.e28;
+    int iter = 0;
+    const int its = 6;
+    while (dif > 10.0 && iter < 31) {
+        ++iter;
+        for (int l = 1; l <= lv; ++l)
+            memset(L[l].e, 0, lsize(L[l].nx, L[l].ny, L[l].nz) * sizeof(double));
+        mg_poisson(p, &L[0], L[0].r, L[0].e, 2, its, &dif, 0);
+        for (int l = 1; l <= lv; ++l) {
+            mg_resid(p, &L[l - 1], L[l - 1].e, L[l - 1].r, L[l - 1].t);
+            mg_restriction(&L[l - 1], L[l - 1].t, &L[l], L[l].r);
+            mg_poisson(p, &L[l], L[l].r, L[l].e, l == lv ? 1 : 2, its, &dif, l);
+        }
+        for (int l = lv - 1; l >= 0; --l) {
+            const size_t n = lsize(L[l].nx, L[l].ny, L[l].nz);
+            memset(L[l].t, 0, n * sizeof(double));
+            mg_prolongation(&L[l + 1], L[l + 1].e, &L[l], L[l].t);
+            for (size_t q = 0; q < n; ++q) L[l].e[q] = L[l].e[q] + L[l].t[q];
+            mg_poisson(p, &L[l], L[l].r, L[l].e, 2, its, &dif, l);
+        }
+    }
+    memcpy(sg->phi, L[0].e, n0 * sizeof(double));
+    sg->cycles = iter;
+    sg->dif = dif;
+    return iter;
+}
+
+/* dens = UU(1, 0:nx+1, 0:ny+1, 0:nz+1) (HLLC.f90:16, mainHD.f90:51). */
+void orc_density(const orc_params *p, const double *u, double *dens) {
+    for (int k = 0; k <= p->nz + 1; ++k)
+        for (int j = 0; j <= p->ny + 1; ++j)
+            for (int i = 0; i <= p->nx + 1; ++i) dens[phc(p, i, j, k)] = u[cell(p, i, j, k)];
+}
+
+/* mainHD.f90:74-99 with self-gravity: each Godunov call (:80, :88) solves
+ * the potential from U^n's density, starting from the last solution, and
+ * adds S_Solver's source with PHIT = PHIP + PHI (HLLC.f90:15-24).  dens is
+ * scratch of (nx+2)(ny+2)(nz+2) doubles.  Returns dt; sg->it advances. */
+double orc_advance_sg(const orc_params *p, double *u, double *up, double *upp, double *prim,
+                      double *ff, double *gg, double *hh, double *src, orc_sg *sg, double *dens) {
+    const double dt = orc_time_step(p, prim);            /* :74 */
+    orc_fluxes(p, u, prim, ff, gg, hh, 1);               /* :76 */
+    const double dth = dt / 2.0;                         /* :78 */
+    orc_density(p, u, dens);
+    orc_multigrid(p, sg, dens);
+    orc_gravity_source_phi(p, prim, sg->phi, src);
+    orc_godunov(p, upp, u, ff, gg, hh, src, dth);        /* :80 */
+    orc_boundaries2(p, upp);                             /* :82 */
+    orc_winds(p, upp);
+    orc_calc_prims(p, prim, upp);                        /* :84 */
+    orc_fluxes(p, upp, prim, ff, gg, hh, 2);             /* :86 */
+    orc_density(p, u, dens);
+    orc_multigrid(p, sg, dens);
+    orc_gravity_source_phi(p, prim, sg->phi, src);
+    orc_godunov(p, up, u, ff, gg, hh, src, dt);          /* :88 */
+    orc_boundaries1(p, up);                              /* :90 */
+    orc_winds(p, up);
+    orc_step(p, u, up, upp, dt);                         /* :92 */
+    orc_calc_prims(p, prim, u);                          /* :94 */
+    if (p->cooling) orc_cooling(p, prim, u, dt);         /* :96-98 */
+    sg->it += 1;
     return dt;
 }
 

@@ -78,6 +78,29 @@ void orc_hll_face(const orc_params *p, const double *pl, const double *pr,
 double orc_advance(const orc_params *p, double *u, double *up, double *upp, double *prim,
                    double *ff, double *gg, double *hh, double *src);
 
+/* Self-gravity (source/sources.f90:99-897, HLLC.f90:15-24). */
+typedef struct orc_sg {
+    int levels;      /* lvlm, user/parameters.f90:94 */
+    int boundary;    /* 1: phi's ghost faces kept (serial build); 2: multipole
+                        values every 10 steps (MPI build, domain edge) */
+    double gconst;   /* Gconst, user/parameters.f90:45 */
+    double *phi;     /* PHI(0:nx+1, 0:ny+1, 0:nz+1), i fastest */
+    double *work;    /* orc_sg_work_len doubles: Error, Residue, ret per level */
+    int it;          /* step counter (globals it) */
+    int cycles;      /* out: V-cycles of the last MultiGrid */
+    double dif;      /* out: its final Dif */
+} orc_sg;
+size_t orc_sg_work_len(const orc_params *p, int levels);
+int orc_multigrid(const orc_params *p, orc_sg *sg, const double *dens);
+void orc_moments(const orc_params *p, const double *rho, double *c0, double *cm, double *sm);
+void orc_phi_boundaries(const orc_params *p, double gconst, const double *c0, const double *cm,
+                        const double *sm, double *phi);
+void orc_density(const orc_params *p, const double *u, double *dens);
+void orc_gravity_source_phi(const orc_params *p, const double *prim, const double *phi,
+                            double *src);
+double orc_advance_sg(const orc_params *p, double *u, double *up, double *upp, double *prim,
+                      double *ff, double *gg, double *hh, double *src, orc_sg *sg, double *dens);
+
 /* Cooling integrated to convergence (the reference formula sub-cycled). */
 double orc_cool_subcycled(const orc_params *p, double n, double T, double dt, double eps,
                           int max_sub, int *nsub);

@@ -8,7 +8,11 @@ seeded initial state for a few steps, and stores
                   in the reference's Fortran (e, i, j, k) order;
   * ``prims``  -- UPrim after 0, 1 and N steps (the reference's BIN output);
   * ``steps``  -- the step counts of those snapshots;
-  * ``params`` -- the problem as JSON (fields of tlaloque_amd.params.HydroParams).
+  * ``params`` -- the problem as JSON (fields of tlaloque_amd.params.HydroParams);
+  * for the self-gravity cases (``ref_sg_*``, built with -DGRAV) also
+    ``phi0`` -- the initial PHI, C shape (nz+2, ny+2, nx+2), whose ghost
+                faces are the boundary values the serial build keeps;
+    ``phit`` -- PHIT = PHIP + PHI after 1 and N steps (HydroCore.f90:284-290).
 
 Run in the build container only (needs /root/reference and flang):
     python tests/golden/make_ref_fixtures.py
@@ -104,6 +108,78 @@ def cases():
     return out
 
 
+def sphere_state(p, r0, rho_in, rho_out, pres):
+    """A dense sphere of radius r0 at rest in uniform pressure (cgs)."""
+    x = P.xcoord(p, np.arange(-1, p.nx + 3))
+    y = P.ycoord(p, np.arange(-1, p.ny + 3))
+    z = P.zcoord(p, np.arange(-1, p.nz + 3))
+    r = np.sqrt(x[None, None, :] ** 2 + y[None, :, None] ** 2 + z[:, None, None] ** 2)
+    rho = np.where(r < r0, rho_in, rho_out)
+    return cons_from_prims(p, rho, np.zeros(r.shape + (3,)), np.full(r.shape, pres))
+
+
+def point_phi0(p, u0):
+    """PHI with zero interior and -G M / r on the ghost layer (M: the mesh's
+    mass at the origin)."""
+    mass = u0[2:-2, 2:-2, 2:-2, 0].sum() * p.dx * p.dy * p.dz
+    x = P.xcoord(p, np.arange(0, p.nx + 2))
+    y = P.ycoord(p, np.arange(0, p.ny + 2))
+    z = P.zcoord(p, np.arange(0, p.nz + 2))
+    r = np.sqrt(x[None, None, :] ** 2 + y[None, :, None] ** 2 + z[:, None, None] ** 2)
+    phi = -P.GR * mass / r
+    phi[1:-1, 1:-1, 1:-1] = 0.0
+    return phi
+
+
+def gravity_cases():
+    out = []
+    # Self-gravity of a dense sphere, 16^3 (four levels to 1^3), outflow.
+    L = 1.0 * P.PC
+    p = P.HydroParams(nx=16, ny=16, nz=16, xl=-L, xr=L, yl=-L, yr=L, zl=-L, zr=L,
+                      bc=(P.BC_OUTFLOW,) * 6, eta=0.02, self_gravity=P.SG_FIXED)
+    u0 = sphere_state(p, 0.5 * L, 1e-20, 1e-22, 1e-10)
+    out.append(("sg_sphere", p, u0, 4, point_phi0(p, u0)))
+    # A 16x16x8 box (three levels to 2x2x1), mixed boundaries, an off-centre
+    # sphere and a point mass (8 Msun, as pointsource_gravity.f90:23 fixes).
+    p = P.HydroParams(nx=16, ny=16, nz=8, xl=-L, xr=L, yl=-L, yr=L, zl=-0.5 * L, zr=0.5 * L,
+                      bc=(P.BC_OUTFLOW, P.BC_REFLECTIVE, P.BC_PERIODIC, P.BC_PERIODIC,
+                          P.BC_OUTFLOW, P.BC_OUTFLOW),
+                      eta=0.02, self_gravity=P.SG_FIXED,
+                      masses=[P.PointMass(0.31 * L, -0.22 * L, 0.13 * L, 8.0 * P.MSUN)])
+    u0 = sphere_state(p, 0.3 * L, 3e-20, 1e-22, 3e-10)
+    rng = np.random.default_rng(51)
+    u0[..., 1:4] += u0[..., :1] * 1e5 * rng.uniform(-1, 1, u0.shape[:3] + (3,))
+    u0[..., 4] = 3e-10 / (p.gamma - 1.0) + 0.5 * (u0[..., 1:4] ** 2).sum(-1) / u0[..., 0]
+    out.append(("sg_mass_box", p, u0, 4, point_phi0(p, u0)))
+    return out
+
+
+def main_gravity():
+    for name, p, u0, nsteps, phi0 in gravity_cases():
+        o = oracle.Oracle(p)
+        o.set_phi(phi0)
+        o.set_cons(u0)
+        ts = [0.0]
+        phis = []
+        for _ in range(nsteps):
+            o.advance()
+            ts.append(o.t)
+            phis.append(o.phi.copy())
+        tfin = 0.5 * (ts[-2] + ts[-1])
+        r = ref_build.build_and_run(name, p, u0, tfin, gravity=True, phi0=phi0)
+        prims, phit = r["prims"], r["phit"]
+        if len(prims) != nsteps + 1 or len(phit) != nsteps:
+            raise RuntimeError(f"{name}: reference ran {len(prims) - 1} steps, expected {nsteps}")
+        keep = [0, 1, nsteps]
+        np.savez_compressed(os.path.join(OUT, f"ref_{name}.npz"), u0=u0,
+                            prims=np.stack([prims[s] for s in keep]), steps=np.array(keep),
+                            params=np.array(params_json(p)), phi0=phi0,
+                            phit=np.stack([phit[0], phit[-1]]))
+        sl = (slice(2, -2),) * 3
+        d = float(np.abs(o.prim[sl] - prims[-1][sl]).max())
+        print(f"{name}: {nsteps} steps, oracle-vs-reference max |diff| interior = {d:.3e}")
+
+
 def params_json(p) -> str:
     d = dataclasses.asdict(p)
     return json.dumps(d)
@@ -133,4 +209,6 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if "--gravity" not in sys.argv:
+        main()
+    main_gravity()

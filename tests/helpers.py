@@ -27,7 +27,21 @@ def params_from_json(s: str) -> P.HydroParams:
 
 
 def ref_cases():
-    return sorted(os.path.basename(f)[4:-4] for f in glob.glob(os.path.join(GOLDEN, "ref_*.npz")))
+    """The hydro reference runs (the self-gravity ones are ``sg_ref_cases``)."""
+    return sorted(os.path.basename(f)[4:-4] for f in glob.glob(os.path.join(GOLDEN, "ref_*.npz"))
+                  if not os.path.basename(f).startswith("ref_sg_"))
+
+
+def sg_ref_cases():
+    """The reference runs built with -DGRAV (self-gravity)."""
+    return sorted(os.path.basename(f)[4:-4] for f in glob.glob(os.path.join(GOLDEN, "ref_sg_*.npz")))
+
+
+def load_ref_sg(name: str):
+    """(params, u0, prims, steps, phi0, phit) of a -DGRAV reference run."""
+    z = np.load(os.path.join(GOLDEN, f"ref_{name}.npz"), allow_pickle=False)
+    p = params_from_json(str(z["params"]))
+    return p, z["u0"], z["prims"], [int(s) for s in z["steps"]], z["phi0"], z["phit"]
 
 
 def load_ref(name: str):

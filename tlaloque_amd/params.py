@@ -33,6 +33,10 @@ LIMITER_NONE, LIMITER_VANLEER, LIMITER_MINMOD, LIMITER_UMIST, LIMITER_WOODWARD, 
 SOLVER_HLLC, SOLVER_HLL = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1          # FP32: FP32 storage and sweeps, FP64 guards
 COOL_OFF, COOL_EXACT, COOL_SURROGATE, COOL_SUBCYCLED = 0, 1, 2, 3
+# Self-gravity (source/sources.f90:99-205): off; phi's ghost faces kept as set
+# (the serial build); multipole values on them every 10 steps (the MPI build
+# at the domain edge).
+SG_OFF, SG_FIXED, SG_MULTIPOLE = 0, 1, 2
 
 
 @dataclass
@@ -88,6 +92,9 @@ class HydroParams:
     winds: List[Wind] = field(default_factory=list)
     masses: List[PointMass] = field(default_factory=list)
     softening: float = 0.0
+    self_gravity: int = SG_OFF
+    mg_levels: int = 0          # lvlm (user/parameters.f90:94); 0: as many as the mesh halves
+    gconst: float = GR          # Gconst (user/parameters.f90:45)
 
     @property
     def dx(self) -> float:
@@ -112,6 +119,18 @@ class HydroParams:
 
     def replace(self, **kw) -> "HydroParams":
         return dataclasses.replace(self, **kw)
+
+
+def mg_levels(p: HydroParams) -> int:
+    """Multigrid levels below the mesh: p.mg_levels, or the most halvings
+    that keep nx, ny, nz whole (nx = mpix 2^size with lvlm = size,
+    user/parameters.f90:26,94)."""
+    if p.mg_levels > 0:
+        return int(p.mg_levels)
+    L = 0
+    while all(n % (2 << L) == 0 for n in (p.nx, p.ny, p.nz)):
+        L += 1
+    return L
 
 
 def xcoord(p: HydroParams, i):
