@@ -65,6 +65,14 @@ extern "C" {
  * fallback; one substep is exactly cooling.f90:67-112). */
 #define TLQ_COOL_SUBCYCLED 3
 
+/* Self-gravity (the -DGRAV build, sources.f90:99-897): off; on with the
+ * potential's ghost faces kept as uploaded (the serial build, whose neighbour
+ * ranks are never negative, sources.f90:114-136); on with multipole values on
+ * the faces every 10 steps (the MPI build at the domain edge). */
+#define TLQ_SG_OFF 0
+#define TLQ_SG_FIXED 1
+#define TLQ_SG_MULTIPOLE 2
+
 /* Device arrays of the reference's globals (source/globals.f90:5-11). */
 #define TLQ_BUF_U 0
 #define TLQ_BUF_UP 1
@@ -90,6 +98,9 @@ typedef struct tlq_params {
     int precision;                  /* TLQ_FP64 / TLQ_FP32 */
     int cooling;                    /* TLQ_COOL_* */
     double softening;               /* Plummer length of the point masses (0: 1/r) */
+    int self_gravity;               /* TLQ_SG_*: MultiGrid inside Godunov (HLLC.f90:15-24) */
+    int mg_levels;                  /* lvlm (parameters.f90:94); 0: as many as the mesh halves */
+    double gconst;                  /* Gconst (parameters.f90:45); GR for the reference's */
 } tlq_params;
 
 /* spherical_wind_type, source/winds.f90:5-13. */
@@ -145,6 +156,22 @@ int tlq_get_time(tlq_ctx *ctx, double *dt, double *t, int *it);
  * (warmstart.f90:1-10; its autosave #warmstart.f90#:11-13 reads a VTK). */
 int tlq_resume(tlq_ctx *ctx, double t, int it, int fill_ghosts);
 int tlq_sync(tlq_ctx *ctx);
+
+/* Self-gravity.  PHI(0:nx+1, 0:ny+1, 0:nz+1) (globals.f90:19, i fastest) in
+ * FP64: upload its initial values (the faces are the boundary values) before
+ * tlq_init_state, as user_initconds sets PHI (user.f90:139-141), and read it
+ * back.  tlq_init_state then solves once (mainHD.f90:50-52); every fused step
+ * solves once from U^n and both stages add S_Solver's source with
+ * PHIT = PHIP + PHI (HLLC.f90:15-24; the reference solves twice from the same
+ * U^n, mainHD.f90:80,88); tlq_godunov solves at every call, as the
+ * reference.  tlq_gravity_solve runs one MultiGrid call on the current U at
+ * the current step count; *cycles and *dif receive its V-cycles and final
+ * Dif.  tlq_gravity_info: levels (-1 when off), the last solve's cycles and
+ * Dif, and the step count the mod(it, 10) test reads. */
+int tlq_upload_phi(tlq_ctx *ctx, const double *phi);
+int tlq_download_phi(tlq_ctx *ctx, double *phi);
+int tlq_gravity_solve(tlq_ctx *ctx, int *cycles, double *dif);
+int tlq_gravity_info(tlq_ctx *ctx, int *levels, int *cycles, double *dif, int *it);
 
 /* Diagnostics: device-side check that every interior density is finite and
  * positive and every energy finite; *bad receives the count of bad cells. */

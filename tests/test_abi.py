@@ -58,6 +58,11 @@ _BAD = [
     ("cooling", dict(cooling=4), -1, "cooling"),
     ("gamma", dict(gamma=1.0), -1, "physical"),
     ("domain", dict(xr=-2.0), -1, "physical"),
+    ("self_gravity id", dict(self_gravity=3), -1, "self_gravity"),
+    ("non-cubic cells", dict(self_gravity=1, xr=3.0), -4, "cubic"),
+    ("mg_levels", dict(self_gravity=1, mg_levels=4), -1, "divisible"),
+    ("coarsest level", dict(self_gravity=1, nx=34, ny=34, nz=34), -4, "4096"),
+    ("gconst", dict(self_gravity=2, gconst=-1.0), -1, "gconst"),
 ]
 
 

@@ -79,6 +79,8 @@ program layout
   print '(i0)', transfer(c_loc(p%solver), b) - b
   print '(i0)', transfer(c_loc(p%cooling), b) - b
   print '(i0)', transfer(c_loc(p%softening), b) - b
+  print '(i0)', transfer(c_loc(p%self_gravity), b) - b
+  print '(i0)', transfer(c_loc(p%gconst), b) - b
   print '(i0)', c_sizeof(w)
 end program layout
 """)
@@ -90,7 +92,8 @@ end program layout
                                           check=True).stdout.split()]
     want = [ctypes.sizeof(TlqParams), TlqParams.xl.offset, TlqParams.limiter.offset,
             TlqParams.bc.offset, TlqParams.solver.offset, TlqParams.cooling.offset,
-            TlqParams.softening.offset, ctypes.sizeof(TlqWind)]
+            TlqParams.softening.offset, TlqParams.self_gravity.offset, TlqParams.gconst.offset,
+            ctypes.sizeof(TlqWind)]
     assert got == want
 
 

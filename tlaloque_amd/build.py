@@ -27,6 +27,7 @@ UNITS = [
     ("tlq_fp64.hip", ["-ffp-contract=off"]),
     ("tlq_fp32.hip", []),
     ("tlq_surrogate.hip", []),
+    ("tlq_gravity.hip", ["-ffp-contract=off"]),
     ("tlq_api.hip", []),
 ]
 

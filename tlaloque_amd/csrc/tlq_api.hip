@@ -14,6 +14,7 @@
 
 #include "../../include/tlq.h"
 #include "tlq_fused.hpp"
+#include "tlq_gravity.hpp"
 #include "tlq_launch.hpp"
 #include "tlq_surrogate.hpp"
 
@@ -40,8 +41,9 @@ struct PendingTiming {
     hipEvent_t a, b;
 };
 
-const char *const kKernelNames[] = {"stage1", "stage2", "init", "cooling"};
-constexpr int kNumKernels = 4;
+// "multigrid" times a whole self-gravity solve (many launches).
+const char *const kKernelNames[] = {"stage1", "stage2", "init", "cooling", "multigrid"};
+constexpr int kNumKernels = 5;
 
 }  // namespace
 
@@ -81,6 +83,11 @@ struct tlq_ctx {
     unsigned *cool_fqueue = nullptr; // surrogate cells queued for the fallback
     unsigned *cool_seg = nullptr;    // per segment: cells queued, cells that fell back
     unsigned *cool_fall = nullptr;
+    // self-gravity: multigrid levels (level 0's E is PHI) and the step count
+    // MultiGrid's mod(it, 10) test reads
+    SgState sg;
+    bool sg_on = false;
+    int it_host = 0;
 };
 
 namespace {
@@ -202,6 +209,15 @@ template <class R> int do_borders(tlq_ctx *c, void *buf, int order) {
     return TLQ_OK;
 }
 
+// MultiGrid(dens, PHI) on U^n's density at the current step count
+// (mainHD.f90:50-52; HLLC.f90:16-21).
+template <class R> int do_sg_solve(tlq_ctx *c) {
+    if (!c->sg_on) return TLQ_OK;
+    KTimer kt(c, 4);
+    HIPCHK(sg_solve<R>(c->stream, c->g, cp<R>(c->u[c->cur]), c->sg, c->it_host, c->dprob));
+    return TLQ_OK;
+}
+
 template <class R> int do_init_maxima(tlq_ctx *c) {
     KTimer kt(c, 2);
     const int par = c->cur;  // parity of the step about to run
@@ -219,6 +235,9 @@ template <class R> int do_advance(tlq_ctx *c, int nsteps) {
     }
     for (int s = 0; s < nsteps; ++s) {
         const int par = c->cur;
+        // self-gravity: one solve per step from U^n, used by both stages
+        // (the reference solves twice from the same U^n, mainHD.f90:80,88)
+        if (int rc = do_sg_solve<R>(c)) return rc;
         {
             KTimer kt(c, 0);
             HIPCHK(Launch<R>::stage1(c->stream, c->dprob, c->tiling1, cp<R>(c->u[par]), mp<R>(c->uh),
@@ -237,6 +256,7 @@ template <class R> int do_advance(tlq_ctx *c, int nsteps) {
                                     c->cool_fall, c->host.cooling));
         }
         c->cur ^= 1;
+        ++c->it_host;
     }
     return TLQ_OK;
 }
@@ -330,8 +350,12 @@ template <class R> int do_godunov(tlq_ctx *c, int dst, double dt) {
         if (rc) return rc;
     }
     const R *S = nullptr;
-    if (c->host.nmasses > 0) {
-        int rc = ensure_buf(c, TLQ_BUF_S);
+    if (c->host.nmasses > 0 || c->sg_on) {
+        // with self-gravity every Godunov call solves the potential from
+        // U^n first (HLLC.f90:16-23)
+        int rc = do_sg_solve<R>(c);
+        if (rc) return rc;
+        rc = ensure_buf(c, TLQ_BUF_S);
         if (rc) return rc;
         if ((rc = ensure_buf(c, TLQ_BUF_UPRIM))) return rc;
         HIPCHK(Launch<R>::gravity(c->stream, c->dprob, cp<R>(c->bufs[TLQ_BUF_UPRIM]),
@@ -408,6 +432,24 @@ int tlq_create(const tlq_params *p, tlq_ctx **out) {
     if (!(p->gamma > 1.0) || !(p->cfl > 0.0) || !(p->xr > p->xl) || !(p->yr > p->yl) ||
         !(p->zr > p->zl) || !(p->mu0 > 0.0))
         return fail(TLQ_ERR_ARG, "bad physical parameters");
+    int sg_levels = 0;
+    if (p->self_gravity != TLQ_SG_OFF) {
+        if (p->self_gravity != TLQ_SG_FIXED && p->self_gravity != TLQ_SG_MULTIPOLE)
+            return fail(TLQ_ERR_ARG, "bad self_gravity id");
+        if (p->mg_levels < 0 || p->mg_levels > SG_MAX_LEVELS) return fail(TLQ_ERR_ARG, "bad mg_levels");
+        // poisson_solver and resid take the cell size from x alone (sources.f90:280, 322)
+        const double dx = (p->xr - p->xl) / p->nx, dy = (p->yr - p->yl) / p->ny,
+                     dz = (p->zr - p->zl) / p->nz;
+        if (std::fabs(dy - dx) > 1e-12 * dx || std::fabs(dz - dx) > 1e-12 * dx)
+            return fail(TLQ_ERR_UNSUPPORTED, "self-gravity needs cubic cells (dx = dy = dz)");
+        sg_levels = p->mg_levels > 0 ? p->mg_levels : sg_levels_auto(p->nx, p->ny, p->nz);
+        for (int l = 1; l <= sg_levels; ++l)
+            if (p->nx % (1 << l) || p->ny % (1 << l) || p->nz % (1 << l))
+                return fail(TLQ_ERR_ARG, "mg_levels: nx, ny, nz must be divisible by 2^mg_levels");
+        if (sg_coarse_cells(p->nx, p->ny, p->nz, sg_levels) > SG_COARSE_MAX)
+            return fail(TLQ_ERR_UNSUPPORTED, "coarsest multigrid level above 4096 cells");
+        if (!(p->gconst >= 0.0)) return fail(TLQ_ERR_ARG, "bad gconst");
+    }
 
     tlq_ctx *c = new tlq_ctx();
     c->prm = *p;
@@ -481,6 +523,14 @@ int tlq_create(const tlq_params *p, tlq_ctx **out) {
         hipMemsetAsync(c->cool_seg, 0, nb * sizeof(unsigned), c->stream);
         hipMemsetAsync(c->cool_fall, 0, nb * sizeof(unsigned), c->stream);
     }
+    if (p->self_gravity != TLQ_SG_OFF) {
+        c->sg.boundary = p->self_gravity;
+        c->sg.gconst = p->gconst;
+        if (sg_alloc(c->sg, p->nx, p->ny, p->nz, sg_levels, p->xr - p->xl, c->stream) != hipSuccess)
+            return bail(TLQ_ERR_NOMEM, "hipMalloc failed (multigrid levels)");
+        c->sg_on = true;
+        P.phi = c->sg.lv[0].e;
+    }
     StepState s0{};
     hipMemcpyAsync(c->st, &s0, sizeof s0, hipMemcpyHostToDevice, c->stream);
     hipEventCreate(&c->ev0);
@@ -527,6 +577,7 @@ int tlq_destroy(tlq_ctx *c) {
     if (c->cool_fqueue) hipFree(c->cool_fqueue);
     if (c->cool_seg) hipFree(c->cool_seg);
     if (c->cool_fall) hipFree(c->cool_fall);
+    sg_free(c->sg);
     for (void *q : {(void *)c->sur_theta, (void *)c->sur_m, (void *)c->sur_v, (void *)c->sur_gout,
                     (void *)c->sur_thq, (void *)c->sur_X, (void *)c->sur_Y,
                     (void *)c->sur_partial, (void *)c->sur_loss})
@@ -673,6 +724,7 @@ int tlq_step(tlq_ctx *c, double *t, int *it, double dt) {
     int rc = DISPATCH(c, do_step);
     if (rc) return rc;
     c->fused_ready = false;
+    ++c->it_host;
     if (t) *t = *t + dt;  // HydroCore.f90:208-209
     if (it) *it = *it + 1;
     return TLQ_OK;
@@ -692,6 +744,9 @@ int tlq_init_state(tlq_ctx *c) {
     // t = 0, it = 0 (mainHD.f90:22-27)
     StepState s0{};
     HIPCHK(hipMemcpyAsync(c->st, &s0, sizeof s0, hipMemcpyHostToDevice, c->stream));
+    c->it_host = 0;
+    rc = DISPATCH(c, do_sg_solve);                        // mainHD.f90:50-52
+    if (rc) return rc;
     return DISPATCH(c, do_init_maxima);                   // calc_prims + first time_step
 }
 
@@ -720,7 +775,49 @@ int tlq_resume(tlq_ctx *c, double t, int it, int fill_ghosts) {
     s0.it = it;
     HIPCHK(hipMemcpyAsync(c->st, &s0, sizeof s0, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->it_host = it;
+    if (int rc = DISPATCH(c, do_sg_solve)) return rc;     // from the PHI uploaded, if any
     return DISPATCH(c, do_init_maxima);
+}
+
+int tlq_upload_phi(tlq_ctx *c, const double *phi) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!c->sg_on) return fail(TLQ_ERR_STATE, "self-gravity is off");
+    if (!phi) return fail(TLQ_ERR_ARG, "null phi");
+    const SgLevel &L = c->sg.lv[0];
+    HIPCHK(hipMemcpyAsync(L.e, phi, sizeof(double) * (size_t)(L.nx + 2) * (L.ny + 2) * (L.nz + 2),
+                          hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return TLQ_OK;
+}
+
+int tlq_download_phi(tlq_ctx *c, double *phi) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!c->sg_on) return fail(TLQ_ERR_STATE, "self-gravity is off");
+    if (!phi) return fail(TLQ_ERR_ARG, "null phi");
+    const SgLevel &L = c->sg.lv[0];
+    HIPCHK(hipMemcpyAsync(phi, L.e, sizeof(double) * (size_t)(L.nx + 2) * (L.ny + 2) * (L.nz + 2),
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return TLQ_OK;
+}
+
+int tlq_gravity_solve(tlq_ctx *c, int *cycles, double *dif) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!c->sg_on) return fail(TLQ_ERR_STATE, "self-gravity is off");
+    if (int rc = DISPATCH(c, do_sg_solve)) return rc;
+    if (cycles) *cycles = c->sg.cycles;
+    if (dif) *dif = c->sg.dif;
+    return TLQ_OK;
+}
+
+int tlq_gravity_info(tlq_ctx *c, int *levels, int *cycles, double *dif, int *it) {
+    if (int rc = check_ctx(c)) return rc;
+    if (levels) *levels = c->sg_on ? c->sg.levels : -1;
+    if (cycles) *cycles = c->sg.cycles;
+    if (dif) *dif = c->sg.dif;
+    if (it) *it = c->it_host;
+    return TLQ_OK;
 }
 
 int tlq_get_time(tlq_ctx *c, double *dt, double *t, int *it) {

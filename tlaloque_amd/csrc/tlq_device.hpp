@@ -79,7 +79,14 @@ struct Problem {
     // with steps g_dY, g_dZ); acceleration factor g_A[m] = 2 G M / L^2.
     float g_A[MAX_MASSES], g_X0[MAX_MASSES], g_Y0[MAX_MASSES], g_Z0[MAX_MASSES];
     float g_dY, g_dZ, g_e2;
+    // Self-gravity potential PHI(0:nx+1, 0:ny+1, 0:nz+1) (i fastest), or null.
+    const double *phi;
 };
+
+// PHI at cell (i, j, k), 0 <= i <= nx+1 (likewise j, k).
+__device__ __forceinline__ double phi_at(const Problem &P, int i, int j, int k) {
+    return P.phi[(long long)i + (long long)(P.g.nx + 2) * ((long long)j + (long long)(P.g.ny + 2) * k)];
+}
 
 // ---------------------------------------------------------------------------
 // Arithmetic helpers.  FP64 keeps IEEE division and square root (the oracle
@@ -332,13 +339,20 @@ __device__ __forceinline__ double potential(const Problem &P, int i, int j, int 
     return phi;
 }
 
+// PHIT = PHIP + PHI at a cell (HLLC.f90:22): the point masses' potential plus
+// the self-gravity potential when there is one.
+__device__ __forceinline__ double phit(const Problem &P, int i, int j, int k) {
+    const double pp = potential(P, i, j, k);
+    return P.phi ? pp + phi_at(P, i, j, k) : pp;
+}
+
 // S_Solver (sources.f90:872-897) for one interior cell from its primitives.
 __device__ __forceinline__ void gravity_source(const Problem &P, int i, int j, int k,
                                                double rho, double u, double v, double w,
                                                double s[5]) {
-    const double g0 = (potential(P, i + 1, j, k) - potential(P, i - 1, j, k)) / (2.0 * P.dx);
-    const double g1 = (potential(P, i, j + 1, k) - potential(P, i, j - 1, k)) / (2.0 * P.dy);
-    const double g2 = (potential(P, i, j, k + 1) - potential(P, i, j, k - 1)) / (2.0 * P.dz);
+    const double g0 = (phit(P, i + 1, j, k) - phit(P, i - 1, j, k)) / (2.0 * P.dx);
+    const double g1 = (phit(P, i, j + 1, k) - phit(P, i, j - 1, k)) / (2.0 * P.dy);
+    const double g2 = (phit(P, i, j, k + 1) - phit(P, i, j, k - 1)) / (2.0 * P.dz);
     s[0] = 0.0;
     s[1] = -rho * g0;
     s[2] = -rho * g1;

@@ -178,6 +178,11 @@ __device__ __forceinline__ void gravity_src(const Problem &P, int i, int j, int 
     } else {
         float g[3];
         gravity_fast(P, i, j, k, g);
+        if (P.phi) {  // central differences of the self-gravity potential (FP64)
+            g[0] += (float)((phi_at(P, i + 1, j, k) - phi_at(P, i - 1, j, k)) / (2.0 * P.dx));
+            g[1] += (float)((phi_at(P, i, j + 1, k) - phi_at(P, i, j - 1, k)) / (2.0 * P.dy));
+            g[2] += (float)((phi_at(P, i, j, k + 1) - phi_at(P, i, j, k - 1)) / (2.0 * P.dz));
+        }
         const float rg0 = w[0] * g[0], rg1 = w[0] * g[1], rg2 = w[0] * g[2];
         s[0] = 0.0;
         s[1] = -rg0;
@@ -273,7 +278,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
     if (blockIdx.x == 0 && tid < 4) st->maxima[parity ^ 1][tid] = 0ull;
     const double dtdx = dth / P.dx, dtdy = dth / P.dy, dtdz = dth / P.dz;
     const R gam = (R)P.gamma, gr = (R)(P.gamma / (P.gamma - 1.0));
-    const bool grav = P.nmasses > 0;
+    const bool grav = P.nmasses > 0 || P.phi != nullptr;
     if (tid == 0)
         s_wmask = P.nwinds ? winds_mask(P, x0 - 2, x0 + TX + 1, y0 - 2, y0 + TY + 1, z0 - 2, z1 + 2)
                            : 0ull;
@@ -431,7 +436,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
     const double dtdx = dt / P.dx, dtdy = dt / P.dy, dtdz = dt / P.dz;
     const R gam = (R)P.gamma, gr = (R)(P.gamma / (P.gamma - 1.0));
     const int lim = P.limiter;
-    const bool grav = P.nmasses > 0;
+    const bool grav = P.nmasses > 0 || P.phi != nullptr;
     // Cooling modes 2 and 3 run in a pass of their own after this kernel,
     // which then also forms the CFL maxima.
     const bool cool = P.cooling == 1 && P.ntable > 1;

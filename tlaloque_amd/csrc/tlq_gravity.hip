@@ -1,0 +1,520 @@
+// Self-gravity kernels: the multigrid V-cycle of MultiGrid
+// (source/sources.f90:99-205) and the multipole boundary values of
+// Moments / phi_[xyz]boundaries (sources.f90:599-787), FP64.
+#include "tlq_gravity.hpp"
+
+#include <cmath>
+
+namespace tlq {
+
+namespace {
+
+constexpr int RB_TX = 64, RB_TY = 4;   // red-black relaxation block: i-pairs x rows
+constexpr int NT = 256;
+
+__device__ __forceinline__ long long sidx(int nx, int ny, int i, int j, int k) {
+    return (long long)i + (long long)(nx + 2) * ((long long)j + (long long)(ny + 2) * k);
+}
+
+// Block sum of one double per thread (fixed tree: deterministic).
+template <int THREADS>
+__device__ __forceinline__ double block_sum(double v, double *red) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+    if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (tid == 0)
+        for (int w = 0; w < THREADS / 64; ++w) s += red[w];
+    __syncthreads();
+    return s;  // valid in thread 0
+}
+
+// Level-0 source 4 pi G rho (Residue(0) = 4.0*PI*Gconst*dens, sources.f90:137).
+template <class R>
+__device__ __forceinline__ double src0(const R *U, const Grid &g, double fpg, int i, int j, int k) {
+    return fpg * (double)U[g.idx(i, j, k)];
+}
+
+__global__ void k_sg_fill(double *__restrict__ a, long long n, double v) {
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n;
+         q += (long long)gridDim.x * blockDim.x)
+        a[q] = v;
+}
+
+// Ghost cells of a correction level before a relaxation (sources.f90:337-344,
+// serial branch).  The six whole-plane copies x-, y-, z-, x+, y+, z+ (each
+// ghost plane = minus the first interior plane) leave every ghost cell at
+// (-1)^(number of axes it is a ghost in) times the interior cell reached by
+// moving each ghost index one step inwards: each copy negates and moves one
+// axis inwards, and a cell's latest copy reads a cell whose other ghost axes
+// were copied earlier.  Threads cover the x faces (all j, k), the y faces
+// (interior i) and the z faces (interior i, j): every ghost cell once.
+__global__ void k_sg_bc(double *__restrict__ e, int nx, int ny, int nz) {
+    const long long nxf = 2ll * (ny + 2) * (nz + 2), nyf = 2ll * nx * (nz + 2), nzf = 2ll * nx * ny;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (t < nxf) {
+        const long long q = t >> 1;
+        i = (t & 1) ? nx + 1 : 0;
+        j = (int)(q % (ny + 2));
+        k = (int)(q / (ny + 2));
+    } else if (t < nxf + nyf) {
+        const long long u = t - nxf, q = u >> 1;
+        j = (u & 1) ? ny + 1 : 0;
+        i = 1 + (int)(q % nx);
+        k = (int)(q / nx);
+    } else if (t < nxf + nyf + nzf) {
+        const long long u = t - nxf - nyf, q = u >> 1;
+        k = (u & 1) ? nz + 1 : 0;
+        i = 1 + (int)(q % nx);
+        j = 1 + (int)(q / nx);
+    } else {
+        return;
+    }
+    int ci = i, cj = j, ck = k, ng = 0;
+    if (ci == 0) { ci = 1; ++ng; } else if (ci == nx + 1) { ci = nx; ++ng; }
+    if (cj == 0) { cj = 1; ++ng; } else if (cj == ny + 1) { cj = ny; ++ng; }
+    if (ck == 0) { ck = 1; ++ng; } else if (ck == nz + 1) { ck = nz; ++ng; }
+    const double v = e[sidx(nx, ny, ci, cj, ck)];
+    e[sidx(nx, ny, i, j, k)] = (ng & 1) ? -v : v;
+}
+
+// One red-black half sweep: cells with (i + j + k) % 2 == color take the
+// Gauss-Seidel update of poisson_solver (sources.f90:362-365, w = 1):
+// E = (1/6)(E(i+1) + E(i-1) + E(j+1) + E(j-1) + E(k+1) + E(k-1) - s dxl^2).
+// Thread (t, j, k) owns the pair i = 2t+1, 2t+2 and updates the one of the
+// colour.  With `partial`, each block writes its sum of |change|.
+template <class R, bool FROM_U>
+__global__ __launch_bounds__(RB_TX *RB_TY) void k_sg_relax(
+    double *__restrict__ e, const double *__restrict__ rs, const R *__restrict__ U, Grid g,
+    double fpg, int nx, int ny, int nz, double dxl, int color, double *__restrict__ partial) {
+    __shared__ double red[RB_TX * RB_TY / 64];
+    const int t = blockIdx.x * RB_TX + threadIdx.x;
+    const int j = blockIdx.y * RB_TY + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    const int i = 2 * t + 1 + ((color + 1 + j + k) & 1);
+    double d = 0.0;
+    if (i <= nx && j <= ny) {
+        const long long sy = nx + 2, sz = (long long)(nx + 2) * (ny + 2);
+        const long long c = sidx(nx, ny, i, j, k);
+        double s;
+        if constexpr (FROM_U) s = src0<R>(U, g, fpg, i, j, k);
+        else s = rs[c];
+        const double old = e[c];
+        const double w = 1.0;
+        const double nv = ((1.0 / 6.0) * (e[c + 1] + e[c - 1] + e[c + sy] + e[c - sy] + e[c + sz] +
+                                          e[c - sz] - s * (dxl * dxl))) * w + (1.0 - w) * old;
+        e[c] = nv;
+        d = fabs(nv - old);
+    }
+    if (partial) {
+        const double b = block_sum<RB_TX * RB_TY>(d, red);
+        if (threadIdx.x == 0 && threadIdx.y == 0)
+            partial[blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z)] = b;
+    }
+}
+
+// resid + restriction (sources.f90:270-300, 207-228) fused: each coarse cell
+// averages the residual source - Laplacian(E) of its 8 children, in the
+// reference's order of the 8 terms.
+template <class R, bool FROM_U>
+__global__ void k_sg_restrict(const double *__restrict__ ef, const double *__restrict__ rf,
+                              const R *__restrict__ U, Grid g, double fpg, int nxf, int nyf,
+                              double dxl, double *__restrict__ rc, int nxc, int nyc, int nzc) {
+    const long long n = (long long)nxc * nyc * nzc;
+    const long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int ic = 1 + (int)(q % nxc), jc = 1 + (int)((q / nxc) % nyc), kc = 1 + (int)(q / ((long long)nxc * nyc));
+    const int a = 2 * ic - 1, b = 2 * jc - 1, cc = 2 * kc - 1;
+    const long long sy = nxf + 2, sz = (long long)(nxf + 2) * (nyf + 2);
+    auto res = [&](int i, int j, int k) -> double {
+        const long long c = sidx(nxf, nyf, i, j, k);
+        double s;
+        if constexpr (FROM_U) s = src0<R>(U, g, fpg, i, j, k);
+        else s = rf[c];
+        const double gr = (ef[c - 1] + ef[c + 1] + ef[c - sy] + ef[c + sy] + ef[c - sz] + ef[c + sz] -
+                           6.0 * ef[c]) /
+                          (dxl * dxl);
+        return s - gr;
+    };
+    const double sum = res(a, b, cc) + res(a + 1, b, cc) + res(a, b + 1, cc) + res(a, b, cc + 1) +
+                       res(a + 1, b + 1, cc) + res(a + 1, b, cc + 1) + res(a, b + 1, cc + 1) +
+                       res(a + 1, b + 1, cc + 1);
+    rc[sidx(nxc, nyc, ic, jc, kc)] = (1.0 / 8.0) * sum;
+}
+
+// prolongation + E = E + ret (sources.f90:230-268, 189-195) fused: the
+// 27/9/3/1 weights of the coarse cell and its neighbours (ghosts included)
+// added onto the 8 children.
+__global__ void k_sg_prolong(const double *__restrict__ ec, int nxc, int nyc, int nzc,
+                             double *__restrict__ ef, int nxf, int nyf) {
+    const long long n = (long long)nxc * nyc * nzc;
+    const long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int i = 1 + (int)(q % nxc), j = 1 + (int)((q / nxc) % nyc), k = 1 + (int)(q / ((long long)nxc * nyc));
+    auto V = [&](int ii, int jj, int kk) { return ec[sidx(nxc, nyc, ii, jj, kk)]; };
+    const int a = 2 * i - 1, b = 2 * j - 1, c = 2 * k - 1;
+#pragma unroll
+    for (int dc = 0; dc < 2; ++dc)
+#pragma unroll
+        for (int db = 0; db < 2; ++db)
+#pragma unroll
+            for (int da = 0; da < 2; ++da) {
+                const int si = da ? 1 : -1, sj = db ? 1 : -1, sk = dc ? 1 : -1;
+                const double s = 27.0 * V(i, j, k) + 9.0 * V(i + si, j, k) + 9.0 * V(i, j + sj, k) +
+                                 9.0 * V(i, j, k + sk) + 3.0 * V(i + si, j + sj, k) +
+                                 3.0 * V(i + si, j, k + sk) + 3.0 * V(i, j + sj, k + sk) +
+                                 V(i + si, j + sj, k + sk);
+                const long long f = sidx(nxf, nyf, a + da, b + db, c + dc);
+                ef[f] = ef[f] + (1.0 / 64.0) * s;
+            }
+}
+
+// The coarsest level's solve (poisson_solver, Solve, sources.f90:311-377):
+// Gauss-Seidel to Dif <= 0.001, here red-black in one workgroup, capped.
+__global__ __launch_bounds__(NT) void k_sg_coarse(double *__restrict__ e,
+                                                  const double *__restrict__ r, int nx, int ny,
+                                                  int nz, double dxl, int cap) {
+    __shared__ double red[NT / 64];
+    __shared__ double s_dif;
+    const int n = nx * ny * nz;
+    const long long sy = nx + 2, sz = (long long)(nx + 2) * (ny + 2);
+    for (int iter = 0; iter < cap; ++iter) {
+        double d = 0.0;
+        for (int color = 0; color < 2; ++color) {
+            for (int q = threadIdx.x; q < n; q += NT) {
+                const int i = 1 + q % nx, j = 1 + (q / nx) % ny, k = 1 + q / (nx * ny);
+                if (((i + j + k) & 1) != color) continue;
+                const long long c = sidx(nx, ny, i, j, k);
+                const double old = e[c];
+                const double nv = ((1.0 / 6.0) * (e[c + 1] + e[c - 1] + e[c + sy] + e[c - sy] +
+                                                  e[c + sz] + e[c - sz] - r[c] * (dxl * dxl))) * 1.0 +
+                                  (1.0 - 1.0) * old;
+                e[c] = nv;
+                d += fabs(nv - old);
+            }
+            __syncthreads();
+        }
+        const double b = block_sum<NT>(d, red);
+        if (threadIdx.x == 0) s_dif = b;
+        __syncthreads();
+        const double dif = s_dif;
+        __syncthreads();
+        if (!(dif > 0.001)) break;
+    }
+}
+
+// Sum of n partials in a fixed order (one workgroup) -> out[0].
+__global__ __launch_bounds__(NT) void k_sg_sum(const double *__restrict__ partial, long long n,
+                                               int stride, int count, double *__restrict__ out) {
+    __shared__ double red[NT / 64];
+    for (int q = 0; q < count; ++q) {
+        double s = 0.0;
+        for (long long p = threadIdx.x; p < n; p += NT) s += partial[p * stride + q];
+        const double b = block_sum<NT>(s, red);
+        if (threadIdx.x == 0) out[q] = b;
+    }
+}
+
+// Legendre / aLegendre (sources.f90:9-97).
+__device__ __forceinline__ double legendre(double x, int l) {
+    switch (l) {
+    case 0: return 1.0;
+    case 1: return x;
+    case 2: return 0.5 * (3.0 * (x * x) - 1.0);
+    case 3: return 0.5 * (5.0 * (x * x * x) - 3.0 * x);
+    default: return (1.0 / 8.0) * (35.0 * (x * x * x * x) - 30.0 * (x * x) + 3.0);
+    }
+}
+__device__ __forceinline__ double alegendre(double x, int l, int m) {
+    const double q = 1.0 - x * x, s = sqrt(q);
+    switch (l * 8 + m) {
+    case 9: return -s;
+    case 17: return -3.0 * x * s;
+    case 18: return 3.0 * q;
+    case 25: return (3.0 / 2.0) * (1.0 - 5.0 * (x * x)) * s;
+    case 26: return 15.0 * x * q;
+    case 27: return -15.0 * (q * s);
+    case 33: return -(5.0 / 2.0) * (7.0 * (x * x * x) - 3.0 * x) * s;
+    case 34: return (15.0 / 2.0) * (7.0 * (x * x) - 1.0) * q;
+    case 35: return -105.0 * x * (q * s);
+    case 36: return 105.0 * (q * q);
+    default: return legendre(x, l);
+    }
+}
+
+// Spherical coordinates of a cell centre (pangle, aangle, distance,
+// sources.f90:556-597).
+__device__ __forceinline__ void sph(const Problem &P, int i, int j, int k, double &r, double &ct,
+                                    double &fi) {
+    const double x = xcoord(P, i), y = ycoord(P, j), z = zcoord(P, k);
+    r = sqrt(x * x + y * y + z * z);
+    ct = cos(acos(z / r));
+    fi = atan2(y, x);
+    if (fi < 0) fi = fi + 2. * PI_REF;
+}
+
+// Moment slot of (l, m): 0..4 c0(l); 5 + idx cm(l,m); 15 + idx sm(l,m),
+// idx = l(l-1)/2 + m - 1.
+__device__ __forceinline__ int mslot(int l, int m) { return l * (l - 1) / 2 + m - 1; }
+
+// Moments (sources.f90:599-676) with the sums the MPI build reduces: per
+// block, the 25 partial sums over its cells (dV = dx dy dz).
+template <class R>
+__global__ __launch_bounds__(NT) void k_sg_moments(const Problem *__restrict__ Pp, const R *__restrict__ U,
+                                                   double *__restrict__ partial) {
+    __shared__ double red[NT / 64];
+    const Problem &P = *Pp;
+    const Grid &g = P.g;
+    const long long n = (long long)g.nx * g.ny * g.nz;
+    double acc[SG_NMOM];
+#pragma unroll
+    for (int q = 0; q < SG_NMOM; ++q) acc[q] = 0.0;
+    const double dv = P.dx * P.dy * P.dz;
+    for (long long q = blockIdx.x * (long long)NT + threadIdx.x; q < n; q += (long long)gridDim.x * NT) {
+        const int i = 1 + (int)(q % g.nx), j = 1 + (int)((q / g.nx) % g.ny),
+                  k = 1 + (int)(q / ((long long)g.nx * g.ny));
+        double r, ct, fi;
+        sph(P, i, j, k, r, ct, fi);
+        const double d = (double)U[g.idx(i, j, k)] * dv;
+        acc[0] += d;
+        double rl = 1.0;
+#pragma unroll
+        for (int l = 1; l <= 4; ++l) {
+            rl *= r;
+            acc[l] += legendre(ct, l) * rl * d;
+#pragma unroll
+            for (int m = 1; m <= l; ++m) {
+                const double a = alegendre(ct, l, m) * rl * d;
+                acc[5 + mslot(l, m)] += a * cos(m * fi);
+                acc[15 + mslot(l, m)] += a * sin(m * fi);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < SG_NMOM; ++q) {
+        const double b = block_sum<NT>(acc[q], red);
+        if (threadIdx.x == 0) partial[(long long)blockIdx.x * SG_NMOM + q] = b;
+    }
+}
+
+// phi_[xyz]boundaries (sources.f90:678-787) on the six faces (face cells
+// only): the m > 0 terms carry the factor 2 written there on top of the
+// 2 (l-m)!/(l+m)! of the moments.
+__global__ void k_sg_faces(const Problem *__restrict__ Pp, const double *__restrict__ mom,
+                           double gconst, double *__restrict__ phi) {
+    const Problem &P = *Pp;
+    const int nx = P.g.nx, ny = P.g.ny, nz = P.g.nz;
+    const long long nxf = 2ll * ny * nz, nyf = 2ll * nx * nz, nzf = 2ll * nx * ny;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (t < nxf) {
+        const long long q = t >> 1;
+        i = (t & 1) ? nx + 1 : 0;
+        j = 1 + (int)(q % ny);
+        k = 1 + (int)(q / ny);
+    } else if (t < nxf + nyf) {
+        const long long u = t - nxf, q = u >> 1;
+        j = (u & 1) ? ny + 1 : 0;
+        i = 1 + (int)(q % nx);
+        k = 1 + (int)(q / nx);
+    } else if (t < nxf + nyf + nzf) {
+        const long long u = t - nxf - nyf, q = u >> 1;
+        k = (u & 1) ? nz + 1 : 0;
+        i = 1 + (int)(q % nx);
+        j = 1 + (int)(q / nx);
+    } else {
+        return;
+    }
+    double rb, ct, fi;
+    sph(P, i, j, k, rb, ct, fi);
+    double phi_ = 0.0;
+    phi_ = phi_ - mom[0] / rb;
+    double rl1 = rb;
+    for (int l = 1; l <= 4; ++l) {
+        rl1 *= rb;
+        phi_ = phi_ - mom[l] * legendre(ct, l) / rl1;
+        for (int m = 1; m <= l; ++m)
+            phi_ = phi_ - 2.0 * (mom[5 + mslot(l, m)] * cos(m * fi) + mom[15 + mslot(l, m)] * sin(m * fi)) *
+                              alegendre(ct, l, m) / rl1;
+    }
+    phi[sidx(nx, ny, i, j, k)] = gconst * phi_;
+}
+
+// cm, sm = c * sums, c = 2 Gamma(l-m+1) / Gamma(l+m+1) (sources.f90:625, 647-648).
+__global__ void k_sg_moment_factors(double *__restrict__ mom) {
+    if (threadIdx.x != 0) return;
+    for (int l = 1; l <= 4; ++l)
+        for (int m = 1; m <= l; ++m) {
+            double num = 1.0, den = 1.0;
+            for (int q = 2; q <= l - m; ++q) num *= q;
+            for (int q = 2; q <= l + m; ++q) den *= q;
+            const double c = 2.0 * num / den;
+            mom[5 + mslot(l, m)] = c * mom[5 + mslot(l, m)];
+            mom[15 + mslot(l, m)] = c * mom[15 + mslot(l, m)];
+        }
+}
+
+inline unsigned nblocks(long long n, int t) { return (unsigned)((n + t - 1) / t); }
+
+dim3 relax_grid(const SgLevel &L) {
+    return dim3(nblocks((L.nx + 1) / 2, RB_TX), nblocks(L.ny, RB_TY), (unsigned)L.nz);
+}
+
+long long relax_blocks(const SgLevel &L) {
+    const dim3 gd = relax_grid(L);
+    return (long long)gd.x * gd.y * gd.z;
+}
+
+constexpr int MOM_BLOCKS = 1024;
+
+}  // namespace
+
+int sg_levels_auto(int nx, int ny, int nz) {
+    int L = 0;
+    while (L < SG_MAX_LEVELS && nx % (2 << L) == 0 && ny % (2 << L) == 0 && nz % (2 << L) == 0) ++L;
+    return L;
+}
+
+long long sg_coarse_cells(int nx, int ny, int nz, int levels) {
+    return (long long)(nx >> levels) * (ny >> levels) * (nz >> levels);
+}
+
+hipError_t sg_alloc(SgState &S, int nx, int ny, int nz, int levels, double xspan, hipStream_t stream) {
+    S.levels = levels;
+    for (int l = 0; l <= levels; ++l) {
+        SgLevel &L = S.lv[l];
+        L.nx = nx >> l;
+        L.ny = ny >> l;
+        L.nz = nz >> l;
+        L.dxl = xspan / (double)L.nx;
+        const size_t n = (size_t)(L.nx + 2) * (L.ny + 2) * (L.nz + 2);
+        hipError_t e = hipMalloc(&L.e, n * sizeof(double));
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_sg_fill, dim3(std::min<long long>(nblocks(n, NT), 65535)), dim3(NT), 0,
+                           stream, L.e, (long long)n, 0.0);
+        if (l > 0) {
+            e = hipMalloc(&L.r, n * sizeof(double));
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(k_sg_fill, dim3(std::min<long long>(nblocks(n, NT), 65535)), dim3(NT),
+                               0, stream, L.r, (long long)n, 0.0);
+        }
+    }
+    S.npartial = std::max<long long>(2 * relax_blocks(S.lv[0]), (long long)MOM_BLOCKS * SG_NMOM);
+    hipError_t e = hipMalloc(&S.partial, (size_t)S.npartial * sizeof(double));
+    if (e != hipSuccess) return e;
+    e = hipMalloc(&S.scal, (1 + SG_NMOM) * sizeof(double));
+    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+void sg_free(SgState &S) {
+    for (int l = 0; l <= S.levels; ++l) {
+        if (S.lv[l].e) (void)hipFree(S.lv[l].e);
+        if (S.lv[l].r) (void)hipFree(S.lv[l].r);
+        S.lv[l].e = S.lv[l].r = nullptr;
+    }
+    if (S.partial) (void)hipFree(S.partial);
+    if (S.scal) (void)hipFree(S.scal);
+    S.partial = S.scal = nullptr;
+}
+
+namespace {
+
+// poisson_solver's Relax (sources.f90:378-476): it-1 = 5 iterations of two
+// half sweeps.  With want_dif, the last iteration's sum of |change| goes to
+// S.scal[0].
+template <class R>
+void relax(hipStream_t s, const Grid &g, const R *U, double fpg, SgState &S, int l, bool want_dif) {
+    SgLevel &L = S.lv[l];
+    const dim3 gd = relax_grid(L), bd(RB_TX, RB_TY);
+    const long long nb = (long long)gd.x * gd.y * gd.z;
+    for (int it = 1; it <= 5; ++it)
+        for (int color = 0; color < 2; ++color) {
+            double *part = (want_dif && it == 5) ? S.partial + color * nb : nullptr;
+            if (l == 0)
+                hipLaunchKernelGGL((k_sg_relax<R, true>), gd, bd, 0, s, L.e, nullptr, U, g, fpg, L.nx,
+                                   L.ny, L.nz, L.dxl, color, part);
+            else
+                hipLaunchKernelGGL((k_sg_relax<R, false>), gd, bd, 0, s, L.e, L.r, U, g, fpg, L.nx,
+                                   L.ny, L.nz, L.dxl, color, part);
+        }
+    if (want_dif)
+        hipLaunchKernelGGL(k_sg_sum, dim3(1), dim3(NT), 0, s, S.partial, 2 * nb, 1, 1, S.scal);
+}
+
+void bc(hipStream_t s, SgLevel &L) {
+    const long long n = 2ll * (L.ny + 2) * (L.nz + 2) + 2ll * L.nx * (L.nz + 2) + 2ll * L.nx * L.ny;
+    hipLaunchKernelGGL(k_sg_bc, dim3(nblocks(n, NT)), dim3(NT), 0, s, L.e, L.nx, L.ny, L.nz);
+}
+
+}  // namespace
+
+template <class R>
+hipError_t sg_solve(hipStream_t s, const Grid &g, const R *U, SgState &S, int it,
+                    const Problem *dP) {
+    const double fpg = 4.0 * PI_REF * S.gconst;
+    const int LV = S.levels;
+    SgLevel &L0 = S.lv[0];
+    if (S.boundary == 2 && it % 10 == 0) {   // sources.f90:114-136
+        hipLaunchKernelGGL((k_sg_moments<R>), dim3(MOM_BLOCKS), dim3(NT), 0, s, dP, U, S.partial);
+        hipLaunchKernelGGL(k_sg_sum, dim3(1), dim3(NT), 0, s, S.partial, (long long)MOM_BLOCKS,
+                           SG_NMOM, SG_NMOM, S.scal + 1);
+        hipLaunchKernelGGL(k_sg_moment_factors, dim3(1), dim3(64), 0, s, S.scal + 1);
+        const long long nf = 2ll * (L0.ny * (long long)L0.nz + L0.nx * (long long)L0.nz +
+                                    L0.nx * (long long)L0.ny);
+        hipLaunchKernelGGL(k_sg_faces, dim3(nblocks(nf, NT)), dim3(NT), 0, s, dP, S.scal + 1,
+                           S.gconst, L0.e);
+    }
+    double dif = 1.e28;
+    int iter = 0;
+    while (dif > 10.0 && iter < 31) {   // sources.f90:140
+        ++iter;
+        for (int l = 1; l <= LV; ++l) {
+            const long long n = (long long)(S.lv[l].nx + 2) * (S.lv[l].ny + 2) * (S.lv[l].nz + 2);
+            hipLaunchKernelGGL(k_sg_fill, dim3(std::min<long long>(nblocks(n, NT), 65535)), dim3(NT), 0,
+                               s, S.lv[l].e, n, 0.0);
+        }
+        relax<R>(s, g, U, fpg, S, 0, LV == 0);
+        for (int l = 1; l <= LV; ++l) {
+            SgLevel &F = S.lv[l - 1], &C = S.lv[l];
+            const long long nc = (long long)C.nx * C.ny * C.nz;
+            if (l == 1)
+                hipLaunchKernelGGL((k_sg_restrict<R, true>), dim3(nblocks(nc, NT)), dim3(NT), 0, s, F.e,
+                                   nullptr, U, g, fpg, F.nx, F.ny, F.dxl, C.r, C.nx, C.ny, C.nz);
+            else
+                hipLaunchKernelGGL((k_sg_restrict<R, false>), dim3(nblocks(nc, NT)), dim3(NT), 0, s,
+                                   F.e, F.r, U, g, fpg, F.nx, F.ny, F.dxl, C.r, C.nx, C.ny, C.nz);
+            bc(s, C);
+            if (l == LV)
+                hipLaunchKernelGGL(k_sg_coarse, dim3(1), dim3(NT), 0, s, C.e, C.r, C.nx, C.ny, C.nz,
+                                   C.dxl, SG_COARSE_CAP);
+            else
+                relax<R>(s, g, U, fpg, S, l, false);
+        }
+        for (int l = LV - 1; l >= 0; --l) {
+            SgLevel &F = S.lv[l], &C = S.lv[l + 1];
+            const long long nc = (long long)C.nx * C.ny * C.nz;
+            hipLaunchKernelGGL(k_sg_prolong, dim3(nblocks(nc, NT)), dim3(NT), 0, s, C.e, C.nx, C.ny,
+                               C.nz, F.e, F.nx, F.ny);
+            if (l > 0) bc(s, F);
+            relax<R>(s, g, U, fpg, S, l, l == 0);
+        }
+        hipError_t e = hipMemcpyAsync(&dif, S.scal, sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) return e;
+        e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
+    }
+    S.cycles = iter;
+    S.dif = dif;
+    return hipGetLastError();
+}
+
+template hipError_t sg_solve<float>(hipStream_t, const Grid &, const float *, SgState &, int,
+                                    const Problem *);
+template hipError_t sg_solve<double>(hipStream_t, const Grid &, const double *, SgState &, int,
+                                     const Problem *);
+
+}  // namespace tlq

@@ -30,6 +30,7 @@ module tlq_c
   integer(c_int), parameter :: TLQ_FP64 = 0, TLQ_FP32 = 1
   integer(c_int), parameter :: TLQ_COOL_OFF = 0, TLQ_COOL_EXACT = 1, TLQ_COOL_SURROGATE = 2
   integer(c_int), parameter :: TLQ_COOL_SUBCYCLED = 3
+  integer(c_int), parameter :: TLQ_SG_OFF = 0, TLQ_SG_FIXED = 1, TLQ_SG_MULTIPOLE = 2
 
   integer(c_int), parameter :: TLQ_BUF_U = 0, TLQ_BUF_UP = 1, TLQ_BUF_UPP = 2, TLQ_BUF_UPRIM = 3
   integer(c_int), parameter :: TLQ_BUF_F = 4, TLQ_BUF_G = 5, TLQ_BUF_H = 6, TLQ_BUF_S = 7
@@ -45,6 +46,9 @@ module tlq_c
      integer(c_int) :: bc(6)          ! LB, RB, DB, TB, FB, BB
      integer(c_int) :: solver, precision, cooling
      real(c_double) :: softening
+     integer(c_int) :: self_gravity   ! TLQ_SG_*
+     integer(c_int) :: mg_levels      ! lvlm; 0: as many as the mesh halves
+     real(c_double) :: gconst         ! Gconst
   end type tlq_params_t
 
   ! struct tlq_wind: spherical_wind_type, source/winds.f90:5-13
@@ -103,6 +107,33 @@ module tlq_c
        integer(c_int), value :: buf
        real(c_double), intent(out) :: host(*)
      end function tlq_download
+
+     integer(c_int) function tlq_upload_phi(ctx, phi) bind(C, name="tlq_upload_phi")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: phi(*)
+     end function tlq_upload_phi
+
+     integer(c_int) function tlq_download_phi(ctx, phi) bind(C, name="tlq_download_phi")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: phi(*)
+     end function tlq_download_phi
+
+     integer(c_int) function tlq_gravity_solve(ctx, cycles, dif) bind(C, name="tlq_gravity_solve")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), intent(out) :: cycles
+       real(c_double), intent(out) :: dif
+     end function tlq_gravity_solve
+
+     integer(c_int) function tlq_gravity_info(ctx, levels, cycles, dif, it) &
+          bind(C, name="tlq_gravity_info")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), intent(out) :: levels, cycles, it
+       real(c_double), intent(out) :: dif
+     end function tlq_gravity_info
 
      integer(c_int) function tlq_calc_prims(ctx, src) bind(C, name="tlq_calc_prims")
        import :: c_int, c_ptr

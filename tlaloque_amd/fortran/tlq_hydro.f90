@@ -9,7 +9,9 @@
 ! Settings come from a namelist file given as the first argument:
 !   &mesh     nx, ny, nz, xl, xr, yl, yr, zl, zr
 !   &physics  gamma, mu0, cfl, eta, cooling_limit, limiter, solver, precision,
-!             cooling, bc(6), softening, tfin, cooling_file
+!             cooling, bc(6), softening, tfin, cooling_file, self_gravity,
+!             mg_levels, gconst (self-gravity: PHI starts at 0, so its ghost
+!             faces are 0 with TLQ_SG_FIXED; TLQ_SG_MULTIPOLE sets them)
 !   &run      problem, num_out, outfile, outputpath, max_steps, per_routine,
 !             initfile, dry_run, nwinds, winds(:), nmasses, masses(4,:)
 ! problem: 'sod'    diagonal Sod tube of the reference's test harness
@@ -56,6 +58,8 @@ module driver_setup
   real(8) :: softening = 0d0, tfin = 1d0
   integer :: limiter = TLQ_LIM_VANLEER, solver = TLQ_SOLVER_HLLC, precision = TLQ_FP64
   integer :: cooling = TLQ_COOL_OFF
+  integer :: self_gravity = TLQ_SG_OFF, mg_levels = 0
+  real(8) :: gconst = 6.67430d-8
   integer :: bc(6) = TLQ_BC_REFLECTIVE
   character(len=512) :: cooling_file = ''
   ! &run
@@ -72,7 +76,7 @@ module driver_setup
 
   namelist /mesh/ nx, ny, nz, xl, xr, yl, yr, zl, zr
   namelist /physics/ gamma, mu0, cfl, eta, cooling_limit, limiter, solver, precision, cooling, &
-       bc, softening, tfin, cooling_file
+       bc, softening, tfin, cooling_file, self_gravity, mg_levels, gconst
   namelist /run/ problem, num_out, outfile, outputpath, max_steps, per_routine, initfile, &
        dry_run, nwinds, winds, nmasses, masses, restart_file, restart_t, restart_it, restart_nout
 
@@ -133,6 +137,9 @@ contains
     p%precision = precision
     p%cooling = cooling
     p%softening = softening
+    p%self_gravity = self_gravity
+    p%mg_levels = mg_levels
+    p%gconst = gconst
   end function make_params
 
   ! cell centres, as xcoord/ycoord/zcoord (source/HydroCore.f90:35-75, one rank)

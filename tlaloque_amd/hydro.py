@@ -43,7 +43,8 @@ EXPORTS = (
     "tlq_set_cooling_table", "tlq_upload", "tlq_download", "tlq_calc_prims", "tlq_time_step",
     "tlq_fluxes", "tlq_godunov", "tlq_borders", "tlq_step", "tlq_cooling", "tlq_init_state",
     "tlq_advance", "tlq_advance_n", "tlq_get_time", "tlq_sync", "tlq_check_state",
-    "tlq_fallback_fraction", "tlq_cooling_counts", "tlq_resume", "tlq_timer_start", "tlq_timer_stop", "tlq_kernel_timing",
+    "tlq_fallback_fraction", "tlq_cooling_counts", "tlq_resume", "tlq_upload_phi",
+    "tlq_download_phi", "tlq_gravity_solve", "tlq_gravity_info", "tlq_timer_start", "tlq_timer_stop", "tlq_kernel_timing",
     "tlq_kernel_stats", "tlq_layout", "tlq_copy_probe", "tlq_surrogate_init",
     "tlq_surrogate_set_params", "tlq_surrogate_get_params", "tlq_surrogate_generate",
     "tlq_surrogate_samples", "tlq_surrogate_train", "tlq_surrogate_eval", "tlq_surrogate_grad",
@@ -66,6 +67,7 @@ class TlqParams(ctypes.Structure):
         ("eta", ctypes.c_double), ("cooling_limit", ctypes.c_double),
         ("limiter", ctypes.c_int), ("bc", ctypes.c_int * 6), ("solver", ctypes.c_int),
         ("precision", ctypes.c_int), ("cooling", ctypes.c_int), ("softening", ctypes.c_double),
+        ("self_gravity", ctypes.c_int), ("mg_levels", ctypes.c_int), ("gconst", ctypes.c_double),
     ]
 
 
@@ -112,6 +114,10 @@ def load_library(path: Optional[str] = None):
         "tlq_fallback_fraction": [vp, dp],
         "tlq_cooling_counts": [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
         "tlq_resume": [vp, ctypes.c_double, ip, ip],
+        "tlq_upload_phi": [vp, dp],
+        "tlq_download_phi": [vp, dp],
+        "tlq_gravity_solve": [vp, ctypes.POINTER(ip), dp],
+        "tlq_gravity_info": [vp, ctypes.POINTER(ip), ctypes.POINTER(ip), dp, ctypes.POINTER(ip)],
         "tlq_timer_start": [vp],
         "tlq_timer_stop": [vp, dp],
         "tlq_kernel_timing": [vp, ip],
@@ -155,10 +161,11 @@ def _fptr(a: np.ndarray):
 
 def to_c_params(p: P.HydroParams) -> TlqParams:
     c = TlqParams()
-    for n in ("nx", "ny", "nz", "limiter", "solver", "precision", "cooling"):
+    for n in ("nx", "ny", "nz", "limiter", "solver", "precision", "cooling", "self_gravity",
+              "mg_levels"):
         setattr(c, n, int(getattr(p, n)))
     for n in ("xl", "xr", "yl", "yr", "zl", "zr", "gamma", "mu0", "cfl", "eta", "cooling_limit",
-              "softening"):
+              "softening", "gconst"):
         setattr(c, n, float(getattr(p, n)))
     for q in range(6):
         c.bc[q] = int(p.bc[q])
@@ -268,6 +275,37 @@ class HydroCore:
         self.upload("U", u)
         _check(_lib.tlq_resume(self._ctx, float(t), int(it), int(fill_ghosts)))
 
+    # self-gravity --------------------------------------------------------------
+    @property
+    def phi_shape(self) -> Tuple[int, int, int]:
+        return (self.p.nz + 2, self.p.ny + 2, self.p.nx + 2)
+
+    def upload_phi(self, phi: np.ndarray) -> None:
+        """PHI(0:nx+1, 0:ny+1, 0:nz+1), C shape (nz+2, ny+2, nx+2); its ghost
+        faces are the boundary values."""
+        a = np.ascontiguousarray(phi, dtype=np.float64)
+        if a.shape != self.phi_shape:
+            raise ValueError(f"expected shape {self.phi_shape}, got {a.shape}")
+        _check(_lib.tlq_upload_phi(self._ctx, _dptr(a)))
+
+    def download_phi(self) -> np.ndarray:
+        out = np.empty(self.phi_shape, dtype=np.float64)
+        _check(_lib.tlq_download_phi(self._ctx, _dptr(out)))
+        return out
+
+    def gravity_solve(self) -> Tuple[int, float]:
+        """One MultiGrid call on the current U: (V-cycles, final Dif)."""
+        cyc, dif = ctypes.c_int(), ctypes.c_double()
+        _check(_lib.tlq_gravity_solve(self._ctx, ctypes.byref(cyc), ctypes.byref(dif)))
+        return cyc.value, dif.value
+
+    def gravity_info(self) -> Tuple[int, int, float, int]:
+        """(levels or -1, last solve's V-cycles, its Dif, step count)."""
+        lv, cyc, dif, it = ctypes.c_int(), ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
+        _check(_lib.tlq_gravity_info(self._ctx, ctypes.byref(lv), ctypes.byref(cyc),
+                                     ctypes.byref(dif), ctypes.byref(it)))
+        return lv.value, cyc.value, dif.value, it.value
+
     def run_until(self, tfin: float, max_steps: int = 10 ** 9) -> int:
         """mainHD.f90:61 loop: step while t < tfin (no clamp of the last dt)."""
         _, t, it0 = self.time()
@@ -312,7 +350,7 @@ class HydroCore:
 
     def kernel_stats(self) -> dict:
         out = {}
-        for q in range(4):
+        for q in range(5):
             name = ctypes.create_string_buffer(32)
             ms, n = ctypes.c_double(), ctypes.c_longlong()
             _check(_lib.tlq_kernel_stats(self._ctx, q, name, 32, ctypes.byref(ms), ctypes.byref(n)))
