@@ -19,6 +19,9 @@ per-cell fallback (trained in this run on the GPU), and the converged
 integrator (the surrogate's target and fallback), on the workload as given,
 at T = 3e5 K, and at T = 3e5 K with every density times 100 (strongly
 cooling: dt/t_cool of order 1-10, hundreds of integrator substeps per cell).
+A last leg runs the same workload with self-gravity (the multigrid Poisson
+solve of MultiGrid, multipole boundary values every 10 steps) and reports
+its V-cycles and time per step.
 """
 from __future__ import annotations
 
@@ -129,6 +132,39 @@ def cooling_modes(p, u0, table, warmup, steps):
     return out
 
 
+def self_gravity_leg(p, u0, table, warmup, steps):
+    """The workload with self-gravity (TLQ_SG_MULTIPOLE): zone-updates/s of
+    whole steps, multigrid ms and V-cycles per step."""
+    from tlaloque_amd import params as P
+    from tlaloque_amd.hydro import HydroCore
+    q = p.replace(self_gravity=P.SG_MULTIPOLE)
+    with HydroCore(q, cooling_table=table) as h:
+        t0 = time.perf_counter()
+        h.set_state(u0)
+        lv, c0, _, _ = h.gravity_info()
+        log(f"bench: self-gravity start-up solve: {c0} V-cycles, {time.perf_counter() - t0:.2f}s")
+        h.advance(warmup)
+        h.sync()
+        h.kernel_timing(True)
+        h.timer_start()
+        cycles = []
+        for _ in range(steps):
+            h.advance(1)
+            cycles.append(h.gravity_info()[1])
+        ms = h.timer_stop()
+        ks = h.kernel_stats()
+        h.kernel_timing(False)
+        bad = h.check_state()
+    mg_ms = ks["multigrid"][0] / max(ks["multigrid"][1], 1)
+    out = {"zone_updates_per_s": q.zones * steps / (ms / 1e3), "ms_per_step": ms / steps,
+           "multigrid_ms_per_step": mg_ms, "levels": lv,
+           "vcycles_per_step": float(np.mean(cycles)), "vcycles_startup": c0,
+           "ms_per_vcycle": mg_ms / max(float(np.mean(cycles)), 1e-9),
+           "boundary": "multipole every 10 steps", "nonfinite_cells": bad}
+    log(f"bench: self-gravity: {out}")
+    return out
+
+
 def cpu_baseline(n: int, steps: int, threads: int):
     """The FP64 C oracle on the same workload at n^3 (bounded sample)."""
     import oracle
@@ -180,6 +216,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-l1", action="store_true")
     ap.add_argument("--no-modes", action="store_true", help="skip the cooling-path comparison")
+    ap.add_argument("--no-gravity", action="store_true", help="skip the self-gravity leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -256,6 +293,8 @@ def main():
     }
     if not args.no_modes:
         result["cooling_paths"] = cooling_modes(p, u0, table, args.warmup, args.steps)
+    if not args.no_gravity:
+        result["self_gravity"] = self_gravity_leg(p, u0, table, args.warmup, args.steps)
     del u0
     if not args.no_l1:
         t0 = time.perf_counter()

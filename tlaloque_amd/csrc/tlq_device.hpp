@@ -83,6 +83,12 @@ struct Problem {
     const double *phi;
 };
 
+// Block barrier for LDS hand-off: waits for this wave's LDS operations only,
+// so prefetched global loads are not drained.
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // PHI at cell (i, j, k), 0 <= i <= nx+1 (likewise j, k).
 __device__ __forceinline__ double phi_at(const Problem &P, int i, int j, int k) {
     return P.phi[(long long)i + (long long)(P.g.nx + 2) * ((long long)j + (long long)(P.g.ny + 2) * k)];

@@ -43,12 +43,6 @@ __device__ __forceinline__ bool tile_of_block(const Tiling &T, int &bx, int &by,
     return true;
 }
 
-// Block barrier for LDS hand-off: waits for this wave's LDS operations only,
-// so prefetched global loads are not drained.
-__device__ __forceinline__ void lds_barrier() {
-    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-}
-
 // Face-ghost images of an interior cell along one axis (index a in 1..n):
 // calls f(ghost_layer, negate) for every ghost layer the boundary fill copies
 // from this cell -- two layers (boundariesII) or one (boundariesI).

@@ -51,7 +51,10 @@ __global__ void k_sg_fill(double *__restrict__ a, long long n, double v) {
 // axis inwards, and a cell's latest copy reads a cell whose other ghost axes
 // were copied earlier.  Threads cover the x faces (all j, k), the y faces
 // (interior i) and the z faces (interior i, j): every ghost cell once.
-__global__ void k_sg_bc(double *__restrict__ e, int nx, int ny, int nz) {
+// With e2, the same values also go to the second buffer of the level; with
+// copy_only, the ghost cells are copied from e to e2 unchanged.
+__global__ void k_sg_bc(double *__restrict__ e, double *__restrict__ e2, int nx, int ny, int nz,
+                        int copy_only) {
     const long long nxf = 2ll * (ny + 2) * (nz + 2), nyf = 2ll * nx * (nz + 2), nzf = 2ll * nx * ny;
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     int i, j, k;
@@ -73,12 +76,19 @@ __global__ void k_sg_bc(double *__restrict__ e, int nx, int ny, int nz) {
     } else {
         return;
     }
+    const long long c = sidx(nx, ny, i, j, k);
+    if (copy_only) {
+        e2[c] = e[c];
+        return;
+    }
     int ci = i, cj = j, ck = k, ng = 0;
     if (ci == 0) { ci = 1; ++ng; } else if (ci == nx + 1) { ci = nx; ++ng; }
     if (cj == 0) { cj = 1; ++ng; } else if (cj == ny + 1) { cj = ny; ++ng; }
     if (ck == 0) { ck = 1; ++ng; } else if (ck == nz + 1) { ck = nz; ++ng; }
     const double v = e[sidx(nx, ny, ci, cj, ck)];
-    e[sidx(nx, ny, i, j, k)] = (ng & 1) ? -v : v;
+    const double gv = (ng & 1) ? -v : v;
+    e[c] = gv;
+    if (e2) e2[c] = gv;
 }
 
 // One red-black half sweep: cells with (i + j + k) % 2 == color take the
@@ -113,6 +123,153 @@ __global__ __launch_bounds__(RB_TX *RB_TY) void k_sg_relax(
         const double b = block_sum<RB_TX * RB_TY>(d, red);
         if (threadIdx.x == 0 && threadIdx.y == 0)
             partial[blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z)] = b;
+    }
+}
+
+// One full red-black iteration in one pass, reading `ein` and writing the
+// interior of `eout` (the ghost cells of eout must equal ein's): colour 0
+// from the old values of its neighbours, then colour 1 from the new colour-0
+// values -- the same updates, in the same order of terms, as the two half
+// sweeps of k_sg_relax, at one read and one write of E per iteration.
+// A block owns a ZX x ZY column tile and marches planes k0..k1; at step t it
+// computes colour 0 of plane t+1 on the tile and a ring of 1 (the ring values
+// are recomputed by the neighbour tiles, identically), then colour 1 of plane
+// t on the tile, and writes plane t.  The old plane t+3 (ring 2 around the
+// tile) and the sources of the next step are loaded into registers during
+// step t, and the barriers wait on LDS only, so the loads stay in flight.
+// Block ids are remapped so that each of the 8 XCDs (blocks go to them round
+// robin) takes a contiguous range of tiles: tiles that share ring cells then
+// share one L2.
+constexpr int ZX = 32, ZY = 16, ZZ = 32;
+
+__device__ __forceinline__ int mod3(int t) { return ((t % 3) + 3) % 3; }
+
+template <class R, bool FROM_U>
+__global__ __launch_bounds__(ZX *ZY) void k_sg_rbz(
+    const double *__restrict__ ein, double *__restrict__ eout, const double *__restrict__ rs,
+    const R *__restrict__ U, Grid g, double fpg, int nx, int ny, int nz, double dxl,
+    double *__restrict__ partial) {
+    constexpr int OX = ZX + 4, OY = ZY + 4, NX = ZX + 2, NY = ZY + 2, T = ZX * ZY;
+    static_assert(OX * OY <= 2 * T && NX * NY <= 2 * T, "two ring cells per thread");
+    __shared__ double O[3][OY][OX];   // old planes, ring 2
+    __shared__ double N[3][NY][NX];   // new planes, ring 1
+    __shared__ double red[T / 64];
+    const int tid = threadIdx.x;
+    const long long nb = (long long)gridDim.x * gridDim.y * gridDim.z;
+    long long b = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+    if (nb % 8 == 0) b = (b % 8) * (nb / 8) + b / 8;
+    const int bx = (int)(b % gridDim.x), by = (int)((b / gridDim.x) % gridDim.y),
+              bz = (int)(b / ((long long)gridDim.x * gridDim.y));
+    const int x0 = bx * ZX, y0 = by * ZY;
+    const int k0 = 1 + bz * ZZ, k1 = min(k0 + ZZ - 1, nz);
+    const double h2 = dxl * dxl;
+    auto src = [&](int i, int j, int k) -> double {
+        if constexpr (FROM_U) return src0<R>(U, g, fpg, i, j, k);
+        else return rs[sidx(nx, ny, i, j, k)];
+    };
+    // old plane k (0 <= k <= nz+1), this thread's two ring-2 cells
+    auto fetch_plane = [&](int k, double (&r)[2]) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int q = tid + s * T;
+            const int lx = q % OX - 2, ly = q / OX - 2;
+            const int i = x0 + 1 + lx, j = y0 + 1 + ly;
+            r[s] = (q < OX * OY && i >= 0 && i <= nx + 1 && j >= 0 && j <= ny + 1)
+                       ? ein[sidx(nx, ny, i, j, k)] : 0.0;
+        }
+    };
+    auto store_plane = [&](int k, const double (&r)[2]) {
+        double(*o)[OX] = O[mod3(k)];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int q = tid + s * T;
+            if (q < OX * OY) o[q / OX][q % OX] = r[s];
+        }
+    };
+    // sources of this thread's two ring-1 cells on plane kr (colour 0, interior)
+    auto fetch_src_red = [&](int kr, double (&r)[2]) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int q = tid + s * T;
+            const int i = x0 + q % NX, j = y0 + q / NX;
+            r[s] = (q < NX * NY && i >= 1 && i <= nx && j >= 1 && j <= ny && kr >= 1 && kr <= nz &&
+                    ((i + j + kr) & 1) == 0) ? src(i, j, kr) : 0.0;
+        }
+    };
+    // source of this thread's tile cell on plane t (colour 1, interior)
+    auto fetch_src_blk = [&](int t) -> double {
+        const int i = x0 + 1 + tid % ZX, j = y0 + 1 + tid / ZX;
+        return (i <= nx && j <= ny && t >= 1 && t <= nz && ((i + j + t) & 1) == 1) ? src(i, j, t) : 0.0;
+    };
+    double pf[2], sr[2], sb;
+    if (k0 - 2 >= 0) {
+        fetch_plane(k0 - 2, pf);
+        store_plane(k0 - 2, pf);
+    }
+    fetch_plane(k0 - 1, pf);
+    store_plane(k0 - 1, pf);
+    fetch_plane(k0, pf);        // plane t+2 of the first step (k0 <= nz)
+    fetch_src_red(k0 - 1, sr);
+    sb = fetch_src_blk(k0 - 2);
+    double dsum = 0.0;
+    for (int t = k0 - 2; t <= k1; ++t) {
+        if (t + 2 <= nz + 1) store_plane(t + 2, pf);
+        const double sr0 = sr[0], sr1 = sr[1], sbc = sb;
+        if (t + 3 <= nz + 1) fetch_plane(t + 3, pf);
+        fetch_src_red(t + 2, sr);
+        sb = fetch_src_blk(t + 1);
+        lds_barrier();
+        // colour 0 of plane kr = t+1 on the tile and its ring of 1
+        const int kr = t + 1;
+        {
+            double(*om)[OX] = O[mod3(kr - 1)];
+            double(*oc)[OX] = O[mod3(kr)];
+            double(*op)[OX] = O[mod3(kr + 1)];
+            double(*nn)[NX] = N[mod3(kr)];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int q = tid + s * T;
+                if (q >= NX * NY) continue;
+                const int lx = q % NX - 1, ly = q / NX - 1;
+                const int i = x0 + 1 + lx, j = y0 + 1 + ly;
+                const int a = lx + 2, b = ly + 2;
+                double v = oc[b][a];
+                if (i >= 1 && i <= nx && j >= 1 && j <= ny && kr >= 1 && kr <= nz &&
+                    ((i + j + kr) & 1) == 0) {
+                    const double old = v;
+                    v = ((1.0 / 6.0) * (oc[b][a + 1] + oc[b][a - 1] + oc[b + 1][a] + oc[b - 1][a] +
+                                        op[b][a] + om[b][a] - (s ? sr1 : sr0) * h2)) * 1.0 +
+                        (1.0 - 1.0) * old;
+                    if (lx >= 0 && lx < ZX && ly >= 0 && ly < ZY && kr >= k0 && kr <= k1)
+                        dsum += fabs(v - old);
+                }
+                nn[ly + 1][lx + 1] = v;
+            }
+        }
+        lds_barrier();
+        // colour 1 of plane t on the tile, and the write of plane t
+        if (t >= k0) {
+            const int lx = tid % ZX, ly = tid / ZX;
+            const int i = x0 + 1 + lx, j = y0 + 1 + ly;
+            if (i <= nx && j <= ny) {
+                double(*nc)[NX] = N[mod3(t)];
+                const int a = lx + 1, b = ly + 1;
+                double v = nc[b][a];
+                if (((i + j + t) & 1) == 1) {
+                    const double old = O[mod3(t)][ly + 2][lx + 2];
+                    v = ((1.0 / 6.0) * (nc[b][a + 1] + nc[b][a - 1] + nc[b + 1][a] + nc[b - 1][a] +
+                                        N[mod3(t + 1)][b][a] + N[mod3(t - 1)][b][a] - sbc * h2)) * 1.0 +
+                        (1.0 - 1.0) * old;
+                    dsum += fabs(v - old);
+                }
+                eout[sidx(nx, ny, i, j, t)] = v;
+            }
+        }
+        lds_barrier();
+    }
+    if (partial) {
+        const double bs = block_sum<T>(dsum, red);
+        if (tid == 0) partial[b] = bs;
     }
 }
 
@@ -206,15 +363,18 @@ __global__ __launch_bounds__(NT) void k_sg_coarse(double *__restrict__ e,
     }
 }
 
-// Sum of n partials in a fixed order (one workgroup) -> out[0].
+// Sums of n partials (count interleaved sums, stride apart) in a fixed
+// order: block b sums its contiguous share into out[b * count + q].
 __global__ __launch_bounds__(NT) void k_sg_sum(const double *__restrict__ partial, long long n,
                                                int stride, int count, double *__restrict__ out) {
     __shared__ double red[NT / 64];
+    const long long chunk = (n + gridDim.x - 1) / gridDim.x;
+    const long long lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
     for (int q = 0; q < count; ++q) {
         double s = 0.0;
-        for (long long p = threadIdx.x; p < n; p += NT) s += partial[p * stride + q];
+        for (long long p = lo + threadIdx.x; p < hi; p += NT) s += partial[p * stride + q];
         const double b = block_sum<NT>(s, red);
-        if (threadIdx.x == 0) out[q] = b;
+        if (threadIdx.x == 0) out[(long long)blockIdx.x * count + q] = b;
     }
 }
 
@@ -369,6 +529,19 @@ long long relax_blocks(const SgLevel &L) {
 }
 
 constexpr int MOM_BLOCKS = 1024;
+constexpr int SUM_BLOCKS = 256;
+constexpr long long RBZ_MIN_CELLS = 32768;   // levels from 32^3 up take the one-pass iteration
+
+bool use_rbz(const SgLevel &L) { return (long long)L.nx * L.ny * L.nz >= RBZ_MIN_CELLS; }
+
+dim3 rbz_grid(const SgLevel &L) {
+    return dim3(nblocks(L.nx, ZX), nblocks(L.ny, ZY), nblocks(L.nz, ZZ));
+}
+
+long long rbz_blocks(const SgLevel &L) {
+    const dim3 gd = rbz_grid(L);
+    return (long long)gd.x * gd.y * gd.z;
+}
 
 }  // namespace
 
@@ -395,6 +568,12 @@ hipError_t sg_alloc(SgState &S, int nx, int ny, int nz, int levels, double xspan
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_sg_fill, dim3(std::min<long long>(nblocks(n, NT), 65535)), dim3(NT), 0,
                            stream, L.e, (long long)n, 0.0);
+        if (use_rbz(L)) {
+            e = hipMalloc(&L.e2, n * sizeof(double));
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(k_sg_fill, dim3(std::min<long long>(nblocks(n, NT), 65535)), dim3(NT),
+                               0, stream, L.e2, (long long)n, 0.0);
+        }
         if (l > 0) {
             e = hipMalloc(&L.r, n * sizeof(double));
             if (e != hipSuccess) return e;
@@ -402,8 +581,14 @@ hipError_t sg_alloc(SgState &S, int nx, int ny, int nz, int levels, double xspan
                                0, stream, L.r, (long long)n, 0.0);
         }
     }
-    S.npartial = std::max<long long>(2 * relax_blocks(S.lv[0]), (long long)MOM_BLOCKS * SG_NMOM);
+    S.phi0 = S.lv[0].e;
+    long long np = (long long)MOM_BLOCKS * SG_NMOM;
+    for (int l = 0; l <= levels; ++l)
+        np = std::max(np, use_rbz(S.lv[l]) ? rbz_blocks(S.lv[l]) : 2 * relax_blocks(S.lv[l]));
+    S.npartial = np;
     hipError_t e = hipMalloc(&S.partial, (size_t)S.npartial * sizeof(double));
+    if (e != hipSuccess) return e;
+    e = hipMalloc(&S.partial2, (size_t)SUM_BLOCKS * SG_NMOM * sizeof(double));
     if (e != hipSuccess) return e;
     e = hipMalloc(&S.scal, (1 + SG_NMOM) * sizeof(double));
     if (e != hipSuccess) return e;
@@ -413,22 +598,50 @@ hipError_t sg_alloc(SgState &S, int nx, int ny, int nz, int levels, double xspan
 void sg_free(SgState &S) {
     for (int l = 0; l <= S.levels; ++l) {
         if (S.lv[l].e) (void)hipFree(S.lv[l].e);
+        if (S.lv[l].e2) (void)hipFree(S.lv[l].e2);
         if (S.lv[l].r) (void)hipFree(S.lv[l].r);
-        S.lv[l].e = S.lv[l].r = nullptr;
+        S.lv[l].e = S.lv[l].e2 = S.lv[l].r = nullptr;
     }
     if (S.partial) (void)hipFree(S.partial);
+    if (S.partial2) (void)hipFree(S.partial2);
     if (S.scal) (void)hipFree(S.scal);
-    S.partial = S.scal = nullptr;
+    S.partial = S.partial2 = S.scal = S.phi0 = nullptr;
 }
 
 namespace {
 
-// poisson_solver's Relax (sources.f90:378-476): it-1 = 5 iterations of two
-// half sweeps.  With want_dif, the last iteration's sum of |change| goes to
+// Fixed-order sum of n partials (count interleaved sums) into out[0..count).
+void sum_partials(hipStream_t s, SgState &S, const double *partial, long long n, int count,
+                  double *out) {
+    hipLaunchKernelGGL(k_sg_sum, dim3(SUM_BLOCKS), dim3(NT), 0, s, partial, n, count, count,
+                       S.partial2);
+    hipLaunchKernelGGL(k_sg_sum, dim3(1), dim3(NT), 0, s, S.partial2, (long long)SUM_BLOCKS, count,
+                       count, out);
+}
+
+// poisson_solver's Relax (sources.f90:378-476): it-1 = 5 iterations, each a
+// colour-0 then a colour-1 sweep -- on large levels one pass per iteration
+// into the second buffer (the buffers then swap), else two in-place half
+// sweeps.  With want_dif, the last iteration's sum of |change| goes to
 // S.scal[0].
 template <class R>
 void relax(hipStream_t s, const Grid &g, const R *U, double fpg, SgState &S, int l, bool want_dif) {
     SgLevel &L = S.lv[l];
+    if (use_rbz(L)) {
+        const dim3 gd = rbz_grid(L);
+        for (int it = 1; it <= 5; ++it) {
+            double *part = (want_dif && it == 5) ? S.partial : nullptr;
+            if (l == 0)
+                hipLaunchKernelGGL((k_sg_rbz<R, true>), gd, dim3(ZX * ZY), 0, s, L.e, L.e2, nullptr,
+                                   U, g, fpg, L.nx, L.ny, L.nz, L.dxl, part);
+            else
+                hipLaunchKernelGGL((k_sg_rbz<R, false>), gd, dim3(ZX * ZY), 0, s, L.e, L.e2, L.r, U,
+                                   g, fpg, L.nx, L.ny, L.nz, L.dxl, part);
+            std::swap(L.e, L.e2);
+        }
+        if (want_dif) sum_partials(s, S, S.partial, rbz_blocks(L), 1, S.scal);
+        return;
+    }
     const dim3 gd = relax_grid(L), bd(RB_TX, RB_TY);
     const long long nb = (long long)gd.x * gd.y * gd.z;
     for (int it = 1; it <= 5; ++it)
@@ -441,13 +654,20 @@ void relax(hipStream_t s, const Grid &g, const R *U, double fpg, SgState &S, int
                 hipLaunchKernelGGL((k_sg_relax<R, false>), gd, bd, 0, s, L.e, L.r, U, g, fpg, L.nx,
                                    L.ny, L.nz, L.dxl, color, part);
         }
-    if (want_dif)
-        hipLaunchKernelGGL(k_sg_sum, dim3(1), dim3(NT), 0, s, S.partial, 2 * nb, 1, 1, S.scal);
+    if (want_dif) sum_partials(s, S, S.partial, 2 * nb, 1, S.scal);
 }
 
 void bc(hipStream_t s, SgLevel &L) {
     const long long n = 2ll * (L.ny + 2) * (L.nz + 2) + 2ll * L.nx * (L.nz + 2) + 2ll * L.nx * L.ny;
-    hipLaunchKernelGGL(k_sg_bc, dim3(nblocks(n, NT)), dim3(NT), 0, s, L.e, L.nx, L.ny, L.nz);
+    hipLaunchKernelGGL(k_sg_bc, dim3(nblocks(n, NT)), dim3(NT), 0, s, L.e, L.e2, L.nx, L.ny, L.nz, 0);
+}
+
+// Level 0's ghost cells into its second buffer (they hold PHI's boundary
+// values, which the one-pass iterations read from either buffer).
+void copy_shell(hipStream_t s, SgLevel &L) {
+    if (!L.e2) return;
+    const long long n = 2ll * (L.ny + 2) * (L.nz + 2) + 2ll * L.nx * (L.nz + 2) + 2ll * L.nx * L.ny;
+    hipLaunchKernelGGL(k_sg_bc, dim3(nblocks(n, NT)), dim3(NT), 0, s, L.e, L.e2, L.nx, L.ny, L.nz, 1);
 }
 
 }  // namespace
@@ -460,14 +680,14 @@ hipError_t sg_solve(hipStream_t s, const Grid &g, const R *U, SgState &S, int it
     SgLevel &L0 = S.lv[0];
     if (S.boundary == 2 && it % 10 == 0) {   // sources.f90:114-136
         hipLaunchKernelGGL((k_sg_moments<R>), dim3(MOM_BLOCKS), dim3(NT), 0, s, dP, U, S.partial);
-        hipLaunchKernelGGL(k_sg_sum, dim3(1), dim3(NT), 0, s, S.partial, (long long)MOM_BLOCKS,
-                           SG_NMOM, SG_NMOM, S.scal + 1);
+        sum_partials(s, S, S.partial, (long long)MOM_BLOCKS, SG_NMOM, S.scal + 1);
         hipLaunchKernelGGL(k_sg_moment_factors, dim3(1), dim3(64), 0, s, S.scal + 1);
         const long long nf = 2ll * (L0.ny * (long long)L0.nz + L0.nx * (long long)L0.nz +
                                     L0.nx * (long long)L0.ny);
         hipLaunchKernelGGL(k_sg_faces, dim3(nblocks(nf, NT)), dim3(NT), 0, s, dP, S.scal + 1,
                            S.gconst, L0.e);
     }
+    copy_shell(s, L0);
     double dif = 1.e28;
     int iter = 0;
     while (dif > 10.0 && iter < 31) {   // sources.f90:140
@@ -506,6 +726,12 @@ hipError_t sg_solve(hipStream_t s, const Grid &g, const R *U, SgState &S, int it
         if (e != hipSuccess) return e;
         e = hipStreamSynchronize(s);
         if (e != hipSuccess) return e;
+    }
+    if (L0.e != S.phi0) {   // an odd number of one-pass iterations: PHI is in the second buffer
+        const size_t n = (size_t)(L0.nx + 2) * (L0.ny + 2) * (L0.nz + 2);
+        hipError_t e = hipMemcpyAsync(S.phi0, L0.e, n * sizeof(double), hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return e;
+        std::swap(L0.e, L0.e2);
     }
     S.cycles = iter;
     S.dif = dif;

@@ -33,7 +33,8 @@ constexpr int SG_COARSE_CAP = 20000;    // iterations cap of the coarsest solve
 struct SgLevel {
     int nx, ny, nz;
     double dxl;    // (xr - xl) / nx_l, as poisson_solver and resid compute it
-    double *e;     // E_l (level 0: PHI)
+    double *e;     // E_l (level 0: PHI); the current buffer
+    double *e2;    // second buffer of the one-pass iterations (large levels), or null
     double *r;     // R_l (levels >= 1)
 };
 
@@ -44,6 +45,8 @@ struct SgState {
     SgLevel lv[SG_MAX_LEVELS + 1]{};
     double *partial = nullptr;   // per-block partial sums
     long long npartial = 0;
+    double *partial2 = nullptr;  // second stage of the sums
+    double *phi0 = nullptr;      // level 0's first buffer: PHI as the hydro kernels see it
     double *scal = nullptr;      // [0] Dif, [1..25] moments
     int cycles = 0;              // V-cycles of the last solve
     double dif = 0.0;            // its final Dif
