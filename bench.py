@@ -217,6 +217,8 @@ def main():
     ap.add_argument("--no-l1", action="store_true")
     ap.add_argument("--no-modes", action="store_true", help="skip the cooling-path comparison")
     ap.add_argument("--no-gravity", action="store_true", help="skip the self-gravity leg")
+    ap.add_argument("--gravity-only", action="store_true",
+                    help="run only the self-gravity leg and print its JSON (for profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -232,6 +234,12 @@ def main():
     p, u0 = problems.clumpy(n=n, precision=P.PREC_FP32)
     log(f"bench: clumpy {n}^3 initial state built in {time.perf_counter() - t0:.1f}s")
     table = P.load_cooling_table(os.path.join(ROOT, "tlaloque_amd", "data", "Z1.0.dat"))
+    if args.gravity_only:
+        print(json.dumps({"self_gravity": self_gravity_leg(p, u0, table, args.warmup, args.steps),
+                          "config": {"workload": f"clumpy self-gravitating medium {n}^3 with "
+                                                 "self-gravity (multipole boundaries)"}}),
+              flush=True)
+        return
     h = HydroCore(p, cooling_table=table)
     h.set_state(u0)
     h.advance(args.warmup)

@@ -33,7 +33,7 @@ COPY_KERNEL = "k_stream_copy"
 
 
 def short_name(name: str) -> str:
-    s = name.split("(")[0].replace("void ", "").strip()
+    s = name.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "").strip()
     return s
 
 
@@ -93,22 +93,31 @@ def counter_means(pmc_csv: str, counter: str) -> Dict[str, float]:
 
 
 def traffic_report(fetch_csv: str, write_csv: str, copy_bytes: int, zones: int,
-                   cmd_fetch: str, cmd_write: str, title: str) -> str:
+                   cmd_fetch: str, cmd_write: str, title: str, calibration: str = "") -> str:
+    """With `calibration` (another traffic report), its counter factors are
+    used instead of a copy kernel in these runs."""
     fetch = counter_means(fetch_csv, "FETCH_SIZE")
     write = counter_means(write_csv, "WRITE_SIZE")
-    ck = [k for k in fetch if COPY_KERNEL in k]
-    if not ck or not any(COPY_KERNEL in k for k in write):
-        raise SystemExit("no copy-kernel dispatch in the counter runs: cannot calibrate")
-    ck = ck[0]
-    # the copy reads copy_bytes and writes copy_bytes
-    f_fetch = copy_bytes / (fetch[ck] * 1024.0)
-    f_write = copy_bytes / (write[ck] * 1024.0)
+    if calibration:
+        cal = read_traffic(calibration)["calibration"]
+        f_fetch, f_write = cal["fetch_factor"], cal["write_factor"]
+        cal_line = (f"Calibration: the factors of `{calibration}` (FETCH_SIZE x {f_fetch:.4f}, "
+                    f"WRITE_SIZE x {f_write:.4f}, bytes per counted byte, from a streaming copy).")
+    else:
+        ck = [k for k in fetch if COPY_KERNEL in k]
+        if not ck or not any(COPY_KERNEL in k for k in write):
+            raise SystemExit("no copy-kernel dispatch in the counter runs: cannot calibrate")
+        ck = ck[0]
+        # the copy reads copy_bytes and writes copy_bytes
+        f_fetch = copy_bytes / (fetch[ck] * 1024.0)
+        f_write = copy_bytes / (write[ck] * 1024.0)
+        cal_line = (f"Calibration: `{COPY_KERNEL}` copies {copy_bytes} B (reads and writes that "
+                    f"many); FETCH_SIZE reads {fetch[ck]:.0f} KiB -> factor {f_fetch:.4f}, "
+                    f"WRITE_SIZE reads {write[ck]:.0f} KiB -> factor {f_write:.4f} (bytes per "
+                    f"counted byte).")
     lines = [f"# {title}", "",
              "Commands (one MI355X, from the repository root; one counter per run):", "",
-             "```", cmd_fetch, cmd_write, "```", "",
-             f"Calibration: `{COPY_KERNEL}` copies {copy_bytes} B (reads and writes that many); "
-             f"FETCH_SIZE reads {fetch[ck]:.0f} KiB -> factor {f_fetch:.4f}, WRITE_SIZE reads "
-             f"{write[ck]:.0f} KiB -> factor {f_write:.4f} (bytes per counted byte).", "",
+             "```", cmd_fetch, cmd_write, "```", "", cal_line, "",
              f"Per-zone figures divide by the {zones} interior zones of one launch.", "",
              "| kernel | FETCH KiB/launch | WRITE KiB/launch | scaled read B | scaled write B | "
              "total B/launch | B/zone |",
@@ -121,7 +130,12 @@ def traffic_report(fetch_csv: str, write_csv: str, copy_bytes: int, zones: int,
         per_zone = (rd + wr) / zones if zones else 0.0
         lines.append(f"| `{k}` | {fetch[k]:.0f} | {write[k]:.0f} | {rd:.4g} | {wr:.4g} | "
                      f"{rd + wr:.4g} | {per_zone:.1f} |")
-        js["kernels"][re.sub(r"<.*", "", k).split("::")[-1]] = {
+        # key: the bare kernel name; a second instantiation of the same kernel
+        # (listed after the one with more traffic) keeps its template arguments
+        key = re.sub(r"<.*", "", k).split("::")[-1]
+        if key in js["kernels"]:
+            key = k.split("::")[-1]
+        js["kernels"][key] = {
             "read_bytes": rd, "write_bytes": wr, "bytes": rd + wr, "bytes_per_zone": per_zone}
     lines += ["", "traffic-json: " + json.dumps(js)]
     return "\n".join(lines) + "\n"
@@ -151,12 +165,13 @@ def main(argv=None):
     t.add_argument("--cmd-fetch", required=True)
     t.add_argument("--cmd-write", required=True)
     t.add_argument("--title", required=True)
+    t.add_argument("--calibration", default="", help="reuse this traffic report's factors")
     a = ap.parse_args(argv)
     if a.what == "stats":
         sys.stdout.write(stats_report(a.trace_csv, a.cmd, a.title))
     else:
         sys.stdout.write(traffic_report(a.fetch_csv, a.write_csv, a.copy_bytes, a.zones,
-                                        a.cmd_fetch, a.cmd_write, a.title))
+                                        a.cmd_fetch, a.cmd_write, a.title, a.calibration))
 
 
 if __name__ == "__main__":
