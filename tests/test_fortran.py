@@ -239,3 +239,35 @@ def test_driver_sources_from_namelist_match_python_binding(tmp_path, driver):
     for e in range(5):
         scale = max(np.abs(w[..., e]).max(), 1e-300)
         assert np.abs(res[True][..., e] - w[..., e]).max() <= 1e-10 * scale, e
+
+
+@pytest.mark.gpu
+@needs_flang
+def test_driver_self_gravity_from_namelist_matches_python_binding(tmp_path, driver):
+    # self_gravity / mg_levels / gconst through the namelist and the bind(C)
+    # params; PHI starts at 0, so the multipole faces carry the boundary
+    from tlaloque_amd.hydro import HydroCore
+    L = P.PC
+    p = P.HydroParams(nx=16, ny=16, nz=16, xl=-L, xr=L, yl=-L, yr=L, zl=-L, zr=L,
+                      bc=(P.BC_OUTFLOW,) * 6, self_gravity=P.SG_MULTIPOLE, eta=0.02)
+    x = P.xcoord(p, np.arange(-1, p.nx + 3))
+    r = np.sqrt(x[None, None, :] ** 2 + x[None, :, None] ** 2 + x[:, None, None] ** 2)
+    rho = np.where(r < 0.4 * L, 1e-20, 1e-22)
+    u0 = problems.cons_from_prims(p, rho, np.zeros(r.shape + (3,)), np.full(r.shape, 1e-10))
+    init = tmp_path / "u0.bin"
+    np.ascontiguousarray(u0).astype("<f8").tofile(init)
+    out = tmp_path / "D"
+    out.mkdir()
+    nml = tmp_path / "sg.nml"
+    _namelist(nml, p, "file", out, tfin=1e30, num_out=1, outfile="bin",
+              extra=f", initfile='{init}', max_steps=12")
+    nml.write_text(nml.read_text().replace(
+        "/\n&run", f", self_gravity={P.SG_MULTIPOLE}, mg_levels=0, gconst={P.GR!r} /\n&run", 1))
+    _run(driver, nml)
+    w_drv = vtk.read_bin(vtk.bin_name(str(out), 1), p)[2:-2, 2:-2, 2:-2]
+    with HydroCore(p) as h:
+        h.set_state(u0)
+        h.advance(12)
+        h.calc_prims("U")
+        w = h.download("UPrim")[2:-2, 2:-2, 2:-2]
+    assert np.array_equal(w_drv, w)
