@@ -256,18 +256,88 @@ def test_driver_self_gravity_from_namelist_matches_python_binding(tmp_path, driv
     u0 = problems.cons_from_prims(p, rho, np.zeros(r.shape + (3,)), np.full(r.shape, 1e-10))
     init = tmp_path / "u0.bin"
     np.ascontiguousarray(u0).astype("<f8").tofile(init)
-    out = tmp_path / "D"
-    out.mkdir()
+    (tmp_path / "D").mkdir()
+    (tmp_path / "phi." / "D").mkdir(parents=True)   # PHIT goes to 'phi' + the BIN path
     nml = tmp_path / "sg.nml"
-    _namelist(nml, p, "file", out, tfin=1e30, num_out=1, outfile="bin",
+    _namelist(nml, p, "file", ".", tfin=1e30, num_out=1, outfile="bin",
               extra=f", initfile='{init}', max_steps=12")
     nml.write_text(nml.read_text().replace(
-        "/\n&run", f", self_gravity={P.SG_MULTIPOLE}, mg_levels=0, gconst={P.GR!r} /\n&run", 1))
-    _run(driver, nml)
-    w_drv = vtk.read_bin(vtk.bin_name(str(out), 1), p)[2:-2, 2:-2, 2:-2]
+        "/\n&run", f", self_gravity={P.SG_MULTIPOLE}, mg_levels=0, gconst={P.GR!r} /\n&run", 1)
+        .replace("outputpath='./'", "outputpath='./D/'"))
+    r = subprocess.run([driver, str(nml)], cwd=tmp_path, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:]
+    w_drv = vtk.read_bin(str(tmp_path / vtk.bin_name("./D/", 1)), p)[2:-2, 2:-2, 2:-2]
+    phit_drv = vtk.read_phi_bin(str(tmp_path / vtk.phi_bin_name("./D/", 1)), p)
     with HydroCore(p) as h:
         h.set_state(u0)
         h.advance(12)
         h.calc_prims("U")
         w = h.download("UPrim")[2:-2, 2:-2, 2:-2]
+        phi = h.download_phi()
     assert np.array_equal(w_drv, w)
+    assert np.array_equal(phit_drv, phi)          # no point masses: PHIT = PHI
+
+
+def _point_phit(p, phi):
+    x = P.xcoord(p, np.arange(0, p.nx + 2))
+    y = P.ycoord(p, np.arange(0, p.ny + 2))
+    z = P.zcoord(p, np.arange(0, p.nz + 2))
+    pp = np.zeros((p.nz + 2, p.ny + 2, p.nx + 2))
+    for m in p.masses:
+        r = np.sqrt((m.x - x[None, None, :]) ** 2 + (m.y - y[None, :, None]) ** 2
+                    + (m.z - z[:, None, None]) ** 2 + p.softening ** 2)
+        pp = pp - P.GR * m.mass / r
+    return pp + phi
+
+
+def _sg_namelist(nml, p, problem, out, outfile, extra, phifile=""):
+    _namelist(nml, p, problem, out, tfin=1e30, num_out=1, outfile=outfile, extra=extra)
+    m = p.masses[0]
+    pf = f", phifile='{phifile}'" if phifile else ""
+    nml.write_text(nml.read_text().replace(
+        "/\n&run", f", self_gravity={p.self_gravity}, softening={p.softening!r}{pf} /\n&run", 1)
+        .replace("&run ", f"&run nmasses=1, masses(1:4,1)={m.x!r},{m.y!r},{m.z!r},{m.mass!r}, ", 1))
+
+
+@needs_flang
+def test_dry_run_writes_the_potential_in_the_reference_formats(tmp_path, driver):
+    # -DGRAV outputs (HydroCore.f90:284-301, 360-372): PHIT to 'phi' + the BIN
+    # path, a ninth DAT column, a "Potential" VTK block; PHIT = PHIP + PHI
+    # with PHI as read from phifile
+    L = P.PC
+    p = P.HydroParams(nx=8, ny=6, nz=4, xl=-L, xr=L, yl=-0.75 * L, yr=0.75 * L, zl=-0.5 * L,
+                      zr=0.5 * L, self_gravity=P.SG_FIXED, softening=0.1 * L,
+                      masses=[P.PointMass(0.2 * L, -0.1 * L, 0.05 * L, 30 * P.MSUN)])
+    u0 = problems.uniform_state(p, 1e-22, 1e-11)
+    init = tmp_path / "u0.bin"
+    np.ascontiguousarray(u0).astype("<f8").tofile(init)
+    rng = np.random.default_rng(4)
+    phi0 = -1e10 * rng.uniform(0.5, 1.0, (p.nz + 2, p.ny + 2, p.nx + 2))
+    phif = tmp_path / "phi0.bin"
+    vtk.write_phi_bin(str(phif), phi0)
+    phit = _point_phit(p, phi0)
+    # outputpath './DATA/' run from tmp_path: PHIT goes to 'phi./DATA/...', a
+    # folder the run directory must hold, as for the reference
+    out = tmp_path / "DATA"
+    out.mkdir()
+    (tmp_path / "phi." / "DATA").mkdir(parents=True)
+    for fmt in ("bin", "vtk", "dat"):
+        nml = tmp_path / f"{fmt}.nml"
+        _sg_namelist(nml, p, "file", ".", fmt, f", initfile='{init}', dry_run=.true.", str(phif))
+        nml.write_text(nml.read_text().replace("outputpath='./'", "outputpath='./DATA/'"))
+        r = subprocess.run([driver, str(nml)], cwd=tmp_path, stdout=subprocess.PIPE,
+                           stderr=subprocess.STDOUT, text=True, timeout=60)
+        assert r.returncode == 0, r.stdout
+    got = vtk.read_phi_bin(str(tmp_path / vtk.phi_bin_name("./DATA/", 0)), p)
+    assert np.allclose(got, phit, rtol=1e-15, atol=0)
+    d = vtk.read_vtk(vtk.vtk_name(str(out), 0))
+    assert list(d["fields"]) == ["Density", "Pressure", "Potential", "Velocity"]
+    assert np.allclose(d["fields"]["Potential"], phit[1:-1, 1:-1, 1:-1], rtol=1e-15, atol=0)
+    ref = tmp_path / "ref.vtk"
+    from helpers import prims_from_cons
+    vtk.write_vtk(str(ref), p, prims_from_cons(u0, p.gamma), phit)
+    assert open(ref, "rb").read() == open(vtk.vtk_name(str(out), 0), "rb").read()
+    rows = np.loadtxt(str(out / "points000N000.dat"))
+    assert rows.shape == (p.nx * p.ny * p.nz, 9)
+    assert np.allclose(rows[:, 8], phit[1:-1, 1:-1, 1:-1].ravel(), rtol=1e-6, atol=0)  # e16.7

@@ -10,8 +10,10 @@
 !   &mesh     nx, ny, nz, xl, xr, yl, yr, zl, zr
 !   &physics  gamma, mu0, cfl, eta, cooling_limit, limiter, solver, precision,
 !             cooling, bc(6), softening, tfin, cooling_file, self_gravity,
-!             mg_levels, gconst (self-gravity: PHI starts at 0, so its ghost
-!             faces are 0 with TLQ_SG_FIXED; TLQ_SG_MULTIPOLE sets them)
+!             mg_levels, gconst, phifile (self-gravity: PHI starts from
+!             phifile, a little-endian real(8) stream of PHI(0:nx+1,0:ny+1,
+!             0:nz+1) whose ghost faces are the boundary values, else from 0;
+!             TLQ_SG_MULTIPOLE sets the faces itself)
 !   &run      problem, num_out, outfile, outputpath, max_steps, per_routine,
 !             initfile, dry_run, nwinds, winds(:), nmasses, masses(4,:)
 ! problem: 'sod'    diagonal Sod tube of the reference's test harness
@@ -40,6 +42,11 @@
 ! reference's analysis scripts read them unchanged.  VTK and BIN always hold
 ! real(8) values under the 'double' header.
 ! Every output also writes <file>.time, one line "t it nout", for restarts.
+! With self-gravity the outputs also carry PHIT = PHIP + PHI as the -DGRAV
+! build writes it (HydroCore.f90:284-290, 295-301, 360-372): BIN to 'phi'
+! prefixed to the BIN path (e.g. phi./DATA/points000N001.bin; that folder
+! must exist, as for the reference), a ninth DAT column, and a "SCALARS
+! Potential" block in the VTK.
 ! dry_run = .true. writes the initial state's output without touching the GPU.
 module driver_setup
   use iso_c_binding
@@ -61,7 +68,7 @@ module driver_setup
   integer :: self_gravity = TLQ_SG_OFF, mg_levels = 0
   real(8) :: gconst = 6.67430d-8
   integer :: bc(6) = TLQ_BC_REFLECTIVE
-  character(len=512) :: cooling_file = ''
+  character(len=512) :: cooling_file = '', phifile = ''
   ! &run
   character(len=32) :: problem = 'sod'
   integer :: num_out = 1, max_steps = 0, nwinds = 0, nmasses = 0
@@ -76,7 +83,7 @@ module driver_setup
 
   namelist /mesh/ nx, ny, nz, xl, xr, yl, yr, zl, zr
   namelist /physics/ gamma, mu0, cfl, eta, cooling_limit, limiter, solver, precision, cooling, &
-       bc, softening, tfin, cooling_file, self_gravity, mg_levels, gconst
+       bc, softening, tfin, cooling_file, self_gravity, mg_levels, gconst, phifile
   namelist /run/ problem, num_out, outfile, outputpath, max_steps, per_routine, initfile, &
        dry_run, nwinds, winds, nmasses, masses, restart_file, restart_t, restart_it, restart_nout
 
@@ -372,6 +379,29 @@ contains
     if (restart_nout >= 0) nout = restart_nout
   end subroutine restart_clock
 
+  ! PHIT = PHIP + PHI (HLLC.f90:22): the point masses' potential
+  ! (pointsource_gravity.f90:30-35, with the softening of tlq_params) added to
+  ! the self-gravity potential PHI, on every cell of PHI(0:nx+1, ...).
+  subroutine make_phit(PHI, PHIT)
+    real(8), intent(in) :: PHI(0:nx+1, 0:ny+1, 0:nz+1)
+    real(8), intent(out) :: PHIT(0:nx+1, 0:ny+1, 0:nz+1)
+    integer :: i, j, k, s
+    real(8) :: pp, r
+    do k = 0, nz + 1
+       do j = 0, ny + 1
+          do i = 0, nx + 1
+             pp = 0d0
+             do s = 1, nmasses
+                r = sqrt((masses(1, s) - xc(i))**2 + (masses(2, s) - yc(j))**2 + &
+                     (masses(3, s) - zc(k))**2 + softening**2)
+                pp = pp - 6.67430d-8 * masses(4, s) / r
+             end do
+             PHIT(i, j, k) = pp + PHI(i, j, k)
+          end do
+       end do
+    end do
+  end subroutine make_phit
+
   ! calc_prims on the host (source/HydroCore.f90:14-21), for dry runs.
   subroutine host_prims(UPrim, U)
     real(8), intent(out) :: UPrim(5, -1:nx+2, -1:ny+2, -1:nz+2)
@@ -417,12 +447,14 @@ contains
   ! Output number nout of UPrim, in the reference's formats
   ! (source/HydroCore.f90:253-395, single rank), and its sidecar
   ! <file>.time holding t, it and nout; then nout += 1 and tout = dtout * nout.
-  subroutine output(UPrim, nout, tout, dtout, t, it)
+  subroutine output(UPrim, nout, tout, dtout, t, it, PHIT)
     real(8), intent(in) :: UPrim(5, -1:nx+2, -1:ny+2, -1:nz+2)
     integer, intent(inout) :: nout
     real(8), intent(inout) :: tout
     real(8), intent(in) :: dtout, t
     integer(c_int), intent(in) :: it
+    real(8), intent(in), optional :: PHIT(0:nx+1, 0:ny+1, 0:nz+1)
+    integer :: ios
     character(len=600) :: fname
     character(len=200) :: line
     character(len=1), parameter :: lf = char(10)
@@ -434,13 +466,28 @@ contains
             form='unformatted', convert='LITTLE_ENDIAN')
        write(u) UPrim
        close(u)
+       if (present(PHIT)) then
+          open(newunit=u, file='phi'//trim(fname), status='replace', access='stream', &
+               form='unformatted', convert='LITTLE_ENDIAN', iostat=ios)
+          if (ios == 0) then
+             write(u) PHIT
+             close(u)
+          else
+             write(*, '(a,a)') ' cannot open ', 'phi'//trim(fname)
+          end if
+       end if
     case ('dat')
        write(fname, '(a,a,i3.3,a,i3.3,a)') trim(outputpath), 'points', 0, 'N', nout, '.dat'
        open(newunit=u, file=trim(fname), status='replace')
        do k = 1, nz
           do j = 1, ny
              do i = 1, nx
-                write(u, '(8(e16.7e3))') dble(i) * dx, dble(j) * dy, dble(k) * dz, UPrim(:, i, j, k)
+                if (present(PHIT)) then
+                   write(u, '(14(e16.7e3))') dble(i) * dx, dble(j) * dy, dble(k) * dz, &
+                        UPrim(:, i, j, k), PHIT(i, j, k)
+                else
+                   write(u, '(14(e16.7e3))') dble(i) * dx, dble(j) * dy, dble(k) * dz, UPrim(:, i, j, k)
+                end if
              end do
           end do
        end do
@@ -468,6 +515,12 @@ contains
        write(u) 'SCALARS Pressure double 1', lf
        write(u) 'LOOKUP_TABLE default', lf
        write(u) (((UPrim(5, i, j, k), i = 1, nx), j = 1, ny), k = 1, nz)
+       if (present(PHIT)) then
+          write(u) lf
+          write(u) 'SCALARS Potential double 1', lf
+          write(u) 'LOOKUP_TABLE default', lf
+          write(u) (((PHIT(i, j, k), i = 1, nx), j = 1, ny), k = 1, nz)
+       end if
        write(u) lf
        write(u) 'VECTORS Velocity double', lf
        write(u) (((UPrim(2, i, j, k), UPrim(3, i, j, k), UPrim(4, i, j, k), i = 1, nx), &
@@ -498,11 +551,12 @@ program tlq_hydro
 
   character(len=512) :: path
   real(8), allocatable :: U(:, :, :, :), UPrim(:, :, :, :), temp(:), logl(:)
+  real(8), allocatable :: PHI(:, :, :), PHIT(:, :, :)
   type(tlq_params_t) :: prm
   type(c_ptr) :: ctx
   real(8) :: t, dt, tout, dtout
   integer(c_int) :: it, cit
-  integer :: nout, ntab
+  integer :: nout, ntab, u_phi
   integer(c_long_long) :: bad
 
   if (command_argument_count() < 1) then
@@ -513,6 +567,14 @@ program tlq_hydro
   call read_settings(trim(path))
 
   allocate(U(5, -1:nx+2, -1:ny+2, -1:nz+2), UPrim(5, -1:nx+2, -1:ny+2, -1:nz+2))
+  allocate(PHI(0:nx+1, 0:ny+1, 0:nz+1), PHIT(0:nx+1, 0:ny+1, 0:nz+1))
+  PHI = 0d0
+  if (len_trim(phifile) > 0) then
+     open(newunit=u_phi, file=trim(phifile), status='old', access='stream', form='unformatted', &
+          convert='LITTLE_ENDIAN', action='read')
+     read(u_phi) PHI
+     close(u_phi)
+  end if
   U = 0d0
   call initflow(U)
 
@@ -531,7 +593,12 @@ program tlq_hydro
 
   if (dry_run) then
      call host_prims(UPrim, U)
-     call output(UPrim, nout, tout, dtout, t, it)
+     if (self_gravity /= TLQ_SG_OFF) then
+        call make_phit(PHI, PHIT)
+        call output(UPrim, nout, tout, dtout, t, it, PHIT)
+     else
+        call output(UPrim, nout, tout, dtout, t, it)
+     end if
   else
      call run_gpu()
   end if
@@ -550,6 +617,7 @@ contains
     if (nmasses > 0) call tlq_require(tlq_set_point_masses(ctx, masses, int(nmasses, c_int)), &
          'tlq_set_point_masses')
     call tlq_require(tlq_upload(ctx, TLQ_BUF_U, U), 'tlq_upload')
+    if (self_gravity /= TLQ_SG_OFF) call tlq_require(tlq_upload_phi(ctx, PHI), 'tlq_upload_phi')
     if (per_routine) then
        if (trim(problem) /= 'restart' .or. restart_fill) &
             call tlq_require(tlq_borders(ctx, TLQ_BUF_U, 1_c_int), 'borders')     ! mainHD.f90:38
@@ -592,7 +660,13 @@ contains
   subroutine write_output()
     if (.not. per_routine) call tlq_require(tlq_calc_prims(ctx, TLQ_BUF_U), 'calc_prims')
     call tlq_require(tlq_download(ctx, TLQ_BUF_UPRIM, UPrim), 'tlq_download')
-    call output(UPrim, nout, tout, dtout, t, it)
+    if (self_gravity /= TLQ_SG_OFF) then
+       call tlq_require(tlq_download_phi(ctx, PHI), 'tlq_download_phi')
+       call make_phit(PHI, PHIT)
+       call output(UPrim, nout, tout, dtout, t, it, PHIT)
+    else
+       call output(UPrim, nout, tout, dtout, t, it)
+    end if
   end subroutine write_output
 
 end program tlq_hydro

@@ -10,6 +10,12 @@ one rank, so its analysis scripts read the GPU path's results unchanged:
        FP32 build writes 4-byte values under that header, which misreads).
   BIN  UPrim(5, -1:nx+2, -1:ny+2, -1:nz+2) as a little-endian real(8) stream.
 
+With self-gravity (the -DGRAV build) the outputs also carry the potential
+PHIT = PHIP + PHI: the VTK gets a "SCALARS Potential double 1" block between
+Pressure and Velocity, and PHIT(0:nx+1, 0:ny+1, 0:nz+1) goes to its own
+little-endian stream, named 'phi' + the BIN file's path
+(HydroCore.f90:284-290, 360-372; ``phi_bin_name``).
+
 ``read_vtk`` parses such a file (the reference's harness uses VTKLegacy.jl
 for this, LoadVTK in tests_analisis.jl).
 
@@ -39,6 +45,12 @@ def bin_name(outputpath: str, nout: int, rank: int = 0) -> str:
     return os.path.join(outputpath, f"points{rank:03d}N{nout:03d}.bin")
 
 
+def phi_bin_name(outputpath: str, nout: int, rank: int = 0) -> str:
+    """'phi' // trim(outputpath) // 'points...bin', as the reference builds it:
+    with outputpath './DATA/' this is 'phi./DATA/points000N001.bin'."""
+    return "phi" + bin_name(outputpath, nout, rank)
+
+
 def _fmt_e(v: float, width: int = 15, digits: int = 7) -> str:
     """Fortran's Ew.d edit descriptor: 0.ddddddd E+xx."""
     if v == 0.0:
@@ -55,8 +67,9 @@ def _fmt_e(v: float, width: int = 15, digits: int = 7) -> str:
     return s.rjust(width)
 
 
-def write_vtk(path: str, p: P.HydroParams, prim: np.ndarray) -> None:
-    """prim: numpy (nz+4, ny+4, nx+4, 5) primitives with ghosts."""
+def write_vtk(path: str, p: P.HydroParams, prim: np.ndarray, phit: np.ndarray = None) -> None:
+    """prim: numpy (nz+4, ny+4, nx+4, 5) primitives with ghosts; phit: the
+    potential (nz+2, ny+2, nx+2) of a self-gravity run, or None."""
     w = prim[2:-2, 2:-2, 2:-2]
     hdr = ["# vtk DataFile Version 2.0", "output from Diable", "BINARY",
            "DATASET STRUCTURED_POINTS",
@@ -71,6 +84,9 @@ def write_vtk(path: str, p: P.HydroParams, prim: np.ndarray) -> None:
         fh.write(np.ascontiguousarray(w[..., 0]).astype(be).tobytes())
         fh.write(b"\nSCALARS Pressure double 1\nLOOKUP_TABLE default\n")
         fh.write(np.ascontiguousarray(w[..., 4]).astype(be).tobytes())
+        if phit is not None:
+            fh.write(b"\nSCALARS Potential double 1\nLOOKUP_TABLE default\n")
+            fh.write(np.ascontiguousarray(phit[1:-1, 1:-1, 1:-1]).astype(be).tobytes())
         fh.write(b"\nVECTORS Velocity double\n")
         fh.write(np.ascontiguousarray(w[..., 1:4]).astype(be).tobytes())
         fh.write(b"\n")
@@ -82,6 +98,14 @@ def write_bin(path: str, prim: np.ndarray) -> None:
 
 def read_bin(path: str, p: P.HydroParams) -> np.ndarray:
     return np.fromfile(path, dtype="<f8").reshape(p.shape + (5,))
+
+
+def write_phi_bin(path: str, phit: np.ndarray) -> None:
+    np.ascontiguousarray(phit).astype("<f8").tofile(path)
+
+
+def read_phi_bin(path: str, p: P.HydroParams) -> np.ndarray:
+    return np.fromfile(path, dtype="<f8").reshape(p.nz + 2, p.ny + 2, p.nx + 2)
 
 
 def read_vtk(path: str) -> Dict[str, object]:
