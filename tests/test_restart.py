@@ -220,3 +220,58 @@ def test_resume_rejects_bad_arguments():
                             (0.1, -1, 0), (0.1, 0, 2)):
             with pytest.raises(TlqError):
                 h.resume(u0, t, it, fill)
+
+
+@pytest.mark.gpu
+@needs_flang
+def test_driver_restart_with_self_gravity_continues_the_run(tmp_path, driver):
+    # the BIN dump plus its PHIT dump ('phi' + the BIN path) restart a
+    # self-gravity run: PHI = PHIT - PHIP comes back with its multipole faces
+    # (refreshed only when mod(it, 10) = 0), and the first solve starts from it
+    from tlaloque_amd.hydro import HydroCore
+    L = P.PC
+    p = P.HydroParams(nx=16, ny=16, nz=16, xl=-L, xr=L, yl=-L, yr=L, zl=-L, zr=L,
+                      bc=(P.BC_OUTFLOW,) * 6, self_gravity=P.SG_MULTIPOLE, eta=0.02)
+    x = P.xcoord(p, np.arange(-1, p.nx + 3))
+    r = np.sqrt(x[None, None, :] ** 2 + x[None, :, None] ** 2 + x[:, None, None] ** 2)
+    rho = np.where(r < 0.4 * L, 1e-20, 1e-22)
+    u0 = problems.cons_from_prims(p, rho, np.zeros(r.shape + (3,)), np.full(r.shape, 1e-10))
+    with HydroCore(p) as h:      # tfin between steps 13 and 14: both runs take 14 steps
+        h.set_state(u0)
+        h.advance(13)
+        t13 = h.time()[1]
+        h.advance(1)
+        tfin = 0.5 * (t13 + h.time()[1])
+    init = tmp_path / "u0.bin"
+    np.ascontiguousarray(u0).astype("<f8").tofile(init)
+    sg = f", self_gravity={P.SG_MULTIPOLE}"
+
+    # one run directory, as a reference run has: the dumps and their PHIT
+    # files ('phi' + the relative BIN path) side by side
+    for sub in ("D", "R", "phi./D", "phi./R"):
+        (tmp_path / sub).mkdir(parents=True)
+
+    def run(out, problem, extra):
+        nml = tmp_path / f"{out}.nml"
+        _namelist(nml, p, problem, ".", tfin=tfin, num_out=2, outfile="bin", extra=extra)
+        nml.write_text(nml.read_text().replace("/\n&run", sg + " /\n&run", 1)
+                       .replace("outputpath='./'", f"outputpath='./{out}/'"))
+        res = subprocess.run([driver, str(nml)], cwd=tmp_path, stdout=subprocess.PIPE,
+                             stderr=subprocess.STDOUT, text=True, timeout=300)
+        assert res.returncode == 0, res.stdout[-2000:]
+        return res.stdout
+
+    run("D", "file", f", initfile='{init}'")
+    t_mid, it_mid, _ = vtk.read_time(str(tmp_path / "D" / "points000N001.bin"))
+    assert 0 < it_mid < 14 and it_mid % 10 != 0
+    log = run("R", "restart", ", restart_file='./D/points000N001.bin'")
+    assert "PHI from" in log
+    a, b = tmp_path / "D" / "points000N002.bin", tmp_path / "R" / "points000N002.bin"
+    assert vtk.read_time(str(a))[1] == vtk.read_time(str(b))[1]
+    wa, wb = vtk.read_bin(str(a), p)[2:-2, 2:-2, 2:-2], vtk.read_bin(str(b), p)[2:-2, 2:-2, 2:-2]
+    for e in range(5):
+        scale = max(np.abs(wa[..., e]).max(), 1e-300)
+        assert np.abs(wa[..., e] - wb[..., e]).max() <= 1e-9 * scale, e
+    pa = vtk.read_phi_bin(str(tmp_path / vtk.phi_bin_name("./D/", 2)), p)
+    pb = vtk.read_phi_bin(str(tmp_path / vtk.phi_bin_name("./R/", 2)), p)
+    assert np.abs(pa - pb).max() <= 1e-10 * np.abs(pa).max()

@@ -33,7 +33,9 @@
 !                   at a fresh start).  U is rebuilt from the primitives;
 !                   t, it and the output number come from the dump's sidecar
 !                   <restart_file>.time unless restart_t, restart_it,
-!                   restart_nout are given.
+!                   restart_nout are given.  With self-gravity, a BIN restart
+!                   also reads PHIT from 'phi' + restart_file when that file
+!                   exists and starts PHI from PHIT - PHIP.
 !                   This is what source/warmstart.f90:1-10 set out to do
 !                   (read a VTK output back), which the reference never
 !                   finished.
@@ -385,22 +387,27 @@ contains
   subroutine make_phit(PHI, PHIT)
     real(8), intent(in) :: PHI(0:nx+1, 0:ny+1, 0:nz+1)
     real(8), intent(out) :: PHIT(0:nx+1, 0:ny+1, 0:nz+1)
-    integer :: i, j, k, s
-    real(8) :: pp, r
+    integer :: i, j, k
     do k = 0, nz + 1
        do j = 0, ny + 1
           do i = 0, nx + 1
-             pp = 0d0
-             do s = 1, nmasses
-                r = sqrt((masses(1, s) - xc(i))**2 + (masses(2, s) - yc(j))**2 + &
-                     (masses(3, s) - zc(k))**2 + softening**2)
-                pp = pp - 6.67430d-8 * masses(4, s) / r
-             end do
-             PHIT(i, j, k) = pp + PHI(i, j, k)
+             PHIT(i, j, k) = phip(i, j, k) + PHI(i, j, k)
           end do
        end do
     end do
   end subroutine make_phit
+
+  real(8) function phip(i, j, k)
+    integer, intent(in) :: i, j, k
+    integer :: s
+    real(8) :: r
+    phip = 0d0
+    do s = 1, nmasses
+       r = sqrt((masses(1, s) - xc(i))**2 + (masses(2, s) - yc(j))**2 + &
+            (masses(3, s) - zc(k))**2 + softening**2)
+       phip = phip - 6.67430d-8 * masses(4, s) / r
+    end do
+  end function phip
 
   ! calc_prims on the host (source/HydroCore.f90:14-21), for dry runs.
   subroutine host_prims(UPrim, U)
@@ -556,7 +563,8 @@ program tlq_hydro
   type(c_ptr) :: ctx
   real(8) :: t, dt, tout, dtout
   integer(c_int) :: it, cit
-  integer :: nout, ntab, u_phi
+  integer :: nout, ntab, u_phi, i, j, k
+  logical :: phi_dump
   integer(c_long_long) :: bad
 
   if (command_argument_count() < 1) then
@@ -567,6 +575,8 @@ program tlq_hydro
   call read_settings(trim(path))
 
   allocate(U(5, -1:nx+2, -1:ny+2, -1:nz+2), UPrim(5, -1:nx+2, -1:ny+2, -1:nz+2))
+  U = 0d0
+  call initflow(U)
   allocate(PHI(0:nx+1, 0:ny+1, 0:nz+1), PHIT(0:nx+1, 0:ny+1, 0:nz+1))
   PHI = 0d0
   if (len_trim(phifile) > 0) then
@@ -575,8 +585,23 @@ program tlq_hydro
      read(u_phi) PHI
      close(u_phi)
   end if
-  U = 0d0
-  call initflow(U)
+  if (trim(problem) == 'restart' .and. self_gravity /= TLQ_SG_OFF .and. .not. restart_fill) then
+     inquire(file='phi'//trim(restart_file), exist=phi_dump)
+     if (phi_dump) then
+        open(newunit=u_phi, file='phi'//trim(restart_file), status='old', access='stream', &
+             form='unformatted', convert='LITTLE_ENDIAN', action='read')
+        read(u_phi) PHIT
+        close(u_phi)
+        do k = 0, nz + 1
+           do j = 0, ny + 1
+              do i = 0, nx + 1
+                 PHI(i, j, k) = PHIT(i, j, k) - phip(i, j, k)
+              end do
+           end do
+        end do
+        write(*, '(a,a)') ' PHI from ', 'phi'//trim(restart_file)
+     end if
+  end if
 
   t = 0d0
   dt = 0d0
