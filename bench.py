@@ -312,11 +312,15 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         t0 = time.perf_counter()
         zps, el = cpu_baseline(256, 20, threads)
+        zps1, el1 = cpu_baseline(128, 4, 1)
         log(f"bench: CPU baseline done in {time.perf_counter() - t0:.1f}s")
         result["cpu_baseline"] = {"value": zps, "unit": "zone-updates/s", "cores": threads,
                                   "kind": "port",
                                   "sample": f"FP64 C oracle (OpenMP, {threads} threads), same "
-                                            f"clumpy workload at 256^3, 20 steps ({el:.1f} s)"}
+                                            f"clumpy workload at 256^3, 20 steps ({el:.1f} s)",
+                                  "single_thread": {"value": zps1, "cores": 1,
+                                                    "sample": f"same oracle, 1 thread, clumpy 128^3, "
+                                                              f"4 steps ({el1:.1f} s)"}}
     print(json.dumps(result), flush=True)
 
 

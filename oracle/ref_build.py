@@ -65,7 +65,8 @@ def _r(x: float) -> str:
     return repr(float(x)) + "_8"
 
 
-def _parameters_module(p, tfin: float, cooling_file: Optional[str], levels: int = 0) -> str:
+def _parameters_module(p, tfin: float, cooling_file: Optional[str], levels: int = 0,
+                       num_out: int = 100000) -> str:
     lb, rb, db, tb, fb, bb = (BC_NAME[int(b)] for b in p.bc)
     lines = [
         "module parameters",
@@ -76,7 +77,7 @@ def _parameters_module(p, tfin: float, cooling_file: Optional[str], levels: int 
         f"  integer, parameter :: ndim = 3, neq = 5, size = {int(levels)}",
         f"  integer, parameter :: nxtot = {p.nx}, nytot = {p.ny}, nztot = {p.nz}",
         "  integer, parameter :: choice = 2",
-        "  integer, parameter :: num_out = 100000",
+        f"  integer, parameter :: num_out = {int(num_out)}",
         "  integer, parameter :: outfile = BIN",
         "  character(*), parameter :: outputpath = './DATA/'",
     ]
@@ -170,14 +171,20 @@ def _user_module(p, gravity: bool = False) -> str:
 
 def build_and_run(name: str, p, u0: np.ndarray, tfin: float, cooling: bool = False,
                   timeout: float = 600.0, gravity: bool = False,
-                  phi0: Optional[np.ndarray] = None) -> Dict[str, object]:
+                  phi0: Optional[np.ndarray] = None, num_out: int = 100000,
+                  fflags: Sequence[str] = ()) -> Dict[str, object]:
     """Compile the reference for problem ``p`` and run it from ``u0`` (Fortran
     layout, C shape (nz+4, ny+4, nx+4, 5)) until t >= tfin.  With ``gravity``,
     ``phi0`` (C shape (nz+2, ny+2, nx+2)) is the initial PHI; the levels are
     ``tlaloque_amd.params.mg_levels(p)``.
 
-    Returns {"prims": [UPrim after 0, 1, ... steps], "log": stdout} and, with
-    gravity, "phit": [PHIT after 1, 2, ... steps]."""
+    ``num_out`` is the reference's output count (an output whenever t passes
+    tfin/num_out: the default writes after every step); ``fflags`` replace
+    the optimisation flags.
+
+    Returns {"prims": [UPrim after 0, 1, ... steps], "log": stdout,
+    "seconds": wall time of the run} and, with gravity, "phit": [PHIT after
+    1, 2, ... steps]."""
     if not available():
         raise RuntimeError("reference sources or flang not available")
     case = os.path.join(OUT, name)
@@ -196,10 +203,11 @@ def build_and_run(name: str, p, u0: np.ndarray, tfin: float, cooling: bool = Fal
         if any(m.mass != 8.0 * MSUN for m in p.masses) or (p.masses and p.softening != 0.0):
             raise ValueError("pointsource_gravity.f90:23,32 fixes 8 Msun per star, no softening")
     with open(os.path.join(src, "parameters.f90"), "w") as fh:
-        fh.write(_parameters_module(p, tfin, cfile, levels))
+        fh.write(_parameters_module(p, tfin, cfile, levels, num_out))
     with open(os.path.join(src, "user.f90"), "w") as fh:
         fh.write(_user_module(p, gravity))
-    flags = FFLAGS + (["-DCOOL"] if cooling else []) + (["-DGRAV"] if gravity else [])
+    base = [f for f in FFLAGS if not f.startswith("-O")] + list(fflags) if fflags else FFLAGS
+    flags = base + (["-DCOOL"] if cooling else []) + (["-DGRAV"] if gravity else [])
     objs: List[str] = []
     order: Sequence[str] = REF_SOURCES_BEFORE_USER + ["user"] + REF_SOURCES_AFTER_USER
     for s in order:
@@ -220,12 +228,15 @@ def build_and_run(name: str, p, u0: np.ndarray, tfin: float, cooling: bool = Fal
         ph = np.zeros((p.nz + 2, p.ny + 2, p.nx + 2)) if phi0 is None else phi0
         np.ascontiguousarray(ph, dtype="<f8").tofile(os.path.join(run, "phi0.bin"))
         os.makedirs(os.path.join(run, "phi.", "DATA"))
+    import time
+    t0 = time.perf_counter()
     res = subprocess.run([exe], cwd=run, check=True, capture_output=True, text=True,
                          timeout=timeout)
+    seconds = time.perf_counter() - t0
     files = sorted(glob.glob(os.path.join(run, "DATA", "points000N*.bin")))
     shape = u0.shape
     prims = [np.fromfile(f, dtype="<f8").reshape(shape) for f in files]
-    out = {"prims": prims, "log": res.stdout}
+    out = {"prims": prims, "log": res.stdout, "seconds": seconds}
     if gravity:
         pf = sorted(glob.glob(os.path.join(run, "phi.", "DATA", "points000N*.bin")))[1:]
         out["phit"] = [np.fromfile(f, dtype="<f8").reshape(p.nz + 2, p.ny + 2, p.nx + 2)
