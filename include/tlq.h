@@ -101,6 +101,7 @@ typedef struct tlq_params {
     int self_gravity;               /* TLQ_SG_*: MultiGrid inside Godunov (HLLC.f90:15-24) */
     int mg_levels;                  /* lvlm (parameters.f90:94); 0: as many as the mesh halves */
     double gconst;                  /* Gconst (parameters.f90:45); GR for the reference's */
+    double mg_tol;                  /* V-cycles while Dif > mg_tol; 0: 10 (sources.f90:140) */
 } tlq_params;
 
 /* spherical_wind_type, source/winds.f90:5-13. */

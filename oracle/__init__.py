@@ -62,7 +62,8 @@ class OrcParams(ctypes.Structure):
 class OrcSG(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_int), ("boundary", ctypes.c_int), ("gconst", ctypes.c_double),
                 ("phi", ctypes.POINTER(ctypes.c_double)), ("work", ctypes.POINTER(ctypes.c_double)),
-                ("it", ctypes.c_int), ("cycles", ctypes.c_int), ("dif", ctypes.c_double)]
+                ("it", ctypes.c_int), ("cycles", ctypes.c_int), ("dif", ctypes.c_double),
+                ("tol", ctypes.c_double)]
 
 
 def build(force: bool = False) -> str:
@@ -192,6 +193,7 @@ class Oracle:
             sg = OrcSG()
             sg.levels, sg.boundary, sg.gconst = lv, int(params.self_gravity), float(params.gconst)
             sg.phi, sg.work, sg.it = _ptr(self.phi), _ptr(self.work), 0
+            sg.tol = float(getattr(params, "mg_tol", 0.0))
             self.sg = sg
 
     @property

@@ -869,7 +869,8 @@ int orc_multigrid(const orc_params *p, orc_sg *sg, const double *dens) {
     double dif = 1.e28;
     int iter = 0;
     const int its = 6;
-    while (dif > 10.0 && iter < 31) {
+    const double tol = sg->tol > 0.0 ? sg->tol : 10.0;   /* sources.f90:140 uses 10 */
+    while (dif > tol && iter < 31) {
         ++iter;
         for (int l = 1; l <= lv; ++l)
             memset(L[l].e, 0, lsize(L[l].nx, L[l].ny, L[l].nz) * sizeof(double));

@@ -89,6 +89,7 @@ typedef struct orc_sg {
     int it;          /* step counter (globals it) */
     int cycles;      /* out: V-cycles of the last MultiGrid */
     double dif;      /* out: its final Dif */
+    double tol;      /* V-cycles while Dif > tol; 0: the reference's 10 */
 } orc_sg;
 size_t orc_sg_work_len(const orc_params *p, int levels);
 int orc_multigrid(const orc_params *p, orc_sg *sg, const double *dens);

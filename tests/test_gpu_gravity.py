@@ -17,7 +17,13 @@ so the comparisons are tight:
   * multipole boundary values (the MPI build) against the oracle's, and
     their refresh when mod(it, 10) = 0 only;
   * an FP32 run with self-gravity and a point mass against the FP64 oracle,
-    L1 of rho and p within 1e-3.
+    L1 of rho and p within 1e-3;
+  * the self-gravity tests of the reference's harness (tests_utilities.jl):
+    the potential-accuracy sphere (where the reference's absolute stopping
+    test ends the solve after one V-cycle, 6% from converged: there the GPU
+    and the oracle agree to that iteration error, and converge to the same
+    potential when solved again), and the Evrard and Truelove collapses
+    against the oracle, FP64 and FP32.
 """
 import numpy as np
 import pytest
@@ -191,3 +197,118 @@ def test_fp32_run_with_gravity_meets_the_l1_bar():
     vr = np.abs(w[..., 1:4] - ref[..., 1:4]).mean() / np.abs(ref[..., 1:4]).mean()
     print("L1 rho", l1r, "p", l1p, "v", vr)
     assert l1r <= 1e-3 and l1p <= 1e-3 and vr <= 1e-3
+
+
+# ---------------------------------------------------------------------------
+# The self-gravity tests of the reference's harness (tests_utilities.jl:
+# 247-278, 548-697), at reduced meshes.
+
+def test_harness_potential_accuracy_sphere():
+    # "Gravitational Potential Accuracy Test (Multigrid)": rho = 1 g/cm^3 in
+    # r < 0.25 cm, cgs G.  PHI is of order 1e-8 there, so the absolute
+    # stopping test Dif <= 10 ends the reference's solve after one V-cycle,
+    # 6% from the converged potential (oracle); one red-black cycle lands at
+    # the same distance by another route.  Repeated solves converge both to
+    # the same discrete solution, within 1.3% of the analytic uniform-sphere
+    # potential (the sphere is a staircase of cells).
+    p, u0, phi0 = problems.grav_sphere(n=32)
+    o = oracle.Oracle(p)
+    o.set_phi(phi0)
+    o.set_cons(u0)
+    one_orc = o.phi.copy()
+    assert o.sg.cycles == 1
+    for _ in range(30):
+        o.multigrid(o.dens)
+    x = P.xcoord(p, np.arange(0, p.nx + 2))
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    r = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+    R = 0.25
+    exact = np.where(r < R, -2 * np.pi * P.GR * (R ** 2 - r ** 2 / 3), -P.GR * (4 / 3) * np.pi * R ** 3 / r)
+    with _hc()(p) as h:
+        h.upload_phi(phi0)
+        h.set_state(u0)
+        one_gpu = h.download_phi()
+        assert h.gravity_info()[1] == 1
+        for _ in range(30):
+            h.gravity_solve()
+        conv_gpu = h.download_phi()
+    inner = (slice(1, -1),) * 3
+    s = np.abs(exact).max()
+    print("one cycle: gpu vs oracle", np.abs(one_gpu - one_orc).max() / s,
+          "gpu vs exact", np.abs(one_gpu - exact)[inner].max() / s,
+          "converged: gpu vs oracle", np.abs(conv_gpu - o.phi).max() / s,
+          "vs exact", np.abs(conv_gpu - exact)[inner].max() / s)
+    assert np.abs(one_gpu - exact)[inner].max() <= 0.10 * s
+    assert np.abs(one_gpu - one_orc).max() <= 0.08 * s
+    assert np.abs(conv_gpu - o.phi).max() <= 1e-10 * s
+    assert np.abs(conv_gpu - exact)[inner].max() <= 0.02 * s
+
+
+@pytest.mark.parametrize("name,make,steps,cfl,tol,bound64", [
+    ("truelove", problems.truelove, 10, None, 0.0, 1e-6),
+    ("evrard", problems.evrard, 20, 0.4, 0.0, 3e-3),
+    ("evrard", problems.evrard, 20, 0.4, 1e-12, 1e-7)])
+@pytest.mark.parametrize("prec", [P.PREC_FP64, P.PREC_FP32])
+def test_harness_collapse_matches_oracle(name, make, steps, cfl, tol, bound64, prec):
+    # "Evrard Collapse" (G = 1, reflective) and "Truelove Collapse Test" (cgs,
+    # gamma = 1.001) at 32^3 with multipole faces: GPU against the FP64 oracle
+    # after the same number of steps.
+    # * Truelove (cgs): the reference's stopping test (Dif <= 10) means a
+    #   converged potential, and the L1 errors are below 1e-6.
+    # * Evrard (G = 1): PHI is of order 2, and Dif <= 10 summed over 32^3
+    #   cells stops each solve short of converged, the reference's and the
+    #   red-black one at different points; over 20 steps rho and p differ by
+    #   L1 ~1e-3.  Dif (level 0's last sweep) also misses smooth error: one
+    #   cycle from the last step's potential ends with Dif ~1e-11 while
+    #   ~1e-5 away.  With mg_tol = 1e-12 every solve runs its 31 cycles and
+    #   reaches the discrete solution (residual ~1e-14).  The collapse at
+    #   32^3 is itself sensitive: the oracle started from U0 (1 + 1e-14 noise)
+    #   ends 20 steps later ~2e-5 (L1 rho) from its unperturbed run, because
+    #   limiter and wave-region branches flip.  The converged GPU run must lie
+    #   within 3x that measured spread of the oracle's.
+    # Evrard runs at CFL 0.4: the harness's 0.8 (set for 128^3) lets the
+    # first 32^3 step (dt = 0.245; the CFL ignores gravity) drive densities
+    # negative, in the oracle as in the reference's scheme.
+    p, u0, phi0 = make(n=32, precision=prec)
+    p = p.replace(mg_tol=tol)
+    if cfl:
+        p = p.replace(cfl=cfl)
+
+    def oracle_run(u):
+        o = oracle.Oracle(p.replace(precision=P.PREC_FP64))
+        o.set_phi(phi0)
+        o.set_cons(u)
+        for _ in range(steps):
+            o.advance()
+        return o
+
+    o = oracle_run(u0)
+    ref = o.prim[2:-2, 2:-2, 2:-2]
+    if tol > 0.0:   # the problem's own sensitivity to rounding
+        rng = np.random.default_rng(0)
+        q = oracle_run(u0 * (1.0 + 1e-14 * rng.standard_normal(u0.shape)))
+        w2 = q.prim[2:-2, 2:-2, 2:-2]
+        spread = max(l1_rel(w2[..., 0], ref[..., 0]), l1_rel(w2[..., 4], ref[..., 4]),
+                     np.abs(q.phi - o.phi).max() / np.abs(o.phi).max())
+        bound64 = max(bound64, 3.0 * spread)
+        print(name, "oracle spread under 1e-14 noise", spread)
+    with _hc()(p) as h:
+        h.upload_phi(phi0)
+        h.set_state(u0)
+        h.advance(steps)
+        _, t, it = h.time()
+        h.calc_prims("U")
+        w = h.download("UPrim")[2:-2, 2:-2, 2:-2]
+        phi = h.download_phi()
+        assert h.check_state() == 0
+    l1r, l1p = l1_rel(w[..., 0], ref[..., 0]), l1_rel(w[..., 4], ref[..., 4])
+    ep = np.abs(phi - o.phi).max() / np.abs(o.phi).max()
+    print(name, prec, "L1 rho", l1r, "p", l1p, "max rel", max_rel(w, ref), "phi", ep,
+          "t", t, o.t)
+    bound = bound64 if prec == P.PREC_FP64 else max(bound64, 1e-4)
+    assert it == steps and abs(t - o.t) <= max(bound, 1e-7) * o.t
+    assert l1r <= bound and l1p <= bound and ep <= bound
+    # the collapse has started: the central cells are denser than at t = 0
+    c = p.nx // 2
+    core = (slice(c - 1, c + 1),) * 3
+    assert w[core + (0,)].mean() > u0[2:-2, 2:-2, 2:-2, 0][core].mean()

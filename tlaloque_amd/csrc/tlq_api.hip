@@ -449,6 +449,7 @@ int tlq_create(const tlq_params *p, tlq_ctx **out) {
         if (sg_coarse_cells(p->nx, p->ny, p->nz, sg_levels) > SG_COARSE_MAX)
             return fail(TLQ_ERR_UNSUPPORTED, "coarsest multigrid level above 4096 cells");
         if (!(p->gconst >= 0.0)) return fail(TLQ_ERR_ARG, "bad gconst");
+        if (!(p->mg_tol >= 0.0)) return fail(TLQ_ERR_ARG, "bad mg_tol");
     }
 
     tlq_ctx *c = new tlq_ctx();
@@ -526,6 +527,7 @@ int tlq_create(const tlq_params *p, tlq_ctx **out) {
     if (p->self_gravity != TLQ_SG_OFF) {
         c->sg.boundary = p->self_gravity;
         c->sg.gconst = p->gconst;
+        c->sg.tol = p->mg_tol > 0.0 ? p->mg_tol : 10.0;
         if (sg_alloc(c->sg, p->nx, p->ny, p->nz, sg_levels, p->xr - p->xl, c->stream) != hipSuccess)
             return bail(TLQ_ERR_NOMEM, "hipMalloc failed (multigrid levels)");
         c->sg_on = true;

@@ -690,7 +690,7 @@ hipError_t sg_solve(hipStream_t s, const Grid &g, const R *U, SgState &S, int it
     copy_shell(s, L0);
     double dif = 1.e28;
     int iter = 0;
-    while (dif > 10.0 && iter < 31) {   // sources.f90:140
+    while (dif > S.tol && iter < 31) {   // sources.f90:140 (tol 10)
         ++iter;
         for (int l = 1; l <= LV; ++l) {
             const long long n = (long long)(S.lv[l].nx + 2) * (S.lv[l].ny + 2) * (S.lv[l].nz + 2);

@@ -42,6 +42,7 @@ struct SgState {
     int levels = 0;
     int boundary = 0;       // 1: faces kept as set; 2: multipole every 10 steps
     double gconst = 0.0;
+    double tol = 10.0;           // V-cycles while Dif > tol (sources.f90:140 uses 10)
     SgLevel lv[SG_MAX_LEVELS + 1]{};
     double *partial = nullptr;   // per-block partial sums
     long long npartial = 0;

@@ -49,6 +49,7 @@ module tlq_c
      integer(c_int) :: self_gravity   ! TLQ_SG_*
      integer(c_int) :: mg_levels      ! lvlm; 0: as many as the mesh halves
      real(c_double) :: gconst         ! Gconst
+     real(c_double) :: mg_tol         ! V-cycles while Dif > mg_tol; 0: the reference's 10
   end type tlq_params_t
 
   ! struct tlq_wind: spherical_wind_type, source/winds.f90:5-13

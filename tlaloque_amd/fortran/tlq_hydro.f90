@@ -10,7 +10,7 @@
 !   &mesh     nx, ny, nz, xl, xr, yl, yr, zl, zr
 !   &physics  gamma, mu0, cfl, eta, cooling_limit, limiter, solver, precision,
 !             cooling, bc(6), softening, tfin, cooling_file, self_gravity,
-!             mg_levels, gconst, phifile (self-gravity: PHI starts from
+!             mg_levels, gconst, mg_tol, phifile (self-gravity: PHI starts from
 !             phifile, a little-endian real(8) stream of PHI(0:nx+1,0:ny+1,
 !             0:nz+1) whose ghost faces are the boundary values, else from 0;
 !             TLQ_SG_MULTIPOLE sets the faces itself)
@@ -68,7 +68,7 @@ module driver_setup
   integer :: limiter = TLQ_LIM_VANLEER, solver = TLQ_SOLVER_HLLC, precision = TLQ_FP64
   integer :: cooling = TLQ_COOL_OFF
   integer :: self_gravity = TLQ_SG_OFF, mg_levels = 0
-  real(8) :: gconst = 6.67430d-8
+  real(8) :: gconst = 6.67430d-8, mg_tol = 0d0
   integer :: bc(6) = TLQ_BC_REFLECTIVE
   character(len=512) :: cooling_file = '', phifile = ''
   ! &run
@@ -85,7 +85,7 @@ module driver_setup
 
   namelist /mesh/ nx, ny, nz, xl, xr, yl, yr, zl, zr
   namelist /physics/ gamma, mu0, cfl, eta, cooling_limit, limiter, solver, precision, cooling, &
-       bc, softening, tfin, cooling_file, self_gravity, mg_levels, gconst, phifile
+       bc, softening, tfin, cooling_file, self_gravity, mg_levels, gconst, phifile, mg_tol
   namelist /run/ problem, num_out, outfile, outputpath, max_steps, per_routine, initfile, &
        dry_run, nwinds, winds, nmasses, masses, restart_file, restart_t, restart_it, restart_nout
 
@@ -149,6 +149,7 @@ contains
     p%self_gravity = self_gravity
     p%mg_levels = mg_levels
     p%gconst = gconst
+    p%mg_tol = mg_tol
   end function make_params
 
   ! cell centres, as xcoord/ycoord/zcoord (source/HydroCore.f90:35-75, one rank)

@@ -68,6 +68,7 @@ class TlqParams(ctypes.Structure):
         ("limiter", ctypes.c_int), ("bc", ctypes.c_int * 6), ("solver", ctypes.c_int),
         ("precision", ctypes.c_int), ("cooling", ctypes.c_int), ("softening", ctypes.c_double),
         ("self_gravity", ctypes.c_int), ("mg_levels", ctypes.c_int), ("gconst", ctypes.c_double),
+        ("mg_tol", ctypes.c_double),
     ]
 
 
@@ -165,7 +166,7 @@ def to_c_params(p: P.HydroParams) -> TlqParams:
               "mg_levels"):
         setattr(c, n, int(getattr(p, n)))
     for n in ("xl", "xr", "yl", "yr", "zl", "zr", "gamma", "mu0", "cfl", "eta", "cooling_limit",
-              "softening", "gconst"):
+              "softening", "gconst", "mg_tol"):
         setattr(c, n, float(getattr(p, n)))
     for q in range(6):
         c.bc[q] = int(p.bc[q])

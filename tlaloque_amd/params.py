@@ -95,6 +95,7 @@ class HydroParams:
     self_gravity: int = SG_OFF
     mg_levels: int = 0          # lvlm (user/parameters.f90:94); 0: as many as the mesh halves
     gconst: float = GR          # Gconst (user/parameters.f90:45)
+    mg_tol: float = 0.0         # V-cycles while Dif > mg_tol; 0: the reference's 10 (sources.f90:140)
 
     @property
     def dx(self) -> float:

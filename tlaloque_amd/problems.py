@@ -217,3 +217,72 @@ def cluster(n=1024, precision=P.PREC_FP32, seed=7, box_pc=4.0, cooling=P.COOL_SU
                       precision=precision, cooling=cooling, winds=winds, masses=masses,
                       softening=2.0 * dx)
     return p, uniform_state(p, 10.0 * p.mu0 * P.AMU, 10.0 * P.KB * 100.0)
+
+
+# ---------------------------------------------------------------------------
+# The self-gravity problems of the reference's harness (tests_utilities.jl),
+# at a chosen resolution.  Each returns (params, U, PHI0); PHI0 has shape
+# (nz+2, ny+2, nx+2).
+
+def _centres3(p: P.HydroParams, lo: int = -1, pad: int = 3):
+    x = P.xcoord(p, np.arange(lo, p.nx + pad))
+    y = P.ycoord(p, np.arange(lo, p.ny + pad))
+    z = P.zcoord(p, np.arange(lo, p.nz + pad))
+    return np.meshgrid(z, y, x, indexing="ij")[::-1]
+
+
+def grav_sphere(n=150, precision=P.PREC_FP64, mode=P.SG_FIXED):
+    """"Gravitational Potential Accuracy Test (Multigrid)" (tests_utilities.jl:
+    247-256, 548-596): rho = 1 inside r < 0.25 of the unit box, 0 outside,
+    p = 1.5e-5 (gamma-1), outflow, cgs G.  PHI0: zero inside, the exact
+    potential of the mesh's mass, -G M / r, on the ghost layer (the harness
+    starts PHI from pointmass_potential, which with no stars leaves it 0)."""
+    p = P.HydroParams(nx=n, ny=n, nz=n, xl=-0.5, xr=0.5, yl=-0.5, yr=0.5, zl=-0.5, zr=0.5,
+                      gamma=5.0 / 3.0, cfl=0.8, eta=0.5e-2, bc=(P.BC_OUTFLOW,) * 6,
+                      precision=precision, self_gravity=mode)
+    X, Y, Z = _centres3(p)
+    r = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+    rho = np.where(r > 0.25, 0.0, 1.0)
+    pres = np.full(r.shape, 1.5e-5 * (p.gamma - 1.0))
+    u = cons_from_prims(p, rho, np.zeros(r.shape + (3,)), pres)
+    mass = rho[2:-2, 2:-2, 2:-2].sum() * p.dx * p.dy * p.dz
+    phi0 = -P.GR * mass / r[1:-1, 1:-1, 1:-1]
+    phi0[1:-1, 1:-1, 1:-1] = 0.0
+    return p, u, phi0
+
+
+def evrard(n=128, precision=P.PREC_FP64, mode=P.SG_MULTIPOLE):
+    """"Evrard Collapse" (tests_utilities.jl:258-267, 597-646): G = 1,
+    M = R = 1, rho = M / (2 pi R^2 r) inside R with specific internal
+    energy 0.05 (U(5) = 0.05 rho), 1e-4 outside, at rest, reflective box
+    [-2, 2]^3, PHI0 = 0.  (The harness's serial build would keep PHI's faces
+    at 0; ``mode`` defaults to the MPI build's multipole faces.)"""
+    p = P.HydroParams(nx=n, ny=n, nz=n, xl=-2.0, xr=2.0, yl=-2.0, yr=2.0, zl=-2.0, zr=2.0,
+                      gamma=5.0 / 3.0, cfl=0.8, eta=0.5e-2, bc=(P.BC_REFLECTIVE,) * 6,
+                      precision=precision, self_gravity=mode, gconst=1.0)
+    X, Y, Z = _centres3(p)
+    r = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+    inside = r <= 1.0
+    u = np.zeros(p.shape + (5,))
+    u[..., 0] = np.where(inside, 1.0 / (2.0 * np.pi * r), 1e-4)
+    u[..., 4] = np.where(inside, 0.05 / (2.0 * np.pi * r), 1e-4 * 0.05)
+    return p, u, np.zeros((p.nz + 2, p.ny + 2, p.nx + 2))
+
+
+def truelove(n=128, precision=P.PREC_FP64, mode=P.SG_MULTIPOLE):
+    """"Truelove Collapse Test" (tests_utilities.jl:269-278, 648-697): a
+    sphere of rho = 1e-15 g/cm^3 and radius 7.8e15 cm in gas 100x thinner at
+    the same pressure rho_out cs^2, cs = 0.167 km/s, gamma = 1.001, outflow
+    box of +-535 AU, cgs G, PHI0 = 0."""
+    L = 535.0 * P.AU
+    p = P.HydroParams(nx=n, ny=n, nz=n, xl=-L, xr=L, yl=-L, yr=L, zl=-L, zr=L, gamma=1.001,
+                      cfl=0.8, eta=0.5e-2, bc=(P.BC_OUTFLOW,) * 6, precision=precision,
+                      self_gravity=mode)
+    X, Y, Z = _centres3(p)
+    r = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+    rho_out, cs = 0.01 * 1.0e-15, 0.167 * P.KMS
+    pres = rho_out * cs ** 2
+    u = np.zeros(p.shape + (5,))
+    u[..., 0] = np.where(r > 7.8e15, rho_out, 1.0e-15)
+    u[..., 4] = pres / (p.gamma - 1.0)
+    return p, u, np.zeros((p.nz + 2, p.ny + 2, p.nx + 2))
