@@ -43,6 +43,18 @@ BYTES_STEP = BYTES_STAGE1 + BYTES_STAGE2
 
 
 TRAFFIC_REPORT = os.path.join(ROOT, "profiles", "traffic_clumpy512.md")
+SG_TRAFFIC_REPORT = os.path.join(ROOT, "profiles", "traffic_selfgravity512.md")
+BYTES_RELAX0 = 20           # per cell and red-black iteration: read PHI and rho, write PHI
+
+
+def sg_traffic():
+    """Measured HBM bytes of one finest-level relaxation launch (calibrated
+    rocprof counters), or None."""
+    if not os.path.exists(SG_TRAFFIC_REPORT):
+        return None
+    from tlaloque_amd.profiling import read_traffic
+    k = read_traffic(SG_TRAFFIC_REPORT)["kernels"].get("k_sg_rbz")
+    return k["bytes"] if k else None
 COPY_PROBE_BYTES = 2 << 30
 
 
@@ -156,11 +168,19 @@ def self_gravity_leg(p, u0, table, warmup, steps):
         h.kernel_timing(False)
         bad = h.check_state()
     mg_ms = ks["multigrid"][0] / max(ks["multigrid"][1], 1)
+    r0_ms = ks["mg_relax0"][0] / max(ks["mg_relax0"][1], 1)
+    # one red-black iteration on the finest level: read PHI (8 B) and the FP32
+    # density (4 B), write PHI (8 B) per cell
+    ach = BYTES_RELAX0 * q.zones / (r0_ms / 1e3) / 1e9
     out = {"zone_updates_per_s": q.zones * steps / (ms / 1e3), "ms_per_step": ms / steps,
            "multigrid_ms_per_step": mg_ms, "levels": lv,
            "vcycles_per_step": float(np.mean(cycles)), "vcycles_startup": c0,
            "ms_per_vcycle": mg_ms / max(float(np.mean(cycles)), 1e-9),
-           "boundary": "multipole every 10 steps", "nonfinite_cells": bad}
+           "boundary": "multipole every 10 steps", "nonfinite_cells": bad,
+           "roofline": {"bound": "hbm", "kernel": "k_sg_rbz (finest level)", "achieved": ach,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                        "bytes_per_zone": BYTES_RELAX0, "avg_launch_ms": r0_ms,
+                        "launches": ks["mg_relax0"][1], "traffic": sg_traffic()}}
     log(f"bench: self-gravity: {out}")
     return out
 

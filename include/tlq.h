@@ -222,7 +222,8 @@ int tlq_surrogate_grad(tlq_ctx *ctx, const float *x, const float *y, int n, doub
 /* Timing: HIP events on the context's stream.  tlq_timer_* brackets any
  * enqueued work; with kernel timing on, every kernel launch of the fused step
  * is bracketed and its durations accumulate per kernel (names as in
- * tlq_kernel_stats: "stage1", "stage2", "cooling", ...). */
+ * tlq_kernel_stats: "stage1", "stage2", "init", "cooling", "multigrid" (whole
+ * self-gravity solves), "mg_relax0" (the finest level's relaxation launches)). */
 int tlq_timer_start(tlq_ctx *ctx);
 int tlq_timer_stop(tlq_ctx *ctx, double *ms);
 int tlq_kernel_timing(tlq_ctx *ctx, int enable);

@@ -41,9 +41,10 @@ struct PendingTiming {
     hipEvent_t a, b;
 };
 
-// "multigrid" times a whole self-gravity solve (many launches).
-const char *const kKernelNames[] = {"stage1", "stage2", "init", "cooling", "multigrid"};
-constexpr int kNumKernels = 5;
+// "multigrid" times a whole self-gravity solve (many launches); "mg_relax0"
+// the one-pass relaxation launches on the finest level.
+const char *const kKernelNames[] = {"stage1", "stage2", "init", "cooling", "multigrid", "mg_relax0"};
+constexpr int kNumKernels = 6;
 
 }  // namespace
 
@@ -214,7 +215,14 @@ template <class R> int do_borders(tlq_ctx *c, void *buf, int order) {
 template <class R> int do_sg_solve(tlq_ctx *c) {
     if (!c->sg_on) return TLQ_OK;
     KTimer kt(c, 4);
+    c->sg.time_relax0 = c->ktiming;
     HIPCHK(sg_solve<R>(c->stream, c->g, cp<R>(c->u[c->cur]), c->sg, c->it_host, c->dprob));
+    if (c->ktiming) {
+        c->ktotal[5] += c->sg.relax0_ms;
+        c->kcount[5] += c->sg.relax0_n;
+    }
+    c->sg.relax0_ms = 0.0;
+    c->sg.relax0_n = 0;
     return TLQ_OK;
 }
 

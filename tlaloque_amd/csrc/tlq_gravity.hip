@@ -631,10 +631,20 @@ void relax(hipStream_t s, const Grid &g, const R *U, double fpg, SgState &S, int
         const dim3 gd = rbz_grid(L);
         for (int it = 1; it <= 5; ++it) {
             double *part = (want_dif && it == 5) ? S.partial : nullptr;
-            if (l == 0)
+            if (l == 0) {
+                hipEvent_t a = nullptr, b = nullptr;
+                if (S.time_relax0) {
+                    hipEventCreate(&a);
+                    hipEventCreate(&b);
+                    hipEventRecord(a, s);
+                }
                 hipLaunchKernelGGL((k_sg_rbz<R, true>), gd, dim3(ZX * ZY), 0, s, L.e, L.e2, nullptr,
                                    U, g, fpg, L.nx, L.ny, L.nz, L.dxl, part);
-            else
+                if (S.time_relax0) {
+                    hipEventRecord(b, s);
+                    S.relax0_ev.emplace_back(a, b);
+                }
+            } else
                 hipLaunchKernelGGL((k_sg_rbz<R, false>), gd, dim3(ZX * ZY), 0, s, L.e, L.e2, L.r, U,
                                    g, fpg, L.nx, L.ny, L.nz, L.dxl, part);
             std::swap(L.e, L.e2);
@@ -726,6 +736,15 @@ hipError_t sg_solve(hipStream_t s, const Grid &g, const R *U, SgState &S, int it
         if (e != hipSuccess) return e;
         e = hipStreamSynchronize(s);
         if (e != hipSuccess) return e;
+        for (auto &ev : S.relax0_ev) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, ev.first, ev.second);
+            S.relax0_ms += ms;
+            ++S.relax0_n;
+            hipEventDestroy(ev.first);
+            hipEventDestroy(ev.second);
+        }
+        S.relax0_ev.clear();
     }
     if (L0.e != S.phi0) {   // an odd number of one-pass iterations: PHI is in the second buffer
         const size_t n = (size_t)(L0.nx + 2) * (L0.ny + 2) * (L0.nz + 2);

@@ -21,6 +21,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <utility>
+#include <vector>
+
 #include "tlq_device.hpp"
 
 namespace tlq {
@@ -51,6 +54,12 @@ struct SgState {
     double *scal = nullptr;      // [0] Dif, [1..25] moments
     int cycles = 0;              // V-cycles of the last solve
     double dif = 0.0;            // its final Dif
+    // Timing of level 0's relaxation launches (HIP events on the solve's
+    // stream) when time_relax0 is set: summed into relax0_ms / relax0_n.
+    bool time_relax0 = false;
+    double relax0_ms = 0.0;
+    long long relax0_n = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> relax0_ev;
 };
 
 // Most halvings that keep nx, ny, nz whole.

@@ -351,7 +351,7 @@ class HydroCore:
 
     def kernel_stats(self) -> dict:
         out = {}
-        for q in range(5):
+        for q in range(6):
             name = ctypes.create_string_buffer(32)
             ms, n = ctypes.c_double(), ctypes.c_longlong()
             _check(_lib.tlq_kernel_stats(self._ctx, q, name, 32, ctypes.byref(ms), ctypes.byref(n)))
