@@ -169,16 +169,16 @@ template <class R> __device__ __forceinline__ R lim_avg(int opt, R a, R b) {
             if (a * b <= 0.0) return 0.0;
             return a * b * (a + b) / (a * a + b * b);
         } else {
-            // a*b (a+b)/(a^2+b^2) evaluated on a/m, b/m with m = max(|a|,|b|):
-            // no a*b product to underflow for cgs density differences (~1e-25),
-            // no overflow, and no reciprocal of a subnormal (0 is returned when
-            // m is subnormal, where the product form underflows to 0 as well).
+            // a*b (a+b)/(a^2+b^2) evaluated on a 2^-x, b 2^-x with 2^x the
+            // binade of m = max(|a|,|b|) (exact scaling, |a|, |b| 2^-x < 1):
+            // no a*b product to underflow for cgs density differences (~1e-25)
+            // and no overflow.
             const R m = fmaxf(__builtin_fabsf(a), __builtin_fabsf(b));
-            const R im = __builtin_amdgcn_rcpf(m);
-            const R as = a * im, bs = b * im;
+            const int x = __builtin_amdgcn_frexp_expf(m);
+            const R as = __builtin_ldexpf(a, -x), bs = __builtin_ldexpf(b, -x);
             const R ab = as * bs;
             const R r = ab * (a + b) * __builtin_amdgcn_rcpf(__builtin_fmaf(as, as, bs * bs));
-            return (ab > 0.f && m >= 1.17549435e-38f) ? r : 0.f;
+            return ab > 0.f ? r : 0.f;
         }
     case LIM_MINMOD:
         s = __builtin_copysign(R(1), a);

@@ -108,19 +108,62 @@ __device__ __forceinline__ void prims(const Problem &P, const R u[5], R w[5]) {
     else cons2prim_fast(P.gamma - 1.0, u, w);
 }
 
-// Conserved state of cell (i, j, k), or a uniform dummy outside the array
-// (only tiles that overhang the mesh read such cells, for faces they discard).
+// Addressing of the z march.  A thread computes the byte offset of its
+// column (i, j) inside a plane once; every load and store of the march then
+// adds it to a block-uniform plane address (the scalar-base form of the
+// global memory instructions), and the range check is a lane mask fixed for
+// the whole march.  Cell (i, j, k) of field e sits at byte
+//   e*fbytes + (k+1)*pbytes + off(i, j),  off(i, j) = idx(i, j, -1)*sizeof(R),
+// which is below pbytes for -1 <= i <= nx+2, -1 <= j <= ny+2.
+struct Column {
+    unsigned off;  // byte offset of (i, j) inside its plane (0 outside the array)
+    bool in;       // -1 <= i <= nx+2 and -1 <= j <= ny+2
+};
+
+struct Planes {
+    long long fbytes;  // bytes per field
+    long long pbytes;  // bytes per z plane
+};
+
 template <class R>
-__device__ __forceinline__ void load_raw(const Problem &P, const R *__restrict__ A, int i, int j,
-                                         int k, R u[5]) {
-    const Grid &g = P.g;
-    if (i >= -1 && i <= g.nx + 2 && j >= -1 && j <= g.ny + 2 && k >= -1 && k <= g.nz + 2) {
-        load5(A, g.nc, g.idx(i, j, k), u);
-    } else {
-        u[0] = R(1);
-        u[1] = u[2] = u[3] = R(0);
-        u[4] = R(1);
-    }
+__device__ __forceinline__ Planes planes_of(const Grid &g) {
+    return Planes{g.nc * (long long)sizeof(R), g.sxy * (long long)sizeof(R)};
+}
+
+template <class R>
+__device__ __forceinline__ Column column_of(const Grid &g, int i, int j) {
+    Column c;
+    c.in = i >= -1 && i <= g.nx + 2 && j >= -1 && j <= g.ny + 2;
+    c.off = c.in ? (unsigned)((g.base - g.sxy + i + (long long)g.sx * j) * (long long)sizeof(R)) : 0u;
+    return c;
+}
+
+// Address of field e, plane k (block-uniform).
+template <class T>
+__device__ __forceinline__ T *plane_ptr(T *A, const Planes &pl, int e, int k) {
+    return A + (pl.pbytes * (k + 1) + e * pl.fbytes);
+}
+
+// Conserved state of cell (i, j, k) of column c.  A column outside the array
+// (only tiles that overhang the mesh have them) reads the first element of
+// the plane instead: in bounds, and it only feeds faces of cells that are not
+// written.  The z marches only read planes -1 <= k <= nz+2.
+template <class R>
+__device__ __forceinline__ void load_col(const R *__restrict__ A, const Planes &pl, Column c, int k,
+                                         R u[5]) {
+#pragma unroll
+    for (int e = 0; e < 5; ++e)
+        u[e] = *reinterpret_cast<const R *>(plane_ptr(reinterpret_cast<const char *>(A), pl, e, k) +
+                                            c.off);
+}
+
+// Stores the five fields of cell (i, j, k) of column c (which lies inside the array).
+template <class R>
+__device__ __forceinline__ void store_col(R *__restrict__ A, const Planes &pl, Column c, int k,
+                                          const R u[5]) {
+#pragma unroll
+    for (int e = 0; e < 5; ++e)
+        *reinterpret_cast<R *>(plane_ptr(reinterpret_cast<char *>(A), pl, e, k) + c.off) = u[e];
 }
 
 // Slope average with the limiter fixed at compile time (LIM > 0) or read
@@ -249,6 +292,8 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
                                                     StepState *__restrict__ st, int parity) {
     constexpr int NT = TX * TY;
     static_assert(2 * (TX + TY) <= NT, "one halo cell per thread");
+    constexpr int XW = (TX + 63) / 64 * 64;  // first thread of the extra x faces
+    static_assert(XW + TY <= NT, "extra x faces fit the block");
     constexpr int LX = TX + 2, LY = TY + 2;
     __shared__ R sW[5][LY][LX];
     __shared__ R sU[5][LY][LX];
@@ -266,6 +311,9 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
     const int i = x0 + tx, j = y0 + ty;
     const bool active = i <= g.nx && j <= g.ny;
     const HaloTask ht = halo_task1<TX, TY>(tid);
+    const Planes pl = planes_of<R>(g);
+    const Column cc = column_of<R>(g, i, j);
+    const Column ch = column_of<R>(g, x0 + ht.lx, y0 + ht.ly);
 
     const double dt = dt_from_maxima(P, st->maxima[parity]);
     const double dth = dt / 2.0;
@@ -280,14 +328,14 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
     R uc[5], wc[5], hm[5], pf_n[5], pf_h[5];
     {
         R up[5], wp[5];
-        load_raw(P, U, i, j, z0 - 1, up);
+        load_col(U, pl, cc, z0 - 1, up);
         prims(P, up, wp);
-        load_raw(P, U, i, j, z0, uc);
+        load_col(U, pl, cc, z0, uc);
         prims(P, uc, wc);
         riemann_t<2, SOL>(gam, gr, wp, wc, up, uc, hm);
     }
-    load_raw(P, U, i, j, z0 + 1, pf_n);
-    if (ht.kind) load_raw(P, U, x0 + ht.lx, y0 + ht.ly, z0, pf_h);
+    load_col(U, pl, cc, z0 + 1, pf_n);
+    if (ht.kind) load_col(U, pl, ch, z0, pf_h);
 
     for (int k = z0; k <= z1; ++k) {
         R un[5], wn[5], hr[5], hp[5];
@@ -296,9 +344,12 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
             un[e] = pf_n[e];
             hr[e] = pf_h[e];
         }
-        if (k < z1) {  // prefetch plane k+1
-            load_raw(P, U, i, j, k + 2, pf_n);
-            if (ht.kind) load_raw(P, U, x0 + ht.lx, y0 + ht.ly, k + 1, pf_h);
+        // The plane index is block-uniform; readfirstlane lets the compiler
+        // form the plane addresses in scalar registers.
+        const int ku = __builtin_amdgcn_readfirstlane(k);
+        if (k < z1) {  // prefetch plane k+1 (planes z0-1 .. z1+1 <= nz+1)
+            load_col(U, pl, cc, ku + 2, pf_n);
+            if (ht.kind) load_col(U, pl, ch, ku + 1, pf_h);
         }
         prims(P, un, wn);
         riemann_t<2, SOL>(gam, gr, wc, wn, uc, un, hp);
@@ -318,35 +369,40 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
             }
         }
         lds_barrier();
-        for (int q = tid; q < (TX + 1) * TY + TX * (TY + 1); q += NT) {
+        // Faces: the left x face and the lower y face of every tile cell,
+        // then the upper y faces of the last row (first wave) and the right
+        // x faces of the last column (next wave), so that no wave runs both
+        // directions in one pass.
+        auto xface = [&](int fx, int fy) {  // between tile cells fx-1 and fx
             R wl[5], wr[5], ul[5], ur[5], f[5];
-            if (q < (TX + 1) * TY) {  // x face left of tile cell fx in row fy
-                const int fy = q / (TX + 1), fx = q % (TX + 1);
 #pragma unroll
-                for (int e = 0; e < 5; ++e) {
-                    wl[e] = sW[e][fy + 1][fx];
-                    wr[e] = sW[e][fy + 1][fx + 1];
-                    ul[e] = sU[e][fy + 1][fx];
-                    ur[e] = sU[e][fy + 1][fx + 1];
-                }
-                riemann_t<0, SOL>(gam, gr, wl, wr, ul, ur, f);
-#pragma unroll
-                for (int e = 0; e < 5; ++e) sF[e][fy][fx] = f[e];
-            } else {  // y face below tile row fy in column fx
-                const int r = q - (TX + 1) * TY;
-                const int fy = r / TX, fx = r % TX;
-#pragma unroll
-                for (int e = 0; e < 5; ++e) {
-                    wl[e] = sW[e][fy][fx + 1];
-                    wr[e] = sW[e][fy + 1][fx + 1];
-                    ul[e] = sU[e][fy][fx + 1];
-                    ur[e] = sU[e][fy + 1][fx + 1];
-                }
-                riemann_t<1, SOL>(gam, gr, wl, wr, ul, ur, f);
-#pragma unroll
-                for (int e = 0; e < 5; ++e) sG[e][fy][fx] = f[e];
+            for (int e = 0; e < 5; ++e) {
+                wl[e] = sW[e][fy + 1][fx];
+                wr[e] = sW[e][fy + 1][fx + 1];
+                ul[e] = sU[e][fy + 1][fx];
+                ur[e] = sU[e][fy + 1][fx + 1];
             }
-        }
+            riemann_t<0, SOL>(gam, gr, wl, wr, ul, ur, f);
+#pragma unroll
+            for (int e = 0; e < 5; ++e) sF[e][fy][fx] = f[e];
+        };
+        auto yface = [&](int fx, int fy) {  // between tile rows fy-1 and fy
+            R wl[5], wr[5], ul[5], ur[5], f[5];
+#pragma unroll
+            for (int e = 0; e < 5; ++e) {
+                wl[e] = sW[e][fy][fx + 1];
+                wr[e] = sW[e][fy + 1][fx + 1];
+                ul[e] = sU[e][fy][fx + 1];
+                ur[e] = sU[e][fy + 1][fx + 1];
+            }
+            riemann_t<1, SOL>(gam, gr, wl, wr, ul, ur, f);
+#pragma unroll
+            for (int e = 0; e < 5; ++e) sG[e][fy][fx] = f[e];
+        };
+        xface(tx, ty);
+        yface(tx, ty);
+        if (tid < TX) yface(tid, TY);
+        else if (tid >= XW && tid < XW + TY) xface(TX, tid - XW);
         lds_barrier();
         if (active) {
             double v[5];
@@ -365,9 +421,10 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
 #pragma unroll
             for (int e = 0; e < 5; ++e) o[e] = v[e];
             wind_override(P, wmask, i, j, k, o);
-            const long long c = g.idx(i, j, k);
+            R oh[5];
 #pragma unroll
-            for (int e = 0; e < 5; ++e) Uh[e * g.nc + c] = (R)o[e];
+            for (int e = 0; e < 5; ++e) oh[e] = (R)o[e];
+            store_col(Uh, pl, cc, ku, oh);
             write_face_ghosts(P, wmask, Uh, i, j, k, v, true);
         }
 #pragma unroll
@@ -425,6 +482,9 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
     const bool active = i <= g.nx && j <= g.ny;
     const HaloTask ht = halo_task2<TX, TY>(tid);
     const R *__restrict__ hsrc = ht.kind == 2 ? U : Uh;
+    const Planes pl = planes_of<R>(g);
+    const Column cc = column_of<R>(g, i, j);
+    const Column ch = column_of<R>(g, x0 + ht.lx, y0 + ht.ly);
 
     const double dt = dt_from_maxima(P, st->maxima[parity]);
     const double dtdx = dt / P.dx, dtdy = dt / P.dy, dtdz = dt / P.dz;
@@ -456,13 +516,13 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
     R pf_q[5], pf_n[5], pf_h[5];
     {
         R wmm[5], wm[5], um[5], t5[5], hzm[5], wl[5], wr[5];
-        load_raw(P, Uh, i, j, z0 - 2, t5);
+        load_col(Uh, pl, cc, z0 - 2, t5);
         prims(P, t5, wmm);
-        load_raw(P, Uh, i, j, z0 - 1, um);
+        load_col(Uh, pl, cc, z0 - 1, um);
         prims(P, um, wm);
-        load_raw(P, Uh, i, j, z0, uc);
+        load_col(Uh, pl, cc, z0, uc);
         prims(P, uc, wc);
-        load_raw(P, Uh, i, j, z0 + 1, up);
+        load_col(Uh, pl, cc, z0 + 1, up);
         prims(P, up, wp);
 #pragma unroll
         for (int e = 0; e < 5; ++e) {
@@ -472,12 +532,12 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
             wr[e] = wc[e] - hz[e];
         }
         riemann_t<2, SOL>(gam, gr, wl, wr, um, uc, hm);
-        load_raw(P, U, i, j, z0 - 1, nm);
-        load_raw(P, U, i, j, z0, nc);
+        load_col(U, pl, cc, z0 - 1, nm);
+        load_col(U, pl, cc, z0, nc);
     }
-    load_raw(P, Uh, i, j, z0 + 2, pf_q);
-    load_raw(P, U, i, j, z0 + 1, pf_n);
-    if (ht.kind) load_raw(P, hsrc, x0 + ht.lx, y0 + ht.ly, z0, pf_h);
+    load_col(Uh, pl, cc, z0 + 2, pf_q);
+    load_col(U, pl, cc, z0 + 1, pf_n);
+    if (ht.kind) load_col(hsrc, pl, ch, z0, pf_h);
 
     float mx = 0.f, my = 0.f, mz = 0.f;
     double mxd = 0.0, myd = 0.0, mzd = 0.0;
@@ -489,10 +549,13 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
             np[e] = pf_n[e];
             hr[e] = pf_h[e];
         }
-        if (k < z1) {  // prefetch plane k+1
-            load_raw(P, Uh, i, j, k + 3, pf_q);
-            load_raw(P, U, i, j, k + 2, pf_n);
-            if (ht.kind) load_raw(P, hsrc, x0 + ht.lx, y0 + ht.ly, k + 1, pf_h);
+        // The plane index is block-uniform; readfirstlane lets the compiler
+        // form the plane addresses in scalar registers.
+        const int ku = __builtin_amdgcn_readfirstlane(k);
+        if (k < z1) {  // prefetch plane k+1 (planes z0-2 .. z1+2 <= nz+2)
+            load_col(Uh, pl, cc, ku + 3, pf_q);
+            load_col(U, pl, cc, ku + 2, pf_n);
+            if (ht.kind) load_col(hsrc, pl, ch, ku + 1, pf_h);
         }
         prims(P, uq, wq);
         R hzp[5];
@@ -618,7 +681,6 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
                               sN[e][ty][tx + 1] + np[e] + nm[e] - R(6) * nc[e];
                 oq[e] = (R)(o[e] + eta * (double)lap);
             }
-            const long long cc = g.idx(i, j, k);
             // calc_prims + cooling (cooling.f90:64-117) + time_step maxima
             if constexpr (sizeof(R) == 8) {
                 const double rho = oq[0];
@@ -657,8 +719,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
                 my = fmaxf(my, fabsf(vy) + cs);
                 mz = fmaxf(mz, fabsf(vz) + cs);
             }
-#pragma unroll
-            for (int e = 0; e < 5; ++e) Unew[e * g.nc + cc] = oq[e];
+            store_col(Unew, pl, cc, ku, oq);
         }
 #pragma unroll
         for (int e = 0; e < 5; ++e) {
