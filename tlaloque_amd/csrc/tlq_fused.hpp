@@ -110,14 +110,12 @@ __device__ __forceinline__ void prims(const Problem &P, const R u[5], R w[5]) {
 
 // Addressing of the z march.  A thread computes the byte offset of its
 // column (i, j) inside a plane once; every load and store of the march then
-// adds it to a block-uniform plane address (the scalar-base form of the
-// global memory instructions), and the range check is a lane mask fixed for
-// the whole march.  Cell (i, j, k) of field e sits at byte
+// adds it to a block-uniform plane address held in scalar registers, with no
+// per-access range check.  Cell (i, j, k) of field e sits at byte
 //   e*fbytes + (k+1)*pbytes + off(i, j),  off(i, j) = idx(i, j, -1)*sizeof(R),
 // which is below pbytes for -1 <= i <= nx+2, -1 <= j <= ny+2.
 struct Column {
     unsigned off;  // byte offset of (i, j) inside its plane (0 outside the array)
-    bool in;       // -1 <= i <= nx+2 and -1 <= j <= ny+2
 };
 
 struct Planes {
@@ -132,10 +130,8 @@ __device__ __forceinline__ Planes planes_of(const Grid &g) {
 
 template <class R>
 __device__ __forceinline__ Column column_of(const Grid &g, int i, int j) {
-    Column c;
-    c.in = i >= -1 && i <= g.nx + 2 && j >= -1 && j <= g.ny + 2;
-    c.off = c.in ? (unsigned)((g.base - g.sxy + i + (long long)g.sx * j) * (long long)sizeof(R)) : 0u;
-    return c;
+    const bool in = i >= -1 && i <= g.nx + 2 && j >= -1 && j <= g.ny + 2;
+    return Column{in ? (unsigned)((g.base - g.sxy + i + (long long)g.sx * j) * (long long)sizeof(R)) : 0u};
 }
 
 // Address of field e, plane k (block-uniform).
