@@ -368,44 +368,54 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
         // Faces: the left x face and the lower y face of every tile cell,
         // then the upper y faces of the last row (first wave) and the right
         // x faces of the last column (next wave), so that no wave runs both
-        // directions in one pass.
-        auto xface = [&](int fx, int fy) {  // between tile cells fx-1 and fx
-            R wl[5], wr[5], ul[5], ur[5], f[5];
+        // directions in one pass.  The cell's own state comes from registers
+        // (it is what this thread stored), and so do the fluxes of its own
+        // two faces in the update.
+        auto xface = [&](int fx, int fy, const R *wr, const R *ur, R f[5]) {  // cells fx-1 | fx
+            R wl[5], ul[5];
 #pragma unroll
             for (int e = 0; e < 5; ++e) {
                 wl[e] = sW[e][fy + 1][fx];
-                wr[e] = sW[e][fy + 1][fx + 1];
                 ul[e] = sU[e][fy + 1][fx];
-                ur[e] = sU[e][fy + 1][fx + 1];
             }
             riemann_t<0, SOL>(gam, gr, wl, wr, ul, ur, f);
 #pragma unroll
             for (int e = 0; e < 5; ++e) sF[e][fy][fx] = f[e];
         };
-        auto yface = [&](int fx, int fy) {  // between tile rows fy-1 and fy
-            R wl[5], wr[5], ul[5], ur[5], f[5];
+        auto yface = [&](int fx, int fy, const R *wr, const R *ur, R f[5]) {  // rows fy-1 | fy
+            R wl[5], ul[5];
 #pragma unroll
             for (int e = 0; e < 5; ++e) {
                 wl[e] = sW[e][fy][fx + 1];
-                wr[e] = sW[e][fy + 1][fx + 1];
                 ul[e] = sU[e][fy][fx + 1];
-                ur[e] = sU[e][fy + 1][fx + 1];
             }
             riemann_t<1, SOL>(gam, gr, wl, wr, ul, ur, f);
 #pragma unroll
             for (int e = 0; e < 5; ++e) sG[e][fy][fx] = f[e];
         };
-        xface(tx, ty);
-        yface(tx, ty);
-        if (tid < TX) yface(tid, TY);
-        else if (tid >= XW && tid < XW + TY) xface(TX, tid - XW);
+        // A face whose upper cell (column fx, row fy in tile indices) is a halo cell.
+        auto edge = [&](int fx, int fy, bool xdir) {
+            R wr[5], ur[5], f[5];
+#pragma unroll
+            for (int e = 0; e < 5; ++e) {
+                wr[e] = sW[e][fy + 1][fx + 1];
+                ur[e] = sU[e][fy + 1][fx + 1];
+            }
+            if (xdir) xface(fx, fy, wr, ur, f);
+            else yface(fx, fy, wr, ur, f);
+        };
+        R fxm[5], gym[5];
+        xface(tx, ty, wc, uc, fxm);
+        yface(tx, ty, wc, uc, gym);
+        if (tid < TX) edge(tid, TY, false);
+        else if (tid >= XW && tid < XW + TY) edge(TX, tid - XW, true);
         lds_barrier();
         if (active) {
             double v[5];
 #pragma unroll
             for (int e = 0; e < 5; ++e)
-                v[e] = godunov1<R>(uc[e], sF[e][ty][tx + 1], sF[e][ty][tx], sG[e][ty + 1][tx],
-                                   sG[e][ty][tx], hp[e], hm[e], dtdx, dtdy, dtdz);
+                v[e] = godunov1<R>(uc[e], sF[e][ty][tx + 1], fxm[e], sG[e][ty + 1][tx], gym[e],
+                                   hp[e], hm[e], dtdx, dtdy, dtdz);
             if (grav) {
                 double s[5];
                 gravity_src<R>(P, i, j, k, wc, s);
@@ -607,8 +617,19 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
                 sYp[e][ly + 1][lx] = b + h;
             }
         };
-        xstate(tx, ty);
-        ystate(tx, ty);
+        // The tile cell's own slopes: its centre value is in registers, and
+        // its lower face states (read only by this thread) stay there too.
+        R xm[5], ym[5];
+#pragma unroll
+        for (int e = 0; e < 5; ++e) {
+            const R b = wc[e];
+            const R hx = R(0.5) * avg<LIM>(lim, b - sW[e][ty + 2][tx + 1], sW[e][ty + 2][tx + 3] - b);
+            xm[e] = b - hx;
+            sXp[e][ty][tx + 1] = b + hx;
+            const R hy = R(0.5) * avg<LIM>(lim, b - sW[e][ty + 1][tx + 2], sW[e][ty + 3][tx + 2] - b);
+            ym[e] = b - hy;
+            sYp[e][ty + 1][tx] = b + hy;
+        }
         if (tid < 2 * TX) ystate(tid % TX, tid < TX ? -1 : TY);
         else if (tid < 2 * TX + 2 * TY) {
             const int h = tid - 2 * TX;
@@ -617,45 +638,56 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
         lds_barrier();
         // Faces: the lower x and y face of every tile cell, then the upper y
         // faces of the last row and the upper x faces of the last column.
-        auto xface = [&](int fx, int fy) {  // between tile cells fx-1 and fx
-            R wl[5], wr[5], ul[5], ur[5], f[5];
+        // The upper cell's state (wr) and conserved value (ur) of a tile
+        // cell's own lower faces come from registers; the fluxes of those two
+        // faces stay in registers for the update.
+        auto xface = [&](int fx, int fy, const R *wr, const R *ur, R f[5]) {  // cells fx-1 | fx
+            R wl[5], ul[5];
 #pragma unroll
             for (int e = 0; e < 5; ++e) {
                 wl[e] = sXp[e][fy][fx];
-                wr[e] = sXm[e][fy][fx + 1];
                 ul[e] = sU[e][fy + 1][fx];
-                ur[e] = sU[e][fy + 1][fx + 1];
             }
             riemann_t<0, SOL>(gam, gr, wl, wr, ul, ur, f);
 #pragma unroll
             for (int e = 0; e < 5; ++e) sF[e][fy][fx] = f[e];
         };
-        auto yface = [&](int fx, int fy) {  // between tile rows fy-1 and fy
-            R wl[5], wr[5], ul[5], ur[5], f[5];
+        auto yface = [&](int fx, int fy, const R *wr, const R *ur, R f[5]) {  // rows fy-1 | fy
+            R wl[5], ul[5];
 #pragma unroll
             for (int e = 0; e < 5; ++e) {
                 wl[e] = sYp[e][fy][fx];
-                wr[e] = sYm[e][fy + 1][fx];
                 ul[e] = sU[e][fy][fx + 1];
-                ur[e] = sU[e][fy + 1][fx + 1];
             }
             riemann_t<1, SOL>(gam, gr, wl, wr, ul, ur, f);
 #pragma unroll
             for (int e = 0; e < 5; ++e) sG[e][fy][fx] = f[e];
         };
+        // The faces of the upper row and right column, whose upper cell is a halo cell.
+        auto edge = [&](int fx, int fy, bool xdir) {
+            R wr[5], ur[5], f[5];
+#pragma unroll
+            for (int e = 0; e < 5; ++e) {
+                wr[e] = xdir ? sXm[e][fy][fx + 1] : sYm[e][fy + 1][fx];
+                ur[e] = sU[e][fy + 1][fx + 1];
+            }
+            if (xdir) xface(fx, fy, wr, ur, f);
+            else yface(fx, fy, wr, ur, f);
+        };
         // sW is read above and sF/sG overwrite it below: all face states
         // must be built before any face is written.
-        xface(tx, ty);
-        yface(tx, ty);
-        if (tid < TX) yface(tid, TY);
-        else if (tid < TX + TY) xface(TX, tid - TX);
+        R fxm[5], gym[5];
+        xface(tx, ty, xm, uc, fxm);
+        yface(tx, ty, ym, uc, gym);
+        if (tid < TX) edge(tid, TY, false);
+        else if (tid < TX + TY) edge(TX, tid - TX, true);
         lds_barrier();
         if (active) {
             double v[5];
 #pragma unroll
             for (int e = 0; e < 5; ++e)
-                v[e] = godunov1<R>(nc[e], sF[e][ty][tx + 1], sF[e][ty][tx], sG[e][ty + 1][tx],
-                                   sG[e][ty][tx], hp[e], hm[e], dtdx, dtdy, dtdz);
+                v[e] = godunov1<R>(nc[e], sF[e][ty][tx + 1], fxm[e], sG[e][ty + 1][tx], gym[e], hp[e],
+                                   hm[e], dtdx, dtdy, dtdz);
             if (grav) {
                 double s[5];
                 gravity_src<R>(P, i, j, k, wc, s);
