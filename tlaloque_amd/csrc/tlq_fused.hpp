@@ -349,7 +349,9 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
         }
         prims(P, un, wn);
         riemann_t<2, SOL>(gam, gr, wc, wn, uc, un, hp);
-        lds_barrier();
+        // No barrier before the plane's stores: sW and sU were last read by
+        // the previous plane's faces, which its flux barrier closed, and sF
+        // and sG are written only after this plane's next barrier.
 #pragma unroll
         for (int e = 0; e < 5; ++e) {
             sW[e][ty + 1][tx + 1] = wc[e];
@@ -453,18 +455,14 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
     static_assert(6 * (TX + TY) <= NT, "one halo cell per thread");
     constexpr int WX = TX + 4, WY = TY + 4;  // two-cell halo (primitives of U^{1/2})
     constexpr int CX = TX + 2, CY = TY + 2;  // one-cell halo (conserved U^{1/2}, U^n)
-    // The primitive plane is dead once the face states are built, so the
-    // flux arrays share its storage.
-    union PlaneOrFlux {
-        R W[5][WY][WX];
-        struct {
-            R F[5][TY][TX + 1];
-            R G[5][TY + 1][TX];
-        } f;
-    };
-    __shared__ PlaneOrFlux sPF;
+    // Every array has one writer phase and one reader phase per plane, and
+    // U^n's plane alternates between two buffers, so a plane's stores need no
+    // barrier against the previous plane's reads (see the loop).
+    __shared__ R sW[5][WY][WX];
+    __shared__ R sF[5][TY][TX + 1];
+    __shared__ R sG[5][TY + 1][TX];
     __shared__ R sU[5][CY][CX];
-    __shared__ R sN[5][CY][CX];
+    __shared__ R sN2[2][5][CY][CX];
     // Reconstructed face states of each cell: x faces of cells -1..TX in the
     // tile rows, y faces of rows -1..TY in the tile columns (m: the cell's
     // lower face, p: its upper face).
@@ -473,9 +471,6 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
     __shared__ unsigned long long s_wmask;
     // FP32 cooling table (one-step cooling in FP32 mode)
     __shared__ TabRow sTab[sizeof(R) == 4 ? MAX_TABLE : 1];
-    auto &sW = sPF.W;
-    auto &sF = sPF.f.F;
-    auto &sG = sPF.f.G;
 
     const Problem &P = *Pp;
     const Grid &g = P.g;
@@ -575,7 +570,13 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
             }
             riemann_t<2, SOL>(gam, gr, wl, wr, uc, up, hp);
         }
-        lds_barrier();
+        // Phases of a plane, each closed by a barrier: (1) store sW, sU, sN;
+        // (2) face states from sW; (3) face fluxes from the states and sU into
+        // sF, sG; (4) the update from sF, sG, sN.  The previous plane read
+        // sW in its phase 2 and sU in its phase 3, both closed by its later
+        // barriers; its phase 4 reads the other sN buffer and sF, sG, which
+        // this plane rewrites only after its own first barrier.
+        auto &sN = sN2[(k - z0) & 1];
 #pragma unroll
         for (int e = 0; e < 5; ++e) {
             sW[e][ty + 2][tx + 2] = wc[e];
@@ -674,8 +675,6 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
             if (xdir) xface(fx, fy, wr, ur, f);
             else yface(fx, fy, wr, ur, f);
         };
-        // sW is read above and sF/sG overwrite it below: all face states
-        // must be built before any face is written.
         R fxm[5], gym[5];
         xface(tx, ty, xm, uc, fxm);
         yface(tx, ty, ym, uc, gym);
