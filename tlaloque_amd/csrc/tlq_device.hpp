@@ -173,12 +173,15 @@ template <class R> __device__ __forceinline__ R lim_avg(int opt, R a, R b) {
             // binade of m = max(|a|,|b|) (exact scaling, |a|, |b| 2^-x < 1):
             // no a*b product to underflow for cgs density differences (~1e-25)
             // and no overflow.
+            // The slope is 0 where a b <= 0: max(a b, 0) gives that without
+            // a select, and the 1e-30 keeps the denominator (>= 1/4 when
+            // m > 0) nonzero when a = b = 0.
             const R m = fmaxf(__builtin_fabsf(a), __builtin_fabsf(b));
             const int x = __builtin_amdgcn_frexp_expf(m);
             const R as = __builtin_ldexpf(a, -x), bs = __builtin_ldexpf(b, -x);
-            const R ab = as * bs;
-            const R r = ab * (a + b) * __builtin_amdgcn_rcpf(__builtin_fmaf(as, as, bs * bs));
-            return ab > 0.f ? r : 0.f;
+            const R ab = fmaxf(as * bs, 0.f);
+            return ab * (a + b) *
+                   __builtin_amdgcn_rcpf(__builtin_fmaf(as, as, __builtin_fmaf(bs, bs, 1e-30f)));
         }
     case LIM_MINMOD:
         s = __builtin_copysign(R(1), a);
@@ -217,8 +220,13 @@ __device__ __forceinline__ void wavespeed(R gamma, const R wl[5], const R wr[5],
     constexpr int n = D + 1;
     const R csl = rsqrt_(rdiv(gamma * wl[4], wl[0]));
     const R csr = rsqrt_(rdiv(gamma * wr[4], wr[0]));
-    sl = rmin(wl[n] - csl, wr[n] - csr);
-    sr = rmax(wl[n] + csl, wr[n] + csr);
+    if constexpr (sizeof(R) == 8) {
+        sl = rmin(wl[n] - csl, wr[n] - csr);
+        sr = rmax(wl[n] + csl, wr[n] + csr);
+    } else {  // the hardware min/max (equal for the finite values)
+        sl = fminf(wl[n] - csl, wr[n] - csr);
+        sr = fmaxf(wl[n] + csl, wr[n] + csr);
+    }
     sst = rdiv(wr[4] - wl[4] + wl[0] * wl[n] * (sl - wl[n]) - wr[0] * wr[n] * (sr - wr[n]),
                wl[0] * (sl - wl[n]) - wr[0] * (sr - wr[n]));
 }
