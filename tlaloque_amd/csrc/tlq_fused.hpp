@@ -168,6 +168,14 @@ template <int LIM, class R> __device__ __forceinline__ R avg(int lim, R a, R b) 
     return lim_avg<R>(LIM > 0 ? LIM : lim, a, b);
 }
 
+// b + s/2 and b - s/2 (the face states of a cell of value b and slope s) as
+// one fused operation each: s/2 is exact, so the result is that of the
+// separate product and sum.
+__device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_(double a, double b, double c) { return __builtin_fma(a, b, c); }
+template <class R> __device__ __forceinline__ R plus_half(R b, R s) { return fma_(R(0.5), s, b); }
+template <class R> __device__ __forceinline__ R minus_half(R b, R s) { return fma_(R(-0.5), s, b); }
+
 // Limited reconstruction of the face between b and c from the four states
 // a, b, c, d along the axis (limiter, HLLC.f90:574-582).
 template <int LIM, class R>
@@ -176,8 +184,8 @@ __device__ __forceinline__ void reconstruct(int lim, const R a[5], const R b[5],
 #pragma unroll
     for (int e = 0; e < 5; ++e) {
         const R dl = b[e] - a[e], dm = c[e] - b[e], dr = d[e] - c[e];
-        wl[e] = b[e] + R(0.5) * avg<LIM>(lim, dl, dm);
-        wr[e] = c[e] - R(0.5) * avg<LIM>(lim, dm, dr);
+        wl[e] = plus_half(b[e], avg<LIM>(lim, dl, dm));
+        wr[e] = minus_half(c[e], avg<LIM>(lim, dm, dr));
     }
 }
 
@@ -511,8 +519,8 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
     }
 
     // z pipeline of U^{1/2}: primitives at k, k+1, conserved at k, k+1, the
-    // half slope of cell k (limiter, HLLC.f90:574-582); U^n at k-1, k; the z
-    // flux at k-1/2.
+    // slope of cell k (limiter, HLLC.f90:574-582); U^n at k-1, k; the z flux
+    // at k-1/2.
     R wc[5], wp[5], uc[5], up[5], hz[5], hm[5], nm[5], nc[5];
     R pf_q[5], pf_n[5], pf_h[5];
     {
@@ -527,10 +535,10 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
         prims(P, up, wp);
 #pragma unroll
         for (int e = 0; e < 5; ++e) {
-            hzm[e] = R(0.5) * avg<LIM>(lim, wm[e] - wmm[e], wc[e] - wm[e]);
-            hz[e] = R(0.5) * avg<LIM>(lim, wc[e] - wm[e], wp[e] - wc[e]);
-            wl[e] = wm[e] + hzm[e];
-            wr[e] = wc[e] - hz[e];
+            hzm[e] = avg<LIM>(lim, wm[e] - wmm[e], wc[e] - wm[e]);
+            hz[e] = avg<LIM>(lim, wc[e] - wm[e], wp[e] - wc[e]);
+            wl[e] = plus_half(wm[e], hzm[e]);
+            wr[e] = minus_half(wc[e], hz[e]);
         }
         riemann_t<2, SOL>(gam, gr, wl, wr, um, uc, hm);
         load_col(U, pl, cc, z0 - 1, nm);
@@ -564,9 +572,9 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
             R wl[5], wr[5];
 #pragma unroll
             for (int e = 0; e < 5; ++e) {
-                hzp[e] = R(0.5) * avg<LIM>(lim, wp[e] - wc[e], wq[e] - wp[e]);
-                wl[e] = wc[e] + hz[e];
-                wr[e] = wp[e] - hzp[e];
+                hzp[e] = avg<LIM>(lim, wp[e] - wc[e], wq[e] - wp[e]);
+                wl[e] = plus_half(wc[e], hz[e]);
+                wr[e] = minus_half(wp[e], hzp[e]);
             }
             riemann_t<2, SOL>(gam, gr, wl, wr, uc, up, hp);
         }
@@ -604,18 +612,18 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
 #pragma unroll
             for (int e = 0; e < 5; ++e) {
                 const R a = sW[e][ly + 2][lx + 1], b = sW[e][ly + 2][lx + 2], c = sW[e][ly + 2][lx + 3];
-                const R h = R(0.5) * avg<LIM>(lim, b - a, c - b);
-                sXm[e][ly][lx + 1] = b - h;
-                sXp[e][ly][lx + 1] = b + h;
+                const R h = avg<LIM>(lim, b - a, c - b);
+                sXm[e][ly][lx + 1] = minus_half(b, h);
+                sXp[e][ly][lx + 1] = plus_half(b, h);
             }
         };
         auto ystate = [&](int lx, int ly) {
 #pragma unroll
             for (int e = 0; e < 5; ++e) {
                 const R a = sW[e][ly + 1][lx + 2], b = sW[e][ly + 2][lx + 2], c = sW[e][ly + 3][lx + 2];
-                const R h = R(0.5) * avg<LIM>(lim, b - a, c - b);
-                sYm[e][ly + 1][lx] = b - h;
-                sYp[e][ly + 1][lx] = b + h;
+                const R h = avg<LIM>(lim, b - a, c - b);
+                sYm[e][ly + 1][lx] = minus_half(b, h);
+                sYp[e][ly + 1][lx] = plus_half(b, h);
             }
         };
         // The tile cell's own slopes: its centre value is in registers, and
@@ -624,12 +632,12 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
 #pragma unroll
         for (int e = 0; e < 5; ++e) {
             const R b = wc[e];
-            const R hx = R(0.5) * avg<LIM>(lim, b - sW[e][ty + 2][tx + 1], sW[e][ty + 2][tx + 3] - b);
-            xm[e] = b - hx;
-            sXp[e][ty][tx + 1] = b + hx;
-            const R hy = R(0.5) * avg<LIM>(lim, b - sW[e][ty + 1][tx + 2], sW[e][ty + 3][tx + 2] - b);
-            ym[e] = b - hy;
-            sYp[e][ty + 1][tx] = b + hy;
+            const R hx = avg<LIM>(lim, b - sW[e][ty + 2][tx + 1], sW[e][ty + 2][tx + 3] - b);
+            xm[e] = minus_half(b, hx);
+            sXp[e][ty][tx + 1] = plus_half(b, hx);
+            const R hy = avg<LIM>(lim, b - sW[e][ty + 1][tx + 2], sW[e][ty + 3][tx + 2] - b);
+            ym[e] = minus_half(b, hy);
+            sYp[e][ty + 1][tx] = plus_half(b, hy);
         }
         if (tid < 2 * TX) ystate(tid % TX, tid < TX ? -1 : TY);
         else if (tid < 2 * TX + 2 * TY) {
