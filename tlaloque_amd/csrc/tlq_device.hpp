@@ -425,13 +425,18 @@ __device__ __forceinline__ double inv64(float r) {
 
 // calc_prims in FP32 with the FP64-guarded pressure recovery.
 __device__ __forceinline__ void cons2prim_fast(double gm1, const float u[5], float w[5]) {
+    // No contraction of these products into the caller's sums: a cell's
+    // primitives then round alike wherever they are used (a tile's first
+    // planes and the same planes inside the neighbouring tile's march).
+#pragma clang fp contract(off)
     const float inv = __builtin_amdgcn_rcpf(u[0]);
     w[0] = u[0];
     w[1] = u[1] * inv;
     w[2] = u[2] * inv;
     w[3] = u[3] * inv;
     const double m1 = u[1], m2 = u[2], m3 = u[3];
-    double p = gm1 * ((double)u[4] - 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) * inv64(u[0]));
+    const double m2s = __builtin_fma(m3, m3, __builtin_fma(m2, m2, m1 * m1));
+    const double p = gm1 * __builtin_fma(-0.5 * m2s, inv64(u[0]), (double)u[4]);
     w[4] = p > 1e-20 ? (float)p : 1e-20f;
 }
 

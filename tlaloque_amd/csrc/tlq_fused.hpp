@@ -329,19 +329,17 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
         s_wmask = P.nwinds ? winds_mask(P, x0 - 2, x0 + TX + 1, y0 - 2, y0 + TY + 1, z0 - 2, z1 + 2)
                            : 0ull;
 
-    R uc[5], wc[5], hm[5], pf_n[5], pf_h[5];
-    {
-        R up[5], wp[5];
-        load_col(U, pl, cc, z0 - 1, up);
-        prims(P, up, wp);
-        load_col(U, pl, cc, z0, uc);
-        prims(P, uc, wc);
-        riemann_t<2, SOL>(gam, gr, wp, wc, up, uc, hm);
-    }
-    load_col(U, pl, cc, z0 + 1, pf_n);
-    if (ht.kind) load_col(U, pl, ch, z0, pf_h);
+    // The march starts one plane below the tile (k = z0 - 1), a step whose
+    // update is not stored: it forms the z flux through the tile's lower
+    // face with the code of every later step, so the two tiles that share
+    // the face compute the same flux.
+    R uc[5], wc[5], hm[5] = {}, pf_n[5], pf_h[5];
+    load_col(U, pl, cc, z0 - 1, uc);
+    prims(P, uc, wc);
+    load_col(U, pl, cc, z0, pf_n);
+    if (ht.kind) load_col(U, pl, ch, z0 - 1, pf_h);
 
-    for (int k = z0; k <= z1; ++k) {
+    for (int k = z0 - 1; k <= z1; ++k) {
         R un[5], wn[5], hr[5], hp[5];
 #pragma unroll
         for (int e = 0; e < 5; ++e) {
@@ -420,7 +418,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
         if (tid < TX) edge(tid, TY, false);
         else if (tid >= XW && tid < XW + TY) edge(TX, tid - XW, true);
         lds_barrier();
-        if (active) {
+        if (active && k >= z0) {
             double v[5];
 #pragma unroll
             for (int e = 0; e < 5; ++e)
@@ -521,36 +519,29 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
     // z pipeline of U^{1/2}: primitives at k, k+1, conserved at k, k+1, the
     // slope of cell k (limiter, HLLC.f90:574-582); U^n at k-1, k; the z flux
     // at k-1/2.
-    R wc[5], wp[5], uc[5], up[5], hz[5], hm[5], nm[5], nc[5];
-    R pf_q[5], pf_n[5], pf_h[5];
+    R wc[5], wp[5], uc[5], up[5], hz[5], hm[5] = {}, nm[5] = {}, nc[5];
+    R pf_q[5], pf_n[5], pf_h[5] = {};
+    // The march starts one plane below the tile (k = z0 - 1): that step only
+    // forms the z flux through the tile's lower face, with the code of every
+    // later step, so the two tiles that share the face compute the same flux.
     {
-        R wmm[5], wm[5], um[5], t5[5], hzm[5], wl[5], wr[5];
+        R wmm[5], t5[5];
         load_col(Uh, pl, cc, z0 - 2, t5);
         prims(P, t5, wmm);
-        load_col(Uh, pl, cc, z0 - 1, um);
-        prims(P, um, wm);
-        load_col(Uh, pl, cc, z0, uc);
+        load_col(Uh, pl, cc, z0 - 1, uc);
         prims(P, uc, wc);
-        load_col(Uh, pl, cc, z0 + 1, up);
+        load_col(Uh, pl, cc, z0, up);
         prims(P, up, wp);
 #pragma unroll
-        for (int e = 0; e < 5; ++e) {
-            hzm[e] = avg<LIM>(lim, wm[e] - wmm[e], wc[e] - wm[e]);
-            hz[e] = avg<LIM>(lim, wc[e] - wm[e], wp[e] - wc[e]);
-            wl[e] = plus_half(wm[e], hzm[e]);
-            wr[e] = minus_half(wc[e], hz[e]);
-        }
-        riemann_t<2, SOL>(gam, gr, wl, wr, um, uc, hm);
-        load_col(U, pl, cc, z0 - 1, nm);
-        load_col(U, pl, cc, z0, nc);
+        for (int e = 0; e < 5; ++e) hz[e] = avg<LIM>(lim, wc[e] - wmm[e], wp[e] - wc[e]);
+        load_col(U, pl, cc, z0 - 1, nc);
     }
-    load_col(Uh, pl, cc, z0 + 2, pf_q);
-    load_col(U, pl, cc, z0 + 1, pf_n);
-    if (ht.kind) load_col(hsrc, pl, ch, z0, pf_h);
+    load_col(Uh, pl, cc, z0 + 1, pf_q);
+    load_col(U, pl, cc, z0, pf_n);
 
     float mx = 0.f, my = 0.f, mz = 0.f;
     double mxd = 0.0, myd = 0.0, mzd = 0.0;
-    for (int k = z0; k <= z1; ++k) {
+    for (int k = z0 - 1; k <= z1; ++k) {
         R uq[5], wq[5], np[5], hr[5], hp[5];
 #pragma unroll
         for (int e = 0; e < 5; ++e) {
@@ -578,183 +569,185 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
             }
             riemann_t<2, SOL>(gam, gr, wl, wr, uc, up, hp);
         }
-        // Phases of a plane, each closed by a barrier: (1) store sW, sU, sN;
-        // (2) face states from sW; (3) face fluxes from the states and sU into
-        // sF, sG; (4) the update from sF, sG, sN.  The previous plane read
-        // sW in its phase 2 and sU in its phase 3, both closed by its later
-        // barriers; its phase 4 reads the other sN buffer and sF, sG, which
-        // this plane rewrites only after its own first barrier.
-        auto &sN = sN2[(k - z0) & 1];
-#pragma unroll
-        for (int e = 0; e < 5; ++e) {
-            sW[e][ty + 2][tx + 2] = wc[e];
-            sU[e][ty + 1][tx + 1] = uc[e];
-            sN[e][ty + 1][tx + 1] = nc[e];
-        }
-        if (ht.kind == 1) {
-            R wh[5];
-            prims(P, hr, wh);
-#pragma unroll
-            for (int e = 0; e < 5; ++e) sW[e][ht.ly + 2][ht.lx + 2] = wh[e];
-            if (ht.lx >= -1 && ht.lx <= TX && ht.ly >= -1 && ht.ly <= TY) {
-#pragma unroll
-                for (int e = 0; e < 5; ++e) sU[e][ht.ly + 1][ht.lx + 1] = hr[e];
-            }
-        } else if (ht.kind == 2) {
-#pragma unroll
-            for (int e = 0; e < 5; ++e) sN[e][ht.ly + 1][ht.lx + 1] = hr[e];
-        }
-        lds_barrier();
-        // Face states: one limited slope per cell and axis (limiter,
-        // HLLC.f90:574-582: the slope of cell i is average(P_i - P_{i-1},
-        // P_{i+1} - P_i) for both of its faces).
-        auto xstate = [&](int lx, int ly) {
+        if (k >= z0) {
+            // Phases of a plane, each closed by a barrier: (1) store sW, sU, sN;
+            // (2) face states from sW; (3) face fluxes from the states and sU into
+            // sF, sG; (4) the update from sF, sG, sN.  The previous plane read
+            // sW in its phase 2 and sU in its phase 3, both closed by its later
+            // barriers; its phase 4 reads the other sN buffer and sF, sG, which
+            // this plane rewrites only after its own first barrier.
+            auto &sN = sN2[(k - z0) & 1];
 #pragma unroll
             for (int e = 0; e < 5; ++e) {
-                const R a = sW[e][ly + 2][lx + 1], b = sW[e][ly + 2][lx + 2], c = sW[e][ly + 2][lx + 3];
-                const R h = avg<LIM>(lim, b - a, c - b);
-                sXm[e][ly][lx + 1] = minus_half(b, h);
-                sXp[e][ly][lx + 1] = plus_half(b, h);
+                sW[e][ty + 2][tx + 2] = wc[e];
+                sU[e][ty + 1][tx + 1] = uc[e];
+                sN[e][ty + 1][tx + 1] = nc[e];
             }
-        };
-        auto ystate = [&](int lx, int ly) {
+            if (ht.kind == 1) {
+                R wh[5];
+                prims(P, hr, wh);
 #pragma unroll
-            for (int e = 0; e < 5; ++e) {
-                const R a = sW[e][ly + 1][lx + 2], b = sW[e][ly + 2][lx + 2], c = sW[e][ly + 3][lx + 2];
-                const R h = avg<LIM>(lim, b - a, c - b);
-                sYm[e][ly + 1][lx] = minus_half(b, h);
-                sYp[e][ly + 1][lx] = plus_half(b, h);
-            }
-        };
-        // The tile cell's own slopes: its centre value is in registers, and
-        // its lower face states (read only by this thread) stay there too.
-        R xm[5], ym[5];
+                for (int e = 0; e < 5; ++e) sW[e][ht.ly + 2][ht.lx + 2] = wh[e];
+                if (ht.lx >= -1 && ht.lx <= TX && ht.ly >= -1 && ht.ly <= TY) {
 #pragma unroll
-        for (int e = 0; e < 5; ++e) {
-            const R b = wc[e];
-            const R hx = avg<LIM>(lim, b - sW[e][ty + 2][tx + 1], sW[e][ty + 2][tx + 3] - b);
-            xm[e] = minus_half(b, hx);
-            sXp[e][ty][tx + 1] = plus_half(b, hx);
-            const R hy = avg<LIM>(lim, b - sW[e][ty + 1][tx + 2], sW[e][ty + 3][tx + 2] - b);
-            ym[e] = minus_half(b, hy);
-            sYp[e][ty + 1][tx] = plus_half(b, hy);
-        }
-        if (tid < 2 * TX) ystate(tid % TX, tid < TX ? -1 : TY);
-        else if (tid < 2 * TX + 2 * TY) {
-            const int h = tid - 2 * TX;
-            xstate((h & 1) ? TX : -1, h >> 1);
-        }
-        lds_barrier();
-        // Faces: the lower x and y face of every tile cell, then the upper y
-        // faces of the last row and the upper x faces of the last column.
-        // The upper cell's state (wr) and conserved value (ur) of a tile
-        // cell's own lower faces come from registers; the fluxes of those two
-        // faces stay in registers for the update.
-        auto xface = [&](int fx, int fy, const R *wr, const R *ur, R f[5]) {  // cells fx-1 | fx
-            R wl[5], ul[5];
-#pragma unroll
-            for (int e = 0; e < 5; ++e) {
-                wl[e] = sXp[e][fy][fx];
-                ul[e] = sU[e][fy + 1][fx];
-            }
-            riemann_t<0, SOL>(gam, gr, wl, wr, ul, ur, f);
-#pragma unroll
-            for (int e = 0; e < 5; ++e) sF[e][fy][fx] = f[e];
-        };
-        auto yface = [&](int fx, int fy, const R *wr, const R *ur, R f[5]) {  // rows fy-1 | fy
-            R wl[5], ul[5];
-#pragma unroll
-            for (int e = 0; e < 5; ++e) {
-                wl[e] = sYp[e][fy][fx];
-                ul[e] = sU[e][fy][fx + 1];
-            }
-            riemann_t<1, SOL>(gam, gr, wl, wr, ul, ur, f);
-#pragma unroll
-            for (int e = 0; e < 5; ++e) sG[e][fy][fx] = f[e];
-        };
-        // The faces of the upper row and right column, whose upper cell is a halo cell.
-        auto edge = [&](int fx, int fy, bool xdir) {
-            R wr[5], ur[5], f[5];
-#pragma unroll
-            for (int e = 0; e < 5; ++e) {
-                wr[e] = xdir ? sXm[e][fy][fx + 1] : sYm[e][fy + 1][fx];
-                ur[e] = sU[e][fy + 1][fx + 1];
-            }
-            if (xdir) xface(fx, fy, wr, ur, f);
-            else yface(fx, fy, wr, ur, f);
-        };
-        R fxm[5], gym[5];
-        xface(tx, ty, xm, uc, fxm);
-        yface(tx, ty, ym, uc, gym);
-        if (tid < TX) edge(tid, TY, false);
-        else if (tid < TX + TY) edge(TX, tid - TX, true);
-        lds_barrier();
-        if (active) {
-            double v[5];
-#pragma unroll
-            for (int e = 0; e < 5; ++e)
-                v[e] = godunov1<R>(nc[e], sF[e][ty][tx + 1], fxm[e], sG[e][ty + 1][tx], gym[e], hp[e],
-                                   hm[e], dtdx, dtdy, dtdz);
-            if (grav) {
-                double s[5];
-                gravity_src<R>(P, i, j, k, wc, s);
-#pragma unroll
-                for (int e = 0; e < 5; ++e) v[e] = v[e] + dt * s[e];
-            }
-            const unsigned long long wmask = s_wmask;
-            // ghosts of U^{n+1} = boundary fill of the pre-wind UP, then winds
-            write_face_ghosts(P, wmask, Unew, i, j, k, v, false);
-            double o[5];
-#pragma unroll
-            for (int e = 0; e < 5; ++e) o[e] = v[e];
-            wind_override(P, wmask, i, j, k, o);
-            // step: U^{n+1} = UP + eta * Lap(U^n)  (HydroCore.f90:184-193)
-            R oq[5];
-#pragma unroll
-            for (int e = 0; e < 5; ++e) {
-                const R lap = sN[e][ty + 1][tx + 2] + sN[e][ty + 1][tx] + sN[e][ty + 2][tx + 1] +
-                              sN[e][ty][tx + 1] + np[e] + nm[e] - R(6) * nc[e];
-                oq[e] = (R)(o[e] + eta * (double)lap);
-            }
-            // calc_prims + cooling (cooling.f90:64-117) + time_step maxima
-            if constexpr (sizeof(R) == 8) {
-                const double rho = oq[0];
-                const double vx = oq[1] / oq[0], vy = oq[2] / oq[0], vz = oq[3] / oq[0];
-                double p = pressure64(P.gamma, rho, oq[1], oq[2], oq[3], oq[4]);
-                if (cool) {
-                    bool cooled;
-                    const double pn = cool_exact(P, rho, p, dt, &cooled);
-                    if (cooled) {
-                        const double ek = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
-                        oq[4] = ek + pn / g1;
-                        p = pn;
-                    }
+                    for (int e = 0; e < 5; ++e) sU[e][ht.ly + 1][ht.lx + 1] = hr[e];
                 }
-                const double cs = sqrt(P.gamma * p / rho);
-                mxd = fmax(mxd, fabs(vx) + cs);
-                myd = fmax(myd, fabs(vy) + cs);
-                mzd = fmax(mzd, fabs(vz) + cs);
-            } else {
-                const float rinv = __builtin_amdgcn_rcpf(oq[0]);
-                const float vx = oq[1] * rinv, vy = oq[2] * rinv, vz = oq[3] * rinv;
-                const double m1 = oq[1], m2 = oq[2], m3 = oq[3];
-                const double ek = 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) * inv64(oq[0]);
-                double pd = g1 * ((double)oq[4] - ek);
-                float p = pd > 1e-20 ? (float)pd : 1e-20f;
-                if (cool) {
-                    bool cooled;
-                    const float pn = cool_fast_tab(P, sTab, oq[0], p, (float)dt, &cooled);
-                    if (cooled) {
-                        oq[4] = (float)(ek + (double)pn * (double)P.f_inv_gm1);
-                        p = pn;
-                    }
-                }
-                const float cs = __builtin_amdgcn_sqrtf((float)P.gamma * p * rinv);
-                mx = fmaxf(mx, fabsf(vx) + cs);
-                my = fmaxf(my, fabsf(vy) + cs);
-                mz = fmaxf(mz, fabsf(vz) + cs);
+            } else if (ht.kind == 2) {
+#pragma unroll
+                for (int e = 0; e < 5; ++e) sN[e][ht.ly + 1][ht.lx + 1] = hr[e];
             }
-            store_col(Unew, pl, cc, ku, oq);
+            lds_barrier();
+            // Face states: one limited slope per cell and axis (limiter,
+            // HLLC.f90:574-582: the slope of cell i is average(P_i - P_{i-1},
+            // P_{i+1} - P_i) for both of its faces).
+            auto xstate = [&](int lx, int ly) {
+#pragma unroll
+                for (int e = 0; e < 5; ++e) {
+                    const R a = sW[e][ly + 2][lx + 1], b = sW[e][ly + 2][lx + 2], c = sW[e][ly + 2][lx + 3];
+                    const R h = avg<LIM>(lim, b - a, c - b);
+                    sXm[e][ly][lx + 1] = minus_half(b, h);
+                    sXp[e][ly][lx + 1] = plus_half(b, h);
+                }
+            };
+            auto ystate = [&](int lx, int ly) {
+#pragma unroll
+                for (int e = 0; e < 5; ++e) {
+                    const R a = sW[e][ly + 1][lx + 2], b = sW[e][ly + 2][lx + 2], c = sW[e][ly + 3][lx + 2];
+                    const R h = avg<LIM>(lim, b - a, c - b);
+                    sYm[e][ly + 1][lx] = minus_half(b, h);
+                    sYp[e][ly + 1][lx] = plus_half(b, h);
+                }
+            };
+            // The tile cell's own slopes: its centre value is in registers, and
+            // its lower face states (read only by this thread) stay there too.
+            R xm[5], ym[5];
+#pragma unroll
+            for (int e = 0; e < 5; ++e) {
+                const R b = wc[e];
+                const R hx = avg<LIM>(lim, b - sW[e][ty + 2][tx + 1], sW[e][ty + 2][tx + 3] - b);
+                xm[e] = minus_half(b, hx);
+                sXp[e][ty][tx + 1] = plus_half(b, hx);
+                const R hy = avg<LIM>(lim, b - sW[e][ty + 1][tx + 2], sW[e][ty + 3][tx + 2] - b);
+                ym[e] = minus_half(b, hy);
+                sYp[e][ty + 1][tx] = plus_half(b, hy);
+            }
+            if (tid < 2 * TX) ystate(tid % TX, tid < TX ? -1 : TY);
+            else if (tid < 2 * TX + 2 * TY) {
+                const int h = tid - 2 * TX;
+                xstate((h & 1) ? TX : -1, h >> 1);
+            }
+            lds_barrier();
+            // Faces: the lower x and y face of every tile cell, then the upper y
+            // faces of the last row and the upper x faces of the last column.
+            // The upper cell's state (wr) and conserved value (ur) of a tile
+            // cell's own lower faces come from registers; the fluxes of those two
+            // faces stay in registers for the update.
+            auto xface = [&](int fx, int fy, const R *wr, const R *ur, R f[5]) {  // cells fx-1 | fx
+                R wl[5], ul[5];
+#pragma unroll
+                for (int e = 0; e < 5; ++e) {
+                    wl[e] = sXp[e][fy][fx];
+                    ul[e] = sU[e][fy + 1][fx];
+                }
+                riemann_t<0, SOL>(gam, gr, wl, wr, ul, ur, f);
+#pragma unroll
+                for (int e = 0; e < 5; ++e) sF[e][fy][fx] = f[e];
+            };
+            auto yface = [&](int fx, int fy, const R *wr, const R *ur, R f[5]) {  // rows fy-1 | fy
+                R wl[5], ul[5];
+#pragma unroll
+                for (int e = 0; e < 5; ++e) {
+                    wl[e] = sYp[e][fy][fx];
+                    ul[e] = sU[e][fy][fx + 1];
+                }
+                riemann_t<1, SOL>(gam, gr, wl, wr, ul, ur, f);
+#pragma unroll
+                for (int e = 0; e < 5; ++e) sG[e][fy][fx] = f[e];
+            };
+            // The faces of the upper row and right column, whose upper cell is a halo cell.
+            auto edge = [&](int fx, int fy, bool xdir) {
+                R wr[5], ur[5], f[5];
+#pragma unroll
+                for (int e = 0; e < 5; ++e) {
+                    wr[e] = xdir ? sXm[e][fy][fx + 1] : sYm[e][fy + 1][fx];
+                    ur[e] = sU[e][fy + 1][fx + 1];
+                }
+                if (xdir) xface(fx, fy, wr, ur, f);
+                else yface(fx, fy, wr, ur, f);
+            };
+            R fxm[5], gym[5];
+            xface(tx, ty, xm, uc, fxm);
+            yface(tx, ty, ym, uc, gym);
+            if (tid < TX) edge(tid, TY, false);
+            else if (tid < TX + TY) edge(TX, tid - TX, true);
+            lds_barrier();
+            if (active) {
+                double v[5];
+#pragma unroll
+                for (int e = 0; e < 5; ++e)
+                    v[e] = godunov1<R>(nc[e], sF[e][ty][tx + 1], fxm[e], sG[e][ty + 1][tx], gym[e], hp[e],
+                                       hm[e], dtdx, dtdy, dtdz);
+                if (grav) {
+                    double s[5];
+                    gravity_src<R>(P, i, j, k, wc, s);
+#pragma unroll
+                    for (int e = 0; e < 5; ++e) v[e] = v[e] + dt * s[e];
+                }
+                const unsigned long long wmask = s_wmask;
+                // ghosts of U^{n+1} = boundary fill of the pre-wind UP, then winds
+                write_face_ghosts(P, wmask, Unew, i, j, k, v, false);
+                double o[5];
+#pragma unroll
+                for (int e = 0; e < 5; ++e) o[e] = v[e];
+                wind_override(P, wmask, i, j, k, o);
+                // step: U^{n+1} = UP + eta * Lap(U^n)  (HydroCore.f90:184-193)
+                R oq[5];
+#pragma unroll
+                for (int e = 0; e < 5; ++e) {
+                    const R lap = sN[e][ty + 1][tx + 2] + sN[e][ty + 1][tx] + sN[e][ty + 2][tx + 1] +
+                                  sN[e][ty][tx + 1] + np[e] + nm[e] - R(6) * nc[e];
+                    oq[e] = (R)(o[e] + eta * (double)lap);
+                }
+                // calc_prims + cooling (cooling.f90:64-117) + time_step maxima
+                if constexpr (sizeof(R) == 8) {
+                    const double rho = oq[0];
+                    const double vx = oq[1] / oq[0], vy = oq[2] / oq[0], vz = oq[3] / oq[0];
+                    double p = pressure64(P.gamma, rho, oq[1], oq[2], oq[3], oq[4]);
+                    if (cool) {
+                        bool cooled;
+                        const double pn = cool_exact(P, rho, p, dt, &cooled);
+                        if (cooled) {
+                            const double ek = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
+                            oq[4] = ek + pn / g1;
+                            p = pn;
+                        }
+                    }
+                    const double cs = sqrt(P.gamma * p / rho);
+                    mxd = fmax(mxd, fabs(vx) + cs);
+                    myd = fmax(myd, fabs(vy) + cs);
+                    mzd = fmax(mzd, fabs(vz) + cs);
+                } else {
+                    const float rinv = __builtin_amdgcn_rcpf(oq[0]);
+                    const float vx = oq[1] * rinv, vy = oq[2] * rinv, vz = oq[3] * rinv;
+                    const double m1 = oq[1], m2 = oq[2], m3 = oq[3];
+                    const double ek = 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) * inv64(oq[0]);
+                    double pd = g1 * ((double)oq[4] - ek);
+                    float p = pd > 1e-20 ? (float)pd : 1e-20f;
+                    if (cool) {
+                        bool cooled;
+                        const float pn = cool_fast_tab(P, sTab, oq[0], p, (float)dt, &cooled);
+                        if (cooled) {
+                            oq[4] = (float)(ek + (double)pn * (double)P.f_inv_gm1);
+                            p = pn;
+                        }
+                    }
+                    const float cs = __builtin_amdgcn_sqrtf((float)P.gamma * p * rinv);
+                    mx = fmaxf(mx, fabsf(vx) + cs);
+                    my = fmaxf(my, fabsf(vy) + cs);
+                    mz = fmaxf(mz, fabsf(vz) + cs);
+                }
+                store_col(Unew, pl, cc, ku, oq);
+            }
         }
 #pragma unroll
         for (int e = 0; e < 5; ++e) {

@@ -22,7 +22,7 @@ template <int TX_, int TY_, int ZC_, int WAVES_> struct Tile {
     static constexpr int TX = TX_, TY = TY_, ZC = ZC_, WAVES = WAVES_;
 };
 #ifndef TLQ_F32_S1
-#define TLQ_F32_S1 32, 8, 64, 3
+#define TLQ_F32_S1 32, 8, 32, 3
 #endif
 #ifndef TLQ_F32_S2
 #define TLQ_F32_S2 64, 8, 64, 1
