@@ -84,6 +84,7 @@ struct tlq_ctx {
     unsigned *cool_fqueue = nullptr; // surrogate cells queued for the fallback
     unsigned *cool_seg = nullptr;    // per segment: cells queued, cells that fell back
     unsigned *cool_fall = nullptr;
+    float *gacc = nullptr;           // FP32 point-mass acceleration field (Problem::gacc)
     // self-gravity: multigrid levels (level 0's E is PHI) and the step count
     // MultiGrid's mod(it, 10) test reads
     SgState sg;
@@ -579,6 +580,7 @@ int tlq_destroy(tlq_ctx *c) {
     for (int q = 0; q < 2; ++q)
         if (c->u[q]) hipFree(c->u[q]);
     if (c->uh) hipFree(c->uh);
+    if (c->gacc) hipFree(c->gacc);
     if (c->dprob) hipFree(c->dprob);
     if (c->st) hipFree(c->st);
     if (c->scratch_max) hipFree(c->scratch_max);
@@ -611,6 +613,27 @@ int tlq_set_winds(tlq_ctx *c, const tlq_wind *w, int n) {
     return push_problem(c);
 }
 
+// gravity_fast's point-mass acceleration of every interior cell, three
+// fields in the grid's layout (the masses do not move during a run, as
+// pointmass_potential's PHIP does not, pointsource_gravity.f90:12-39).
+__global__ void k_gravity_field(const Problem *__restrict__ Pp, float *__restrict__ out) {
+    const Problem &P = *Pp;
+    const Grid &g = P.g;
+    const long long n = (long long)g.nx * g.ny * g.nz;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % g.nx) + 1;
+        const long long r = t / g.nx;
+        const int j = (int)(r % g.ny) + 1, k = (int)(r / g.ny) + 1;
+        float a[3];
+        gravity_fast(P, i, j, k, a);
+        const long long cc = g.idx(i, j, k);
+        out[cc] = a[0];
+        out[g.nc + cc] = a[1];
+        out[2 * g.nc + cc] = a[2];
+    }
+}
+
 int tlq_set_point_masses(tlq_ctx *c, const double *xyzm, int n) {
     if (int rc = check_ctx(c)) return rc;
     if (n < 0 || n > MAX_MASSES || (n > 0 && !xyzm)) return fail(TLQ_ERR_ARG, "bad mass list");
@@ -628,7 +651,24 @@ int tlq_set_point_masses(tlq_ctx *c, const double *xyzm, int n) {
     P.g_dY = (float)(P.dy / L);
     P.g_dZ = (float)(P.dz / L);
     P.g_e2 = (float)((P.softening / L) * (P.softening / L));
-    return push_problem(c);
+    // FP32 mode: the acceleration of the fixed masses, computed once here and
+    // read by the fused step in place of gravity_fast
+    P.gacc = nullptr;
+    if (c->prec == TLQ_FP32 && n > 0) {
+        if (!c->gacc && hipMalloc(&c->gacc, 3 * (size_t)c->g.nc * sizeof(float)) != hipSuccess) {
+            c->gacc = nullptr;
+            return fail(TLQ_ERR_NOMEM, "hipMalloc failed (gravity field)");
+        }
+        P.gacc = c->gacc;
+    }
+    if (int rc = push_problem(c)) return rc;
+    if (P.gacc) {
+        const long long cells = ninterior(c->g);
+        const unsigned nb = (unsigned)std::min<long long>((cells + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_gravity_field, dim3(nb), dim3(256), 0, c->stream, c->dprob, c->gacc);
+        HIPCHK(hipGetLastError());
+    }
+    return TLQ_OK;
 }
 
 int tlq_set_cooling_table(tlq_ctx *c, const double *T, const double *L, int n) {

@@ -81,6 +81,10 @@ struct Problem {
     float g_dY, g_dZ, g_e2;
     // Self-gravity potential PHI(0:nx+1, 0:ny+1, 0:nz+1) (i fastest), or null.
     const double *phi;
+    // FP32 mode: gravity_fast's point-mass acceleration of every interior
+    // cell, three fields in the grid's layout, computed when the masses are
+    // set (they do not move), or null.
+    const float *gacc;
 };
 
 // Block barrier for LDS hand-off: waits for this wave's LDS operations only,

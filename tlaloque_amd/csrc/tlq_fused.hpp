@@ -153,6 +153,17 @@ __device__ __forceinline__ void load_col(const R *__restrict__ A, const Planes &
                                             c.off);
 }
 
+// The precomputed point-mass acceleration of cell (i, j, k) of column c
+// (FP32 mode; no load when there is no field).
+__device__ __forceinline__ void load_gacc(const float *gacc, const Planes &pl, Column c, int k,
+                                          float g[3]) {
+    if (!gacc) return;
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+        g[e] = *reinterpret_cast<const float *>(plane_ptr(reinterpret_cast<const char *>(gacc), pl, e, k) +
+                                                c.off);
+}
+
 // Stores the five fields of cell (i, j, k) of column c (which lies inside the array).
 template <class R>
 __device__ __forceinline__ void store_col(R *__restrict__ A, const Planes &pl, Column c, int k,
@@ -210,15 +221,23 @@ __device__ __forceinline__ double godunov1(R u, R fp, R fm, R gp, R gm, R hp, R 
     }
 }
 
-// Gravity source S (sources.f90:886-893) from the cell's primitives.
+// Gravity source S (sources.f90:886-893) from the cell's primitives.  In
+// FP32 mode `gpre`, when not null, holds the cell's point-mass acceleration
+// as gravity_fast gives it (read from the precomputed field).
 template <class R>
 __device__ __forceinline__ void gravity_src(const Problem &P, int i, int j, int k, const R w[5],
-                                            double s[5]) {
+                                            const float *gpre, double s[5]) {
     if constexpr (sizeof(R) == 8) {
         gravity_source(P, i, j, k, w[0], w[1], w[2], w[3], s);
     } else {
         float g[3];
-        gravity_fast(P, i, j, k, g);
+        if (gpre) {
+            g[0] = gpre[0];
+            g[1] = gpre[1];
+            g[2] = gpre[2];
+        } else {
+            gravity_fast(P, i, j, k, g);
+        }
         if (P.phi) {  // central differences of the self-gravity potential (FP64)
             g[0] += (float)((phi_at(P, i + 1, j, k) - phi_at(P, i - 1, j, k)) / (2.0 * P.dx));
             g[1] += (float)((phi_at(P, i, j + 1, k) - phi_at(P, i, j - 1, k)) / (2.0 * P.dy));
@@ -426,7 +445,9 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
                                    hp[e], hm[e], dtdx, dtdy, dtdz);
             if (grav) {
                 double s[5];
-                gravity_src<R>(P, i, j, k, wc, s);
+                // stage 1 forms the point-mass acceleration itself: its
+                // registers are at the three-waves-per-SIMD limit
+                gravity_src<R>(P, i, j, k, wc, nullptr, s);
 #pragma unroll
                 for (int e = 0; e < 5; ++e) v[e] = v[e] + dth * s[e];
             }
@@ -492,6 +513,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
     const Planes pl = planes_of<R>(g);
     const Column cc = column_of<R>(g, i, j);
     const Column ch = column_of<R>(g, x0 + ht.lx, y0 + ht.ly);
+    const float *const gacc = sizeof(R) == 4 ? P.gacc : nullptr;
 
     const double dt = dt_from_maxima(P, st->maxima[parity]);
     const double dtdx = dt / P.dx, dtdy = dt / P.dy, dtdz = dt / P.dz;
@@ -521,6 +543,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
     // at k-1/2.
     R wc[5], wp[5], uc[5], up[5], hz[5], hm[5] = {}, nm[5] = {}, nc[5];
     R pf_q[5], pf_n[5], pf_h[5] = {};
+    float pf_g[3] = {};  // point-mass acceleration of the next plane (FP32 field)
     // The march starts one plane below the tile (k = z0 - 1): that step only
     // forms the z flux through the tile's lower face, with the code of every
     // later step, so the two tiles that share the face compute the same flux.
@@ -549,6 +572,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
             np[e] = pf_n[e];
             hr[e] = pf_h[e];
         }
+        const float gk[3] = {pf_g[0], pf_g[1], pf_g[2]};
         // The plane index is block-uniform; readfirstlane lets the compiler
         // form the plane addresses in scalar registers.
         const int ku = __builtin_amdgcn_readfirstlane(k);
@@ -556,6 +580,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
             load_col(Uh, pl, cc, ku + 3, pf_q);
             load_col(U, pl, cc, ku + 2, pf_n);
             if (ht.kind) load_col(hsrc, pl, ch, ku + 1, pf_h);
+            load_gacc(gacc, pl, cc, ku + 1, pf_g);
         }
         prims(P, uq, wq);
         R hzp[5];
@@ -689,7 +714,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
                                        hm[e], dtdx, dtdy, dtdz);
                 if (grav) {
                     double s[5];
-                    gravity_src<R>(P, i, j, k, wc, s);
+                    gravity_src<R>(P, i, j, k, wc, gacc ? gk : nullptr, s);
 #pragma unroll
                     for (int e = 0; e < 5; ++e) v[e] = v[e] + dt * s[e];
                 }
