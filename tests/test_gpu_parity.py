@@ -241,3 +241,31 @@ def test_point_mass_gravity_fp32_close_to_oracle():
     assert l1_rel(got[..., 4], ref[..., 4]) <= 1e-3
     for e in (1, 2, 3):
         assert l1_rel(got[..., e], ref[..., e]) <= 1e-3
+
+
+def test_fp32_mass_field_follows_set_point_masses():
+    # In FP32 mode tlq_set_point_masses stores the masses' acceleration per
+    # cell; setting other masses (or none) must replace it: a run whose masses
+    # were changed before its first step equals a run created with the final
+    # masses, bit for bit.
+    from tlaloque_amd import hydro
+    p, u0 = _gravity_problem(P.PREC_FP32)
+    assert p.masses
+    moved = [P.PointMass(x=q.x + 0.3 * p.dx, y=q.y - 0.2 * p.dy, z=q.z, mass=2.0 * q.mass)
+             for q in p.masses]
+    lib = hydro.load_library()
+
+    def run(first, final):
+        with _core(p.replace(masses=first)) as h:
+            if final is not first:
+                m = np.ascontiguousarray(
+                    np.array([[q.x, q.y, q.z, q.mass] for q in final], dtype=np.float64).ravel())
+                ptr = hydro._dptr(m) if len(final) else None
+                assert lib.tlq_set_point_masses(h._ctx, ptr, len(final)) == 0
+            h.set_state(u0)
+            h.advance(3)
+            return h.download("U")
+
+    assert np.array_equal(run(p.masses, moved), run(moved, moved))
+    assert np.array_equal(run(p.masses, []), run([], []))
+    assert not np.array_equal(run(p.masses, p.masses), run(moved, moved))
