@@ -118,7 +118,10 @@ const char *tlq_last_error(void);
 
 /* Sources.  Winds: the spherical_wind array driven by user_boundaries
  * (user/user.f90:57-70, winds.f90:32-90).  Point masses: x, y, z, mass per
- * mass (pointsource_gravity.f90:12-39; source term sources.f90:872-897).
+ * mass (pointsource_gravity.f90:12-39; source term sources.f90:872-897); in
+ * TLQ_FP32 mode the call also stores the masses' acceleration at every cell
+ * (12 bytes per cell of device memory, allocated by the first call with
+ * n > 0), which the fused step reads.
  * Cooling table: loaddata's table (cooling.f90:21-45), T and log10 Lambda. */
 int tlq_set_winds(tlq_ctx *ctx, const tlq_wind *winds, int n);
 int tlq_set_point_masses(tlq_ctx *ctx, const double *xyzm, int n);
