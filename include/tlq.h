@@ -4,7 +4,7 @@
  * Drop-in for the hot path of tlaloque's main loop (source/mainHD.f90:61-100).
  * The reference calls implicit-interface Fortran subroutines on module arrays
  * (source/globals.f90:5-11); a Fortran caller binds these functions with
- * ISO_C_BINDING (tlaloque_amd/fortran/tlq_bindings.f90).  Every function
+ * ISO_C_BINDING (tlaloque_amd/fortran/tlq_c.f90).  Every function
  * returns 0 on success or a negative TLQ_ERR_* code; the last error message
  * is available from tlq_last_error().
  *
