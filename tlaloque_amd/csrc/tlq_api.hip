@@ -637,6 +637,15 @@ __global__ void k_gravity_field(const Problem *__restrict__ Pp, float *__restric
 int tlq_set_point_masses(tlq_ctx *c, const double *xyzm, int n) {
     if (int rc = check_ctx(c)) return rc;
     if (n < 0 || n > MAX_MASSES || (n > 0 && !xyzm)) return fail(TLQ_ERR_ARG, "bad mass list");
+    // FP32 mode: the acceleration of the fixed masses is computed once here
+    // and read by the fused step in place of gravity_fast.  The field is
+    // allocated before anything changes, so a failed call leaves the
+    // context's masses as they were.
+    const bool field = c->prec == TLQ_FP32 && n > 0;
+    if (field && !c->gacc && hipMalloc(&c->gacc, 3 * (size_t)c->g.nc * sizeof(float)) != hipSuccess) {
+        c->gacc = nullptr;
+        return fail(TLQ_ERR_NOMEM, "hipMalloc failed (gravity field)");
+    }
     Problem &P = c->host;
     for (int q = 0; q < 4 * n; ++q) P.masses[q] = xyzm[q];
     P.nmasses = n;
@@ -651,16 +660,7 @@ int tlq_set_point_masses(tlq_ctx *c, const double *xyzm, int n) {
     P.g_dY = (float)(P.dy / L);
     P.g_dZ = (float)(P.dz / L);
     P.g_e2 = (float)((P.softening / L) * (P.softening / L));
-    // FP32 mode: the acceleration of the fixed masses, computed once here and
-    // read by the fused step in place of gravity_fast
-    P.gacc = nullptr;
-    if (c->prec == TLQ_FP32 && n > 0) {
-        if (!c->gacc && hipMalloc(&c->gacc, 3 * (size_t)c->g.nc * sizeof(float)) != hipSuccess) {
-            c->gacc = nullptr;
-            return fail(TLQ_ERR_NOMEM, "hipMalloc failed (gravity field)");
-        }
-        P.gacc = c->gacc;
-    }
+    P.gacc = field ? c->gacc : nullptr;
     if (int rc = push_problem(c)) return rc;
     if (P.gacc) {
         const long long cells = ninterior(c->g);
