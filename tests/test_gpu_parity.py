@@ -269,3 +269,33 @@ def test_fp32_mass_field_follows_set_point_masses():
     assert np.array_equal(run(p.masses, moved), run(moved, moved))
     assert np.array_equal(run(p.masses, []), run([], []))
     assert not np.array_equal(run(p.masses, p.masses), run(moved, moved))
+
+
+@pytest.mark.parametrize("precision", [P.PREC_FP64, P.PREC_FP32])
+def test_rejected_setters_leave_the_context_unchanged(precision):
+    # A cooling table whose temperatures stop increasing, or a mass list that
+    # is too long, is refused with TLQ_ERR_ARG; the context keeps its table
+    # and masses, so a run after the refused calls (and a later setter that
+    # re-sends the problem to the device) equals an undisturbed run bit for bit.
+    from tlaloque_amd import hydro, problems
+    TLQ_ERR_ARG = -1                                          # include/tlq.h
+    p, u0 = problems.clumpy(n=24, precision=precision, temp=3e5)
+    lib = hydro.load_library()
+    T, L = (np.ascontiguousarray(a, dtype=np.float64) for a in cooling_table())
+    bad_T, bad_L = T.copy() * 1.5, L.copy() + 1.0
+    bad_T[-1] = bad_T[-2]
+    m = np.array([[q.x, q.y, q.z, q.mass] for q in p.masses], dtype=np.float64).ravel()
+    many = np.zeros(4 * 1024, dtype=np.float64)
+
+    def run(refuse):
+        with _core(p) as h:
+            if refuse:
+                assert lib.tlq_set_cooling_table(h._ctx, hydro._dptr(bad_T), hydro._dptr(bad_L),
+                                                 len(bad_T)) == TLQ_ERR_ARG
+                assert lib.tlq_set_point_masses(h._ctx, hydro._dptr(many), 1024) == TLQ_ERR_ARG
+                assert lib.tlq_set_point_masses(h._ctx, hydro._dptr(m), len(p.masses)) == 0
+            h.set_state(u0)
+            h.advance(3)
+            return h.download("U")
+
+    assert np.array_equal(run(True), run(False))

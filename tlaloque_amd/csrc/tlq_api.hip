@@ -674,9 +674,11 @@ int tlq_set_point_masses(tlq_ctx *c, const double *xyzm, int n) {
 int tlq_set_cooling_table(tlq_ctx *c, const double *T, const double *L, int n) {
     if (int rc = check_ctx(c)) return rc;
     if (n < 2 || n > MAX_TABLE || !T || !L) return fail(TLQ_ERR_ARG, "bad cooling table");
-    for (int q = 0; q < n; ++q) {
+    // validate the whole table first: a rejected table leaves the current one intact
+    for (int q = 0; q < n; ++q)
         if (!(T[q] > 0.0) || (q > 0 && !(T[q] > T[q - 1])))
             return fail(TLQ_ERR_ARG, "cooling table temperatures must increase");
+    for (int q = 0; q < n; ++q) {
         c->host.tab_T[q] = T[q];
         c->host.tab_logT[q] = std::log10(T[q]);
         c->host.tab_L[q] = L[q];
