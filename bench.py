@@ -185,6 +185,20 @@ def self_gravity_leg(p, u0, table, warmup, steps):
     return out
 
 
+def host_cpu() -> dict:
+    """The host's CPU model and logical CPU count (Linux /proc/cpuinfo)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "logical_cpus": os.cpu_count()}
+
+
 def cpu_baseline(n: int, steps: int, threads: int):
     """The FP64 C oracle on the same workload at n^3 (bounded sample)."""
     import oracle
@@ -338,6 +352,7 @@ def main():
                                   "kind": "port",
                                   "sample": f"FP64 C oracle (OpenMP, {threads} threads), same "
                                             f"clumpy workload at 256^3, 20 steps ({el:.1f} s)",
+                                  "host": host_cpu(),
                                   "single_thread": {"value": zps1, "cores": 1,
                                                     "sample": f"same oracle, 1 thread, clumpy 128^3, "
                                                               f"4 steps ({el1:.1f} s)"}}
