@@ -126,7 +126,8 @@ def test_blast_against_sedov(blasts, solver, precision):
     print(f"blast solver={solver} prec={precision} t={t:.5f} it={it}: R_sedov={R:.4f} "
           f"r_peak={r_peak:.4f} shell L1(rho)={l1:.4f} peak rho={rho.max():.3f}")
     assert abs(r_peak / R - 1.0) < 0.05
-    assert l1 < 0.3
+    # measured 0.060-0.080 over the four solver/precision runs
+    assert l1 < 0.1
     assert vol > 0
 
 
@@ -200,8 +201,10 @@ def test_clumpy_512_fp32_l1_against_fp64():
     workload at its full size (clumpy 512^3 with exact cooling and the point
     mass), at matched step counts.  The FP64 path equals the oracle bit for bit
     (or to 1e-10 with cooling) on the reference fixtures; at this size it
-    stands in for the oracle, which would take hours on the CPU.  The bound is
-    the target's 1e-3."""
+    stands in for the oracle, which would take hours on the CPU.  Measured:
+    rho 6.8e-8, p 5.8e-7, |v| 6.3e-8 (the FP32 storage of the initial state
+    alone gives a few 1e-8).  The bounds are about ten times those, so that an
+    error confined to the cells at tile faces (1/32 of the mesh) shows."""
     from helpers import cooling_table
     p, u0 = problems.clumpy(n=512)
     table = cooling_table()
@@ -217,4 +220,4 @@ def test_clumpy_512_fp32_l1_against_fp64():
     l1 = {"rho": l1_rel(a[..., 0], b[..., 0]), "p": l1_rel(a[..., 4], b[..., 4]),
           "v": l1_rel(np.sqrt((a[..., 1:4] ** 2).sum(-1)), np.sqrt((b[..., 1:4] ** 2).sum(-1)))}
     print(f"clumpy 512^3, 5 steps, FP32 vs FP64: L1 {l1}")
-    assert all(v <= 1e-3 for v in l1.values()), l1
+    assert l1["rho"] <= 1e-6 and l1["v"] <= 1e-6 and l1["p"] <= 5e-6, l1

@@ -1,0 +1,179 @@
+"""GPU parity across the fused kernels' tile faces.
+
+The fused step cuts the mesh into tiles (FP64: 32 x 8 columns by 64 planes in
+both kernels; FP32: 32 x 8 x 32 in stage 1, 64 x 8 x 64 in stage 2) and every
+face between two tiles is computed by both of them.  The reference's flux
+loops run over every face alike (HLLC.f90:94-96, 223-225), so each tile face
+must carry the same flux as an interior face.  These meshes have at least two
+whole tiles and a partial one on every axis of both kernels, mixed boundary
+kinds, a wind whose sphere straddles a tile corner (boundaries then
+user_boundaries, boundaries.f90:11-24), tabulated cooling and a softened
+point mass; the GPU is compared with the FP64 oracle (the reference
+restatement, bit for bit on the reference's own runs).
+
+Bars: FP64 fused and per-routine paths, max relative difference per field
+<= 1e-10 (device exp/log10/pow in the cooling differ from glibc in the last
+bits); FP32 path, L1 <= 1e-3 for rho, p and |v| over the whole mesh and over
+the cells that touch a tile face alone.
+"""
+import numpy as np
+import pytest
+
+from helpers import cooling_table, interior, l1_rel, max_rel, prims_from_cons
+from tlaloque_amd import params as P
+from tlaloque_amd.hydro import HydroCore
+
+pytestmark = pytest.mark.gpu
+
+DX = 1.0e17  # cm (cubic cells)
+
+
+def tile_problem(nx, ny, nz, wind_cell, precision, seed=3):
+    """Clumpy cgs medium (n ~ 0.3-3 cm^-3, T ~ 1e3-1e7 K, |v| ~ 20 km/s),
+    reflective|outflow in x, periodic in y, outflow|reflective in z; one wind
+    centred on the tile corner at cell faces wind_cell (x, y) and 0.3 cells
+    above the z face wind_cell[2]; a 1e4 Msun point mass softened over two
+    cells."""
+    wx, wy, wz = wind_cell
+    w = P.Wind(xc=wx * DX, yc=wy * DX, zc=(wz + 0.3) * DX, vx=2e5, vy=-1e5, vz=0.0,
+               radius=3.5 * DX, mdot=1e-5 * P.MSUN / P.YEAR, vinf=300.0 * P.KMS, temp=1e4,
+               mu=1.0)
+    m = P.PointMass(x=(wx + 9.25) * DX, y=(wy + 3.5) * DX, z=(wz - 6.0) * DX, mass=1e4 * P.MSUN)
+    p = P.HydroParams(nx=nx, ny=ny, nz=nz, xl=0.0, xr=nx * DX, yl=0.0, yr=ny * DX, zl=0.0,
+                      zr=nz * DX, gamma=5.0 / 3.0, cfl=0.4, eta=0.05,
+                      bc=(P.BC_REFLECTIVE, P.BC_OUTFLOW, P.BC_PERIODIC, P.BC_PERIODIC,
+                          P.BC_OUTFLOW, P.BC_REFLECTIVE),
+                      precision=precision, cooling=P.COOL_EXACT, winds=[w], masses=[m],
+                      softening=2.0 * DX)
+    rng = np.random.default_rng(seed)
+    z, y, x = np.meshgrid(*(np.arange(n + 4, dtype=np.float64) for n in (nz, ny, nx)),
+                          indexing="ij")
+    smooth = (np.sin(2 * np.pi * x / 23.0 + 0.4) * np.cos(2 * np.pi * y / 11.0)
+              + np.sin(2 * np.pi * z / 31.0 + 1.3))
+    nd = 10 ** (0.25 * smooth + 0.15 * rng.uniform(-1, 1, x.shape))
+    temp = 10 ** (5.0 + 0.9 * smooth + 0.4 * rng.uniform(-1, 1, x.shape))
+    vel = 2e6 * (0.5 * rng.uniform(-1, 1, x.shape + (3,))
+                 + np.stack([np.sin(2 * np.pi * y / 17.0), np.cos(2 * np.pi * z / 13.0),
+                             np.sin(2 * np.pi * x / 29.0)], -1))
+    rho = nd * p.mu0 * P.AMU
+    pres = nd * P.KB * temp
+    u = np.empty(p.shape + (5,))
+    u[..., 0] = rho
+    u[..., 1:4] = rho[..., None] * vel
+    u[..., 4] = 0.5 * rho * (vel ** 2).sum(-1) + pres / (p.gamma - 1.0)
+    return p, u
+
+
+# FP64: tiles 32 x 8 x 64 in both kernels -> 72 x 20 x 136 is 32+32+8, 8+8+4, 64+64+8
+FP64_MESH = (72, 20, 136)
+FP64_WIND = (32, 8, 64)
+# FP32: stage 1 32 x 8 x 32, stage 2 64 x 8 x 64 -> 136 x 20 x 72 is 4x32+8 and
+# 2x64+8 in x, 8+8+4 in y, 2x32+8 and 64+8 in z
+FP32_MESH = (136, 20, 72)
+FP32_WIND = (64, 8, 32)
+STEPS = 3
+
+
+def _oracle(p, u0, n):
+    import oracle
+    o = oracle.Oracle(p.replace(precision=P.PREC_FP64), table=cooling_table())
+    o.set_cons(u0)
+    for _ in range(n):
+        o.advance()
+    return o
+
+
+def _face_cells(n, tile):
+    """Interior indices (0-based within the interior) of the cells on either
+    side of every face between two tiles of width `tile`."""
+    idx = []
+    for f in range(tile, n, tile):
+        idx += [f - 1, f]
+    return np.array(idx)
+
+
+def test_tile_problem_exercises_the_sources():
+    # the mesh sizes make partial tiles on every axis, the wind sphere reaches
+    # cells of all eight tiles around its corner, and the cooling acts
+    p, u = tile_problem(*FP64_MESH, FP64_WIND, P.PREC_FP64)
+    for n, t in zip(FP64_MESH, (32, 8, 64)):
+        assert n // t >= 2 and n % t
+    o = _oracle(p, u, 1)
+    w0 = prims_from_cons(u, p.gamma)
+    T0 = (w0[..., 4] / w0[..., 0])[2:-2, 2:-2, 2:-2]
+    assert (T0 > 1e4 * P.KB / P.AMU).mean() > 0.5      # most cells above the cooling gate
+    wx, wy, wz = FP64_WIND
+    # wind cells in the tiles on both sides of each face of the corner
+    rho1 = o.prim[..., 0]
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx_ in (0, 1):
+                i, j, k = wx + dx_, wy + dy, wz + dz   # 1-based cells at the corner
+                assert rho1[k + 1, j + 1, i + 1] > 10 * np.median(rho1)
+
+
+def test_fused_fp64_matches_oracle_across_tiles():
+    p, u0 = tile_problem(*FP64_MESH, FP64_WIND, P.PREC_FP64)
+    o = _oracle(p, u0, STEPS)
+    with HydroCore(p, cooling_table=cooling_table()) as h:
+        h.set_state(u0)
+        h.advance(STEPS)
+        dt, t, it = h.time()
+        u = h.download("U")
+    assert it == STEPS and t == pytest.approx(o.t, rel=1e-12)
+    err = max_rel(interior(prims_from_cons(u, p.gamma)), interior(o.prim))
+    print(f"fused FP64 72x20x136: max rel {err:.3e}")
+    assert err <= 1e-10
+
+
+def test_routines_fp64_match_oracle_across_tiles():
+    p, u0 = tile_problem(*FP64_MESH, FP64_WIND, P.PREC_FP64)
+    o = _oracle(p, u0, 2)
+    with HydroCore(p, cooling_table=cooling_table()) as h:
+        h.upload("U", u0)
+        h.borders("U", 1)
+        h.calc_prims("U")
+        t, it = 0.0, 0
+        for _ in range(2):
+            dt = h.time_step()
+            h.fluxes("U", 1)
+            h.godunov("UPP", dt / 2.0)
+            h.borders("UPP", 2)
+            h.calc_prims("UPP")
+            h.fluxes("UPP", 2)
+            h.godunov("UP", dt)
+            h.borders("UP", 1)
+            t, it = h.step(t, it, dt)
+            h.calc_prims("U")
+            h.cooling(dt)
+        w = h.download("UPrim")
+    err = max_rel(interior(w), interior(o.prim))
+    print(f"per-routine FP64 72x20x136: max rel {err:.3e}")
+    assert err <= 1e-10
+
+
+def test_fused_fp32_close_to_oracle_across_tiles():
+    p, u0 = tile_problem(*FP32_MESH, FP32_WIND, P.PREC_FP32)
+    o = _oracle(p, u0, STEPS)
+    with HydroCore(p, cooling_table=cooling_table()) as h:
+        h.set_state(u0)
+        h.advance(STEPS)
+        got = interior(prims_from_cons(h.download("U"), p.gamma))
+        assert h.check_state() == 0
+    ref = interior(o.prim)
+    vg = np.sqrt((got[..., 1:4] ** 2).sum(-1))
+    vr = np.sqrt((ref[..., 1:4] ** 2).sum(-1))
+    nx, ny, nz = FP32_MESH
+    # cells touching a face between two tiles of either kernel, per axis
+    fx = np.union1d(_face_cells(nx, 32), _face_cells(nx, 64))
+    fy = _face_cells(ny, 8)
+    fz = np.union1d(_face_cells(nz, 32), _face_cells(nz, 64))
+    sel = np.zeros(got.shape[:3], dtype=bool)
+    sel[:, :, fx] = True
+    sel[:, fy, :] = True
+    sel[fz, :, :] = True
+    for name, a, b in (("rho", got[..., 0], ref[..., 0]), ("p", got[..., 4], ref[..., 4]),
+                       ("|v|", vg, vr)):
+        whole, faces = l1_rel(a, b), l1_rel(a[sel], b[sel])
+        print(f"FP32 136x20x72 L1 {name}: all {whole:.3e}, tile-face cells {faces:.3e}")
+        assert whole <= 1e-3 and faces <= 1e-3, name

@@ -51,6 +51,7 @@ constexpr int kNumKernels = 6;
 struct tlq_ctx {
     tlq_params prm{};
     Problem host{};            // host copy of the problem description
+    Problem staged{};          // a setter's new problem while it is sent to the device
     Problem *dprob = nullptr;  // device copy
     Grid g{};
     hipStream_t stream = nullptr;
@@ -124,6 +125,26 @@ int ensure_buf(tlq_ctx *c, int b) {
 int push_problem(tlq_ctx *c) {
     HIPCHK(hipMemcpyAsync(c->dprob, &c->host, sizeof(Problem), hipMemcpyHostToDevice, c->stream));
     return TLQ_OK;
+}
+
+// Makes `np` the context's problem: copies it to the device and, once that
+// is enqueued, to the host copy.  On failure both copies stay as they were.
+int commit_problem(tlq_ctx *c, const Problem &np) {
+    c->staged = np;
+    const hipError_t e = hipMemcpyAsync(c->dprob, &c->staged, sizeof(Problem),
+                                        hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) {
+        (void)push_problem(c);  // best effort: the device copy back to the host's
+        return fail(TLQ_ERR_HIP, std::string("problem upload: ") + hipGetErrorString(e));
+    }
+    c->host = c->staged;
+    return TLQ_OK;
+}
+
+bool all_finite(const double *a, int n) {
+    for (int q = 0; q < n; ++q)
+        if (!std::isfinite(a[q])) return false;
+    return true;
 }
 
 template <class R> const R *cp(void *p) { return static_cast<const R *>(p); }
@@ -604,13 +625,16 @@ int tlq_destroy(tlq_ctx *c) {
 int tlq_set_winds(tlq_ctx *c, const tlq_wind *w, int n) {
     if (int rc = check_ctx(c)) return rc;
     if (n < 0 || n > MAX_WINDS || (n > 0 && !w)) return fail(TLQ_ERR_ARG, "bad wind list");
+    for (int q = 0; q < n; ++q)
+        if (!all_finite(&w[q].xc, 11)) return fail(TLQ_ERR_ARG, "non-finite wind parameter");
+    Problem np = c->host;
     for (int q = 0; q < n; ++q) {
         const tlq_wind &s = w[q];
-        c->host.winds[q] = WindDev{s.xc, s.yc, s.zc, s.vx, s.vy, s.vz, s.radius, s.mdot, s.vinf,
-                                   s.temp, s.mu};
+        np.winds[q] = WindDev{s.xc, s.yc, s.zc, s.vx, s.vy, s.vz, s.radius, s.mdot, s.vinf,
+                              s.temp, s.mu};
     }
-    c->host.nwinds = n;
-    return push_problem(c);
+    np.nwinds = n;
+    return commit_problem(c, np);
 }
 
 // gravity_fast's point-mass acceleration of every interior cell, three
@@ -637,36 +661,42 @@ __global__ void k_gravity_field(const Problem *__restrict__ Pp, float *__restric
 int tlq_set_point_masses(tlq_ctx *c, const double *xyzm, int n) {
     if (int rc = check_ctx(c)) return rc;
     if (n < 0 || n > MAX_MASSES || (n > 0 && !xyzm)) return fail(TLQ_ERR_ARG, "bad mass list");
+    if (!all_finite(xyzm, 4 * n)) return fail(TLQ_ERR_ARG, "non-finite mass position or value");
     // FP32 mode: the acceleration of the fixed masses is computed once here
     // and read by the fused step in place of gravity_fast.  The field is
-    // allocated before anything changes, so a failed call leaves the
-    // context's masses as they were.
+    // allocated and the new problem built before anything changes; a call
+    // that fails leaves the context's masses and field as they were.
     const bool field = c->prec == TLQ_FP32 && n > 0;
     if (field && !c->gacc && hipMalloc(&c->gacc, 3 * (size_t)c->g.nc * sizeof(float)) != hipSuccess) {
         c->gacc = nullptr;
         return fail(TLQ_ERR_NOMEM, "hipMalloc failed (gravity field)");
     }
-    Problem &P = c->host;
-    for (int q = 0; q < 4 * n; ++q) P.masses[q] = xyzm[q];
-    P.nmasses = n;
+    Problem np = c->host;
+    for (int q = 0; q < 4 * n; ++q) np.masses[q] = xyzm[q];
+    np.nmasses = n;
     // FP32 gravity in units of L = dx (see gravity_fast)
-    const double L = P.dx;
+    const double L = np.dx;
     for (int m = 0; m < n; ++m) {
-        P.g_A[m] = (float)(2.0 * GR * xyzm[4 * m + 3] / (L * L));
-        P.g_X0[m] = (float)((P.xl - P.dx / 2.0 - xyzm[4 * m + 0]) / L);
-        P.g_Y0[m] = (float)((P.yl - P.dy / 2.0 - xyzm[4 * m + 1]) / L);
-        P.g_Z0[m] = (float)((P.zl - P.dz / 2.0 - xyzm[4 * m + 2]) / L);
+        np.g_A[m] = (float)(2.0 * GR * xyzm[4 * m + 3] / (L * L));
+        np.g_X0[m] = (float)((np.xl - np.dx / 2.0 - xyzm[4 * m + 0]) / L);
+        np.g_Y0[m] = (float)((np.yl - np.dy / 2.0 - xyzm[4 * m + 1]) / L);
+        np.g_Z0[m] = (float)((np.zl - np.dz / 2.0 - xyzm[4 * m + 2]) / L);
     }
-    P.g_dY = (float)(P.dy / L);
-    P.g_dZ = (float)(P.dz / L);
-    P.g_e2 = (float)((P.softening / L) * (P.softening / L));
-    P.gacc = field ? c->gacc : nullptr;
-    if (int rc = push_problem(c)) return rc;
-    if (P.gacc) {
+    np.g_dY = (float)(np.dy / L);
+    np.g_dZ = (float)(np.dz / L);
+    np.g_e2 = (float)((np.softening / L) * (np.softening / L));
+    np.gacc = field ? c->gacc : nullptr;
+    const Problem old = c->host;
+    if (int rc = commit_problem(c, np)) return rc;
+    if (np.gacc) {
         const long long cells = ninterior(c->g);
         const unsigned nb = (unsigned)std::min<long long>((cells + 255) / 256, 65536);
         hipLaunchKernelGGL(k_gravity_field, dim3(nb), dim3(256), 0, c->stream, c->dprob, c->gacc);
-        HIPCHK(hipGetLastError());
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            (void)commit_problem(c, old);
+            return fail(TLQ_ERR_HIP, std::string("gravity field: ") + hipGetErrorString(e));
+        }
     }
     return TLQ_OK;
 }
@@ -676,16 +706,18 @@ int tlq_set_cooling_table(tlq_ctx *c, const double *T, const double *L, int n) {
     if (n < 2 || n > MAX_TABLE || !T || !L) return fail(TLQ_ERR_ARG, "bad cooling table");
     // validate the whole table first: a rejected table leaves the current one intact
     for (int q = 0; q < n; ++q)
-        if (!(T[q] > 0.0) || (q > 0 && !(T[q] > T[q - 1])))
-            return fail(TLQ_ERR_ARG, "cooling table temperatures must increase");
+        if (!(T[q] > 0.0) || !std::isfinite(T[q]) || (q > 0 && !(T[q] > T[q - 1])))
+            return fail(TLQ_ERR_ARG, "cooling table temperatures must be finite and increase");
+    if (!all_finite(L, n)) return fail(TLQ_ERR_ARG, "non-finite log10 Lambda in the cooling table");
+    Problem np = c->host;
+    Problem &P = np;
     for (int q = 0; q < n; ++q) {
-        c->host.tab_T[q] = T[q];
-        c->host.tab_logT[q] = std::log10(T[q]);
-        c->host.tab_L[q] = L[q];
+        P.tab_T[q] = T[q];
+        P.tab_logT[q] = std::log10(T[q]);
+        P.tab_L[q] = L[q];
     }
-    c->host.ntable = n;
-    c->host.tmin = T[0];
-    Problem &P = c->host;
+    P.ntable = n;
+    P.tmin = T[0];
     for (int q = 0; q < n; ++q) {
         P.tab_Tf[q] = (float)T[q];
         P.tab_lgTf[q] = (float)P.tab_logT[q];
@@ -715,7 +747,7 @@ int tlq_set_cooling_table(tlq_ctx *c, const double *T, const double *L, int n) {
     P.tab_uniform = dev < 0.4 * d ? 1 : 0;
     P.f_lgT0 = (float)P.tab_logT[0];
     P.f_inv_dlg = (float)(1.0 / d);
-    return push_problem(c);
+    return commit_problem(c, np);
 }
 
 int tlq_upload(tlq_ctx *c, int buf, const double *host) {
