@@ -221,6 +221,21 @@ int tlq_surrogate_eval(tlq_ctx *ctx, const float *x, long long n, float *out);
  * the network on the n samples (x, y) given. */
 int tlq_surrogate_grad(tlq_ctx *ctx, const float *x, const float *y, int n, double *grad,
                        double *loss);
+/* Allocation-free variant of tlq_surrogate_grad.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight training loops).  x (n triples), y (n
+ * targets), grad (TLQ_MLP_NPARAMS floats) and loss (1 float) are DEVICE
+ * pointers; the call only enqueues two kernels on the context's stream
+ * (read the results after tlq_sync) and allocates nothing: the partial
+ * gradients go to a static scratch array of 256 blocks x (TLQ_MLP_NPARAMS + 1)
+ * floats (4.6 MB) that each GPU holds until the process ends, so n is at
+ * most TLQ_NOALLOC_MAX_SAMPLES (TLQ_ERR_ARG otherwise).  Every call on one
+ * GPU shares that array (each call rewrites all it reads): calls from
+ * contexts whose streams overlap on one GPU race.  Results equal
+ * tlq_surrogate_grad's for the same samples, bit for bit. */
+#define TLQ_NOALLOC_MAX_SAMPLES 16384
+int tlq_surrogate_grad_noalloc(tlq_ctx *ctx, const float *x, const float *y, int n, float *grad,
+                               float *loss);
 
 /* Timing: HIP events on the context's stream.  tlq_timer_* brackets any
  * enqueued work; with kernel timing on, every kernel launch of the fused step

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -86,6 +87,7 @@ struct tlq_ctx {
     unsigned *cool_seg = nullptr;    // per segment: cells queued, cells that fell back
     unsigned *cool_fall = nullptr;
     float *gacc = nullptr;           // FP32 point-mass acceleration field (Problem::gacc)
+    float *noalloc_scratch = nullptr;  // this device's copy of `scratch` (tlq_surrogate_grad_noalloc)
     // self-gravity: multigrid levels (level 0's E is PHI) and the step count
     // MultiGrid's mod(it, 10) test reads
     SgState sg;
@@ -93,7 +95,38 @@ struct tlq_ctx {
     int it_host = 0;
 };
 
+// Scratch of tlq_surrogate_grad_noalloc: the partial gradients (and loss) of
+// up to TLQ_NOALLOC_MAX_SAMPLES / 64 blocks of the gradient kernel, in static
+// device memory (4.6 MB on every GPU the library runs on, held until the
+// process ends).  Host code never passes the array's name to a kernel: it is
+// a host-side stand-in, not a device address.  Its address on each device is
+// looked up once with hipGetSymbolAddress, with that device current, when the
+// first context is created there, and kept in g_scratch_addr.
+constexpr int kNoallocBlocks = TLQ_NOALLOC_MAX_SAMPLES / 64;
+static __device__ __align__(256) float scratch[kNoallocBlocks * (mlp::NP + 1)];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_addr[kMaxDevices] = {};
+
+// The address of `scratch` on the current device (looked up on first use).
+hipError_t scratch_address(float **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void *ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_addr[dev] = static_cast<float *>(ptr);
+    }
+    *out = g_scratch_addr[dev];
+    return hipSuccess;
+}
 
 bool valid_bc(int b) { return b == TLQ_BC_OUTFLOW || b == TLQ_BC_REFLECTIVE || b == TLQ_BC_PERIODIC; }
 
@@ -565,6 +598,8 @@ int tlq_create(const tlq_params *p, tlq_ctx **out) {
     }
     StepState s0{};
     hipMemcpyAsync(c->st, &s0, sizeof s0, hipMemcpyHostToDevice, c->stream);
+    if (scratch_address(&c->noalloc_scratch) != hipSuccess)
+        return bail(TLQ_ERR_HIP, "hipGetSymbolAddress failed for the allocation-free scratch");
     hipEventCreate(&c->ev0);
     hipEventCreate(&c->ev1);
 
@@ -1152,6 +1187,25 @@ int tlq_surrogate_grad(tlq_ctx *c, const float *x, const float *y, int n, double
     if (loss) *loss = l;
     (void)hipFree(dx);
     (void)hipFree(dy);
+    return TLQ_OK;
+}
+
+// tlq_surrogate_grad on device arrays, with its partial gradients in the
+// static scratch of this device: no allocation, no synchronisation.
+int tlq_surrogate_grad_noalloc(tlq_ctx *c, const float *x, const float *y, int n, float *grad,
+                               float *loss) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!c->sur_ready) return fail(TLQ_ERR_STATE, "surrogate has no parameters");
+    if (!x || !y || !grad || !loss) return fail(TLQ_ERR_ARG, "null device pointer");
+    if (n < 1 || n > TLQ_NOALLOC_MAX_SAMPLES)
+        return fail(TLQ_ERR_ARG, "tlq_surrogate_grad_noalloc takes 1 to TLQ_NOALLOC_MAX_SAMPLES "
+                                 "(16384) samples: its scratch holds 256 blocks of 64");
+    if (!c->noalloc_scratch) return fail(TLQ_ERR_STATE, "no scratch address on this device");
+    // the block count of tlq_surrogate_grad, so both give the same sums
+    const int nb = (n + 63) / 64;
+    HIPCHK(sur_grad(c->stream, c->sur_thq, x, y, 0, n, c->noalloc_scratch, nb));
+    HIPCHK(sur_adam(c->stream, c->noalloc_scratch, nb, nullptr, nullptr, grad, nullptr, 0, 0.f,
+                    loss));
     return TLQ_OK;
 }
 

@@ -48,6 +48,7 @@ EXPORTS = (
     "tlq_kernel_stats", "tlq_layout", "tlq_copy_probe", "tlq_surrogate_init",
     "tlq_surrogate_set_params", "tlq_surrogate_get_params", "tlq_surrogate_generate",
     "tlq_surrogate_samples", "tlq_surrogate_train", "tlq_surrogate_eval", "tlq_surrogate_grad",
+    "tlq_surrogate_grad_noalloc",
 )
 MLP_NPARAMS = 4481
 
