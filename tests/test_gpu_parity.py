@@ -12,8 +12,8 @@ runs the same problems through the C-ABI:
 Tolerances: FP64 builds keep the reference's operation order without FMA
 contraction, so results agree to a few ulps except where device libm (exp,
 log10, pow in the cooling) differs from glibc: max relative difference per
-field <= 1e-12 (1e-10 with cooling).  FP32: L1 (sum|d|/sum|ref|) <= 1e-3 for
-rho and p, the target of the hydro path.
+field <= 1e-12 (1e-10 with cooling).  FP32: L1 (sum|d|/sum|ref|) of rho and
+p <= 1e-5, a hundredth of the hydro path's 1e-3 target.
 """
 import numpy as np
 import pytest
@@ -95,9 +95,12 @@ def test_fused_fp32_close_to_reference(case):
         assert h.check_state() == 0
     ref = interior(prims[-1])
     got = interior(w)
-    # LIMITER_NONE reconstructs white noise without limiting: the scheme
-    # amplifies rounding differences there, so its FP32 bar is 1e-2.
-    tol = 1e-2 if p.limiter == P.LIMITER_NONE else 1e-3
+    # Measured 3e-8 to 8e-8 for rho and up to 1.8e-6 for p (bc_mixed).  The
+    # bound is well below the 1e-3 target so that a discrete difference
+    # shows: before the FP32 contact speed kept the reference's exact S* = 0
+    # at reflective walls, FP32 picked the other HLLC star state there and
+    # this case measured 7e-3 (LIMITER_NONE) and 7e-5 (bc_mixed).
+    tol = 1e-5
     l1r, l1p = l1_rel(got[..., 0], ref[..., 0]), l1_rel(got[..., 4], ref[..., 4])
     print(f"{case}: FP32 vs reference L1 rho {l1r:.3e} p {l1p:.3e}")
     assert l1r <= tol, case

@@ -12,9 +12,11 @@ point mass; the GPU is compared with the FP64 oracle (the reference
 restatement, bit for bit on the reference's own runs).
 
 Bars: FP64 fused and per-routine paths, max relative difference per field
-<= 1e-10 (device exp/log10/pow in the cooling differ from glibc in the last
-bits); FP32 path, L1 <= 1e-3 for rho, p and |v| over the whole mesh and over
-the cells that touch a tile face alone.
+<= 1e-12 (measured 2.8e-15 and 1.2e-16: the device's exp/log10/pow in the
+cooling may differ from glibc's in the last bits); FP32 path, L1 of rho, p
+and |v| over the whole mesh and over the cells that touch a tile face alone,
+about ten times the measured 6.6e-8, 4.5e-7 and 1.3e-7 (the target is 1e-3:
+these bounds catch an error confined to the tile-face cells).
 """
 import numpy as np
 import pytest
@@ -123,7 +125,7 @@ def test_fused_fp64_matches_oracle_across_tiles():
     assert it == STEPS and t == pytest.approx(o.t, rel=1e-12)
     err = max_rel(interior(prims_from_cons(u, p.gamma)), interior(o.prim))
     print(f"fused FP64 72x20x136: max rel {err:.3e}")
-    assert err <= 1e-10
+    assert err <= 1e-12
 
 
 def test_routines_fp64_match_oracle_across_tiles():
@@ -149,7 +151,7 @@ def test_routines_fp64_match_oracle_across_tiles():
         w = h.download("UPrim")
     err = max_rel(interior(w), interior(o.prim))
     print(f"per-routine FP64 72x20x136: max rel {err:.3e}")
-    assert err <= 1e-10
+    assert err <= 1e-12
 
 
 def test_fused_fp32_close_to_oracle_across_tiles():
@@ -172,8 +174,8 @@ def test_fused_fp32_close_to_oracle_across_tiles():
     sel[:, :, fx] = True
     sel[:, fy, :] = True
     sel[fz, :, :] = True
-    for name, a, b in (("rho", got[..., 0], ref[..., 0]), ("p", got[..., 4], ref[..., 4]),
-                       ("|v|", vg, vr)):
+    for name, a, b, bound in (("rho", got[..., 0], ref[..., 0], 1e-6),
+                              ("p", got[..., 4], ref[..., 4], 5e-6), ("|v|", vg, vr, 2e-6)):
         whole, faces = l1_rel(a, b), l1_rel(a[sel], b[sel])
         print(f"FP32 136x20x72 L1 {name}: all {whole:.3e}, tile-face cells {faces:.3e}")
-        assert whole <= 1e-3 and faces <= 1e-3, name
+        assert whole <= bound and faces <= bound, name

@@ -227,12 +227,24 @@ __device__ __forceinline__ void wavespeed(R gamma, const R wl[5], const R wr[5],
     if constexpr (sizeof(R) == 8) {
         sl = rmin(wl[n] - csl, wr[n] - csr);
         sr = rmax(wl[n] + csl, wr[n] + csr);
+        sst = rdiv(wr[4] - wl[4] + wl[0] * wl[n] * (sl - wl[n]) - wr[0] * wr[n] * (sr - wr[n]),
+                   wl[0] * (sl - wl[n]) - wr[0] * (sr - wr[n]));
     } else {  // the hardware min/max (equal for the finite values)
         sl = fminf(wl[n] - csl, wr[n] - csr);
         sr = fmaxf(wl[n] + csl, wr[n] + csr);
+        // The numerator in the reference's order with no contraction: at a
+        // face between mirror-image states (a reflective wall) its two
+        // momentum terms are then rounded alike and cancel exactly, S* = 0 as
+        // in the reference, and the region choice at S* = 0 (the left star
+        // state) is the reference's.  A fused multiply-add rounds one term
+        // and not the other, and S* = +-1 ulp picks either side.
+        R num;
+        {
+#pragma clang fp contract(off)
+            num = wr[4] - wl[4] + wl[0] * wl[n] * (sl - wl[n]) - wr[0] * wr[n] * (sr - wr[n]);
+        }
+        sst = rdiv(num, wl[0] * (sl - wl[n]) - wr[0] * (sr - wr[n]));
     }
-    sst = rdiv(wr[4] - wl[4] + wl[0] * wl[n] * (sl - wl[n]) - wr[0] * wr[n] * (sr - wr[n]),
-               wl[0] * (sl - wl[n]) - wr[0] * (sr - wr[n]));
 }
 
 // One HLLC face (HLLC.f90:99-135, 275-314): wl/wr reconstructed primitive
