@@ -180,14 +180,15 @@ int tlq_gravity_info(tlq_ctx *ctx, int *levels, int *cycles, double *dif, int *i
 /* Diagnostics: device-side check that every interior density is finite and
  * positive and every energy finite; *bad receives the count of bad cells. */
 int tlq_check_state(tlq_ctx *ctx, long long *bad);
-/* Of the interior cells at T >= T_min with dt > 0.01 t_cool in the last
- * surrogate-cooling step (the cells the network is evaluated for; below that
- * one substep of cooling.f90:67-112 is the converged answer), the fraction
- * that took the exact-integrator fallback. */
+/* Of the interior cells handed to the network in the last surrogate-cooling
+ * step, the fraction whose answer failed the check and took the converged
+ * integrator instead. */
 int tlq_fallback_fraction(tlq_ctx *ctx, double *frac);
-/* The same step's counts: cells handed to the network (mode 2) or to the
- * converged integrator (mode 3) -- those at T >= T_min with dt > 0.01 t_cool --
- * and, of them, cells that took the fallback (mode 2). */
+/* The same step's counts: cells handed to the network (mode 2: T >= T_min and
+ * dt >= the threshold below, in units of t_cool) or to the converged
+ * integrator (mode 3: T >= T_min and dt > 0.01 t_cool; below 0.01 t_cool one
+ * substep of cooling.f90:67-112 is the converged answer), and, of the
+ * network's cells, those that took the integrator (mode 2). */
 int tlq_cooling_counts(tlq_ctx *ctx, long long *queued, long long *fallbacks);
 
 /* Learned cooling surrogate (MLP 3 -> 64 -> 64 -> 1, GELU, bf16 weights,
@@ -201,6 +202,13 @@ int tlq_cooling_counts(tlq_ctx *ctx, long long *queued, long long *fallbacks);
 #define TLQ_MLP_NPARAMS 4481
 /* Random initial parameters (seeded) and fresh Adam moments. */
 int tlq_surrogate_init(tlq_ctx *ctx, unsigned long long seed);
+/* In TLQ_COOL_SURROGATE mode, cells with 0.01 < dt / t_cool < dtc_min take
+ * the converged integrator instead of the network: there its few substeps
+ * (one per 0.01 t_cool) cost less than the network's 128 activations.
+ * Default TLQ_SURROGATE_MIN_DTC (about where the two cost the same on
+ * MI355X); 0.01 or less hands every such cell to the network. */
+#define TLQ_SURROGATE_MIN_DTC 0.3
+int tlq_surrogate_set_threshold(tlq_ctx *ctx, double dtc_min);
 /* FP32 master parameters in / out (n = TLQ_MLP_NPARAMS); setting them also
  * readies the context for TLQ_COOL_SURROGATE. */
 int tlq_surrogate_set_params(tlq_ctx *ctx, const float *theta, int n);

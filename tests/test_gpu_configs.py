@@ -82,11 +82,44 @@ def test_wind_cooling_paths_agree(theta):
     assert 0.0 <= frac < 0.5
 
 
+def test_wind_dense_medium_runs_the_network(theta):
+    # The wind configuration as given (n = 1 cm^-3, T = 1e4 K) never reaches
+    # the network: its time step is below 1e-4 t_cool almost everywhere (48 of
+    # 16.7M cells above 0.01 t_cool after 150 steps).  The same wind into
+    # n = 100 cm^-3 gas at 1e5 K has dt ~ 0.03 t_cool in the ambient medium
+    # and more in the swept-up shell: the surrogate (every cell above 0.01
+    # t_cool to the network) must track the converged integrator.
+    out = {}
+    for mode in (P.COOL_SUBCYCLED, P.COOL_SURROGATE):
+        p, u0 = problems.wind(n=256, cooling=mode, n_amb=100.0, t_amb=1e5)
+        with HydroCore(p, cooling_table=cooling_table()) as h:
+            if mode == P.COOL_SURROGATE:
+                h.surrogate_params(theta)
+                h.surrogate_threshold(0.01)
+            h.set_state(u0)
+            h.advance(40)
+            nq, nf = h.cooling_counts()
+            out[mode] = h.download("U")[2:-2, 2:-2, 2:-2]
+            assert h.check_state() == 0
+        print(f"dense wind 256^3 mode {mode}: last step queued {nq}, fallbacks {nf}")
+        assert nq > 100000
+        if mode == P.COOL_SURROGATE:
+            assert nf < 0.5 * nq
+    l1 = lambda a, b: float(np.abs(a - b).sum() / np.abs(b).sum())  # noqa: E731
+    e = l1(out[P.COOL_SURROGATE][..., 4], out[P.COOL_SUBCYCLED][..., 4])
+    r = l1(out[P.COOL_SURROGATE][..., 0], out[P.COOL_SUBCYCLED][..., 0])
+    print(f"dense wind 256^3, 40 steps: surrogate vs converged integrator L1 energy {e:.3e}, rho {r:.3e}")
+    assert e <= 1e-2 and r <= 1e-2
+
+
 def test_cluster_1024_surrogate_200_steps(theta):
     p, u0 = problems.cluster(n=1024)
     assert p.cooling == P.COOL_SURROGATE and len(p.winds) == 8 and len(p.masses) == 8
     with HydroCore(p, cooling_table=cooling_table()) as h:
         h.surrogate_params(theta)
+        # the network on every cell with dt > 0.01 t_cool (the default hands
+        # those below dt = t_cool to the integrator, which is cheaper there)
+        h.surrogate_threshold(0.01)
         h.set_state(u0)
         del u0
         fracs, queued, t_prev = [], [], 0.0
@@ -107,4 +140,5 @@ def test_cluster_1024_surrogate_200_steps(theta):
           f"sync), fallback fraction mean {np.mean(fracs):.4f} max {np.max(fracs):.4f}; "
           f"cells queued for the network per step: max {max(queued)}, last {queued[-1]}, "
           f"steps with any {sum(q > 0 for q in queued)}")
+    assert min(queued) > 0                   # the network ran at every step
     assert 0.0 <= np.max(fracs) < 0.5

@@ -106,11 +106,14 @@ def test_surrogate_hydro_run_tracks_converged_integrator(trained):
         with HydroCore(p, cooling_table=cooling_table()) as h:
             if mode == P.COOL_SURROGATE:
                 h.surrogate_params(theta)
+                h.surrogate_threshold(0.01)   # every queued cell to the network
             h.set_state(u0)
             h.advance(5)
             out[mode] = h.download("U")[2:-2, 2:-2, 2:-2]
             if mode == P.COOL_SURROGATE:
                 frac = h.fallback_fraction()
+                nq, _ = h.cooling_counts()
+                assert nq > 1000                  # the network ran on this step's hot cells
             assert h.check_state() == 0
     a, b = out[P.COOL_SURROGATE], out[P.COOL_SUBCYCLED]
     l1 = np.abs(a[..., 4] - b[..., 4]).sum() / np.abs(b[..., 4]).sum()

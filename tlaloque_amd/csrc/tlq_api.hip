@@ -84,8 +84,9 @@ struct tlq_ctx {
     bool sur_ready = false;
     unsigned *cool_queue = nullptr;  // cells queued for the dense cooling pass
     unsigned *cool_fqueue = nullptr; // surrogate cells queued for the fallback
-    unsigned *cool_seg = nullptr;    // per segment: cells queued, cells that fell back
-    unsigned *cool_fall = nullptr;
+    unsigned *cool_seg = nullptr;    // per stage-2 tile: cells queued
+    unsigned *cool_off = nullptr;    // their prefix sums (+ the total)
+    unsigned *cool_fall = nullptr;   // cells of the surrogate's fallback queue
     float *gacc = nullptr;           // FP32 point-mass acceleration field (Problem::gacc)
     float *noalloc_scratch = nullptr;  // this device's copy of `scratch` (tlq_surrogate_grad_noalloc)
     // self-gravity: multigrid levels (level 0's E is PHI) and the step count
@@ -129,6 +130,21 @@ hipError_t scratch_address(float **out) {
 }
 
 bool valid_bc(int b) { return b == TLQ_BC_OUTFLOW || b == TLQ_BC_REFLECTIVE || b == TLQ_BC_PERIODIC; }
+
+// The cooling queue (modes 2 and 3): one segment per stage-2 tile, of the
+// tile's cell count.
+long long cool_segment(const tlq_ctx *c);
+long long cool_segments(const tlq_ctx *c);
+
+long long cool_segment(const tlq_ctx *c) {
+    if (c->prec == TLQ_FP32) {
+        using S2 = TileShape<float>::S2;
+        return (long long)S2::TX * S2::TY * S2::ZC;
+    }
+    using S2 = TileShape<double>::S2;
+    return (long long)S2::TX * S2::TY * S2::ZC;
+}
+long long cool_segments(const tlq_ctx *c) { return c->tiling2.ntiles; }
 
 long long ncell_box(const Grid &g) { return (long long)(g.nx + 4) * (g.ny + 4) * (g.nz + 4); }
 long long ninterior(const Grid &g) { return (long long)g.nx * g.ny * g.nz; }
@@ -310,13 +326,15 @@ template <class R> int do_advance(tlq_ctx *c, int nsteps) {
             KTimer kt(c, 1);
             HIPCHK(Launch<R>::stage2(c->stream, c->dprob, c->tiling2, cp<R>(c->u[par]),
                                      cp<R>(c->uh), mp<R>(c->u[par ^ 1]), c->st, par,
-                                     c->host.limiter, c->host.solver));
+                                     c->host.limiter, c->host.solver, c->cool_queue,
+                                     c->cool_seg));
         }
         if (c->host.cooling >= TLQ_COOL_SURROGATE) {
             KTimer kt(c, 3);
-            HIPCHK(sur_cool_pass<R>(c->stream, c->dprob, c->g, mp<R>(c->u[par ^ 1]), c->sur_thq,
-                                    c->st, par, c->cool_queue, c->cool_seg, c->cool_fqueue,
-                                    c->cool_fall, c->host.cooling));
+            HIPCHK(sur_cool_pass<R>(c->stream, c->dprob, (int)cool_segments(c), cool_segment(c),
+                                    mp<R>(c->u[par ^ 1]), c->sur_thq, c->st, par, c->cool_queue,
+                                    c->cool_seg, c->cool_off, c->cool_fqueue, c->cool_fall,
+                                    c->host.cooling));
         }
         c->cur ^= 1;
         ++c->it_host;
@@ -552,6 +570,7 @@ int tlq_create(const tlq_params *p, tlq_ctx **out) {
     P.f_inv_mu_amu = (float)(1.0 / (p->mu0 * AMU));
     P.f_inv_gm1 = (float)(1.0 / (p->gamma - 1.0));
     P.f_floor_f = (float)(1.0 - p->cooling_limit);
+    P.net_min_dtc = (float)TLQ_SURROGATE_MIN_DTC;
 
     auto bail = [&](int code, const char *m) {
         tlq_destroy(c);
@@ -572,20 +591,34 @@ int tlq_create(const tlq_params *p, tlq_ctx **out) {
     if (hipMalloc(&c->uh, bytes) != hipSuccess) return bail(TLQ_ERR_NOMEM, "hipMalloc failed (U1/2)");
     hipMemsetAsync(c->uh, 0, bytes, c->stream);
     c->bufs[TLQ_BUF_UPP] = c->uh;
+    if (c->prec == TLQ_FP32) {
+        using S1 = TileShape<float>::S1;
+        using S2 = TileShape<float>::S2;
+        c->tiling1 = make_tiling(p->nx, p->ny, p->nz, S1::TX, S1::TY, S1::ZC);
+        c->tiling2 = make_tiling(p->nx, p->ny, p->nz, S2::TX, S2::TY, S2::ZC);
+    } else {
+        using S1 = TileShape<double>::S1;
+        using S2 = TileShape<double>::S2;
+        c->tiling1 = make_tiling(p->nx, p->ny, p->nz, S1::TX, S1::TY, S1::ZC);
+        c->tiling2 = make_tiling(p->nx, p->ny, p->nz, S2::TX, S2::TY, S2::ZC);
+    }
     if (p->cooling >= TLQ_COOL_SURROGATE) {
-        // queue of the dense cooling pass: one 32-bit cell offset per interior cell
+        // queue of the dense cooling pass: one 32-bit cell offset per slot,
+        // one segment of the tile's cell count per stage-2 tile
         if (g.nc >= (1ll << 32)) return bail(TLQ_ERR_UNSUPPORTED, "mesh too large for the cooling queue");
-        const long long ncell = (long long)p->nx * p->ny * p->nz;
-        const size_t nb = (size_t)cool_blocks(ncell);
-        if (hipMalloc(&c->cool_queue, (size_t)cool_queue_len(ncell) * sizeof(unsigned)) != hipSuccess ||
-            hipMalloc(&c->cool_seg, nb * sizeof(unsigned)) != hipSuccess ||
-            hipMalloc(&c->cool_fall, nb * sizeof(unsigned)) != hipSuccess)
+        const size_t nb = (size_t)cool_segments(c);
+        const size_t qlen = nb * (size_t)cool_segment(c);
+        if (hipMalloc(&c->cool_queue, qlen * sizeof(unsigned)) != hipSuccess ||
+            hipMalloc(&c->cool_seg, 2 * nb * sizeof(unsigned)) != hipSuccess ||
+            hipMalloc(&c->cool_off, 2 * (nb + 1) * sizeof(unsigned)) != hipSuccess ||
+            hipMalloc(&c->cool_fall, sizeof(unsigned)) != hipSuccess)
             return bail(TLQ_ERR_NOMEM, "hipMalloc failed (cooling queue)");
         if (p->cooling == TLQ_COOL_SURROGATE &&
-            hipMalloc(&c->cool_fqueue, (size_t)cool_queue_len(ncell) * sizeof(unsigned)) != hipSuccess)
+            hipMalloc(&c->cool_fqueue, qlen * sizeof(unsigned)) != hipSuccess)
             return bail(TLQ_ERR_NOMEM, "hipMalloc failed (fallback queue)");
-        hipMemsetAsync(c->cool_seg, 0, nb * sizeof(unsigned), c->stream);
-        hipMemsetAsync(c->cool_fall, 0, nb * sizeof(unsigned), c->stream);
+        hipMemsetAsync(c->cool_seg, 0, 2 * nb * sizeof(unsigned), c->stream);
+        hipMemsetAsync(c->cool_off, 0, 2 * (nb + 1) * sizeof(unsigned), c->stream);
+        hipMemsetAsync(c->cool_fall, 0, sizeof(unsigned), c->stream);
     }
     if (p->self_gravity != TLQ_SG_OFF) {
         c->sg.boundary = p->self_gravity;
@@ -603,17 +636,6 @@ int tlq_create(const tlq_params *p, tlq_ctx **out) {
     hipEventCreate(&c->ev0);
     hipEventCreate(&c->ev1);
 
-    if (c->prec == TLQ_FP32) {
-        using S1 = TileShape<float>::S1;
-        using S2 = TileShape<float>::S2;
-        c->tiling1 = make_tiling(p->nx, p->ny, p->nz, S1::TX, S1::TY, S1::ZC);
-        c->tiling2 = make_tiling(p->nx, p->ny, p->nz, S2::TX, S2::TY, S2::ZC);
-    } else {
-        using S1 = TileShape<double>::S1;
-        using S2 = TileShape<double>::S2;
-        c->tiling1 = make_tiling(p->nx, p->ny, p->nz, S1::TX, S1::TY, S1::ZC);
-        c->tiling2 = make_tiling(p->nx, p->ny, p->nz, S2::TX, S2::TY, S2::ZC);
-    }
     int rc = push_problem(c);
     if (rc) {
         tlq_destroy(c);
@@ -644,6 +666,7 @@ int tlq_destroy(tlq_ctx *c) {
     if (c->cool_queue) hipFree(c->cool_queue);
     if (c->cool_fqueue) hipFree(c->cool_fqueue);
     if (c->cool_seg) hipFree(c->cool_seg);
+    if (c->cool_off) hipFree(c->cool_off);
     if (c->cool_fall) hipFree(c->cool_fall);
     sg_free(c->sg);
     for (void *q : {(void *)c->sur_theta, (void *)c->sur_m, (void *)c->sur_v, (void *)c->sur_gout,
@@ -973,18 +996,18 @@ int tlq_cooling_counts(tlq_ctx *c, long long *queued, long long *fallbacks) {
     if (int rc = check_ctx(c)) return rc;
     if (!queued || !fallbacks) return fail(TLQ_ERR_ARG, "null output");
     *queued = *fallbacks = 0;
-    if (!c->cool_seg) return TLQ_OK;
-    const size_t nb = (size_t)cool_blocks((long long)c->g.nx * c->g.ny * c->g.nz);
-    std::vector<unsigned> q(nb), f(nb);
-    HIPCHK(hipMemcpyAsync(q.data(), c->cool_seg, nb * sizeof(unsigned), hipMemcpyDeviceToHost,
-                          c->stream));
-    HIPCHK(hipMemcpyAsync(f.data(), c->cool_fall, nb * sizeof(unsigned), hipMemcpyDeviceToHost,
-                          c->stream));
+    if (!c->cool_off) return TLQ_OK;
+    // mode 2: the network's cells (off[nseg]) and its rejections; mode 3: the
+    // integrator's cells (off[2 nseg + 1])
+    const long long nseg = cool_segments(c);
+    const bool net = c->host.cooling == TLQ_COOL_SURROGATE;
+    unsigned q = 0, f = 0;
+    HIPCHK(hipMemcpyAsync(&q, c->cool_off + (net ? nseg : 2 * nseg + 1), sizeof q,
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&f, c->cool_fall, sizeof f, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    for (size_t b = 0; b < nb; ++b) {
-        *queued += q[b];
-        *fallbacks += f[b];
-    }
+    *queued = q;
+    *fallbacks = net ? f : 0;
     return TLQ_OK;
 }
 
@@ -1074,6 +1097,14 @@ int tlq_surrogate_get_params(tlq_ctx *c, float *theta, int n) {
     HIPCHK(hipMemcpyAsync(theta, c->sur_theta, n * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return TLQ_OK;
+}
+
+int tlq_surrogate_set_threshold(tlq_ctx *c, double dtc_min) {
+    if (int rc = check_ctx(c)) return rc;
+    if (!(dtc_min >= 0.0 && dtc_min < 1e30)) return fail(TLQ_ERR_ARG, "bad threshold");
+    Problem np = c->host;
+    np.net_min_dtc = (float)dtc_min;
+    return commit_problem(c, np);
 }
 
 int tlq_surrogate_init(tlq_ctx *c, unsigned long long seed) {

@@ -57,6 +57,9 @@ struct Problem {
     int nwinds, nmasses, ntable;
     int cooling;  // 0 off, 1 exact, 2 surrogate
     double tmin;  // first table temperature (cooling.f90:42)
+    // surrogate mode: cells with dt / t_cool below this (and above 0.01) take
+    // the converged integrator, which costs fewer instructions there
+    float net_min_dtc;
     WindDev winds[MAX_WINDS];
     double masses[4 * MAX_MASSES];
     double tab_logT[MAX_TABLE];  // log10 T_i
@@ -93,9 +96,12 @@ __device__ __forceinline__ void lds_barrier() {
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-// PHI at cell (i, j, k), 0 <= i <= nx+1 (likewise j, k).
+// PHI at cell (i, j, k), 0 <= i <= nx+1 (likewise j, k): a global load (a
+// generic one would also count against the LDS counter that the fused
+// kernels' barriers wait on).
 __device__ __forceinline__ double phi_at(const Problem &P, int i, int j, int k) {
-    return P.phi[(long long)i + (long long)(P.g.nx + 2) * ((long long)j + (long long)(P.g.ny + 2) * k)];
+    typedef const __attribute__((address_space(1))) double gdouble;
+    return ((gdouble *)P.phi)[(long long)i + (long long)(P.g.nx + 2) * ((long long)j + (long long)(P.g.ny + 2) * k)];
 }
 
 // ---------------------------------------------------------------------------
@@ -408,9 +414,10 @@ __device__ __forceinline__ double find_coef(const Problem &P, double temp) {
 }
 
 // apply_cooling for one cell (cooling.f90:67-112) on FP64 primitives.
-// Returns the cooled pressure; *cooled tells whether the T >= T_min gate passed.
+// Returns the cooled pressure; *cooled tells whether the T >= T_min gate
+// passed, and *ce_dt (when given) receives dt / t_cool = Lambda n^2 dt / e_th.
 __device__ __forceinline__ double cool_exact(const Problem &P, double rho, double p, double dt,
-                                             bool *cooled) {
+                                             bool *cooled, double *ce_dt = nullptr) {
     const double temp = p * P.mu0 * AMU / (rho * KB);
     if (!(temp >= P.tmin)) {
         *cooled = false;
@@ -421,6 +428,7 @@ __device__ __forceinline__ double cool_exact(const Problem &P, double rho, doubl
     const double aloss = find_coef(P, temp);
     const double numdens = rho / (P.mu0 * AMU);
     const double ce = aloss * (numdens * numdens) / eth;
+    if (ce_dt) *ce_dt = ce * dt;
     double f = exp(-ce * dt);
     if (f < 1.0 - P.cooling_limit) f = 1.0 - P.cooling_limit;
     if (temp * f < 10.0) f = 10.0 / temp;
@@ -472,9 +480,11 @@ __device__ __forceinline__ void stage_table(const Problem &P, TabRow *tab, int t
         tab[q] = TabRow{P.tab_Tf[q], P.tab_lgTf[q], P.tab_Lf[q], P.tab_slopef[q]};
 }
 
-// apply_cooling in FP32 with the table rows in LDS (see cool_fast).
+// apply_cooling in FP32 with the table rows in LDS (see cool_fast); *ce_dt
+// (when given) receives dt / t_cool.
 __device__ __forceinline__ float cool_fast_tab(const Problem &P, const TabRow *tab, float rho,
-                                               float p, float dt, bool *cooled) {
+                                               float p, float dt, bool *cooled,
+                                               float *ce_dt = nullptr) {
     const float rinv = __builtin_amdgcn_rcpf(rho);
     const float temp = p * P.f_cT * rinv;
     if (!(temp >= P.f_tmin)) {
@@ -509,6 +519,7 @@ __device__ __forceinline__ float cool_fast_tab(const Problem &P, const TabRow *t
     const float n = rho * P.f_inv_mu_amu;
     const float eth = p * P.f_inv_gm1;
     const float ce = lam * n * n * __builtin_amdgcn_rcpf(eth);
+    if (ce_dt) *ce_dt = ce * dt;
     float f = __builtin_amdgcn_exp2f(-ce * dt * 1.44269504088896f);  // exp(-ce dt)
     if (f < P.f_floor_f) f = P.f_floor_f;
     if (temp * f < 10.f) f = 10.f * __builtin_amdgcn_rcpf(temp);

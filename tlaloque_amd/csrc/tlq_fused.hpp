@@ -19,6 +19,7 @@
 // only face ghosts are read and written.
 #pragma once
 
+#include "tlq_cooling.hpp"
 #include "tlq_kernels.hpp"
 
 namespace tlq {
@@ -32,9 +33,9 @@ struct Tiling {
     int per_xcd;        // ceil(ntiles/8); the grid has 8*per_xcd blocks
 };
 
-__device__ __forceinline__ bool tile_of_block(const Tiling &T, int &bx, int &by, int &bz) {
+__device__ __forceinline__ bool tile_of_block(const Tiling &T, int &bx, int &by, int &bz, int &L) {
     const int b = blockIdx.x;
-    const int L = (b & 7) * T.per_xcd + (b >> 3);
+    L = (b & 7) * T.per_xcd + (b >> 3);
     if (L >= T.ntiles) return false;
     bx = L % T.nbx;
     const int r = L / T.nbx;
@@ -154,14 +155,18 @@ __device__ __forceinline__ void load_col(const R *__restrict__ A, const Planes &
 }
 
 // The precomputed point-mass acceleration of cell (i, j, k) of column c
-// (FP32 mode; no load when there is no field).
+// (FP32 mode; no load when there is no field).  The field's pointer comes
+// from the Problem record, so the loads are made global explicitly: as
+// generic (flat) loads they would also count against the LDS counter, and
+// every LDS barrier would wait for the prefetch of the next plane.
 __device__ __forceinline__ void load_gacc(const float *gacc, const Planes &pl, Column c, int k,
                                           float g[3]) {
     if (!gacc) return;
+    typedef const __attribute__((address_space(1))) char gchar;
+    typedef const __attribute__((address_space(1))) float gfloat;
 #pragma unroll
     for (int e = 0; e < 3; ++e)
-        g[e] = *reinterpret_cast<const float *>(plane_ptr(reinterpret_cast<const char *>(gacc), pl, e, k) +
-                                                c.off);
+        g[e] = *(gfloat *)((gchar *)plane_ptr(reinterpret_cast<const char *>(gacc), pl, e, k) + c.off);
 }
 
 // Stores the five fields of cell (i, j, k) of column c (which lies inside the array).
@@ -326,8 +331,8 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
 
     const Problem &P = *Pp;
     const Grid &g = P.g;
-    int bx, by, bz;
-    if (!tile_of_block(T, bx, by, bz)) return;
+    int bx, by, bz, L;
+    if (!tile_of_block(T, bx, by, bz, L)) return;
     const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
     const int x0 = 1 + bx * TX, y0 = 1 + by * TY, z0 = 1 + bz * ZC;
     const int z1 = min(z0 + ZC - 1, g.nz);
@@ -337,6 +342,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
     const Planes pl = planes_of<R>(g);
     const Column cc = column_of<R>(g, i, j);
     const Column ch = column_of<R>(g, x0 + ht.lx, y0 + ht.ly);
+    const float *const gacc = sizeof(R) == 4 ? P.gacc : nullptr;
 
     const double dt = dt_from_maxima(P, st->maxima[parity]);
     const double dth = dt / 2.0;
@@ -353,6 +359,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
     // face with the code of every later step, so the two tiles that share
     // the face compute the same flux.
     R uc[5], wc[5], hm[5] = {}, pf_n[5], pf_h[5];
+    float pf_g[3] = {};
     load_col(U, pl, cc, z0 - 1, uc);
     prims(P, uc, wc);
     load_col(U, pl, cc, z0, pf_n);
@@ -368,9 +375,11 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
         // The plane index is block-uniform; readfirstlane lets the compiler
         // form the plane addresses in scalar registers.
         const int ku = __builtin_amdgcn_readfirstlane(k);
+        const float gk[3] = {pf_g[0], pf_g[1], pf_g[2]};
         if (k < z1) {  // prefetch plane k+1 (planes z0-1 .. z1+1 <= nz+1)
             load_col(U, pl, cc, ku + 2, pf_n);
             if (ht.kind) load_col(U, pl, ch, ku + 1, pf_h);
+            load_gacc(gacc, pl, cc, ku + 1, pf_g);
         }
         prims(P, un, wn);
         riemann_t<2, SOL>(gam, gr, wc, wn, uc, un, hp);
@@ -445,9 +454,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
                                    hp[e], hm[e], dtdx, dtdy, dtdz);
             if (grav) {
                 double s[5];
-                // stage 1 forms the point-mass acceleration itself: its
-                // registers are at the three-waves-per-SIMD limit
-                gravity_src<R>(P, i, j, k, wc, nullptr, s);
+                gravity_src<R>(P, i, j, k, wc, gacc ? gk : nullptr, s);
 #pragma unroll
                 for (int e = 0; e < 5; ++e) v[e] = v[e] + dth * s[e];
             }
@@ -477,7 +484,9 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
                                                     const R *__restrict__ U,
                                                     const R *__restrict__ Uh,
                                                     R *__restrict__ Unew,
-                                                    StepState *__restrict__ st, int parity) {
+                                                    StepState *__restrict__ st, int parity,
+                                                    unsigned *__restrict__ cqueue,
+                                                    unsigned *__restrict__ cseg) {
     constexpr int NT = TX * TY;
     static_assert(6 * (TX + TY) <= NT, "one halo cell per thread");
     constexpr int WX = TX + 4, WY = TY + 4;  // two-cell halo (primitives of U^{1/2})
@@ -496,13 +505,17 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
     __shared__ R sXm[5][TY][TX + 2], sXp[5][TY][TX + 2];
     __shared__ R sYm[5][TY + 2][TX], sYp[5][TY + 2][TX];
     __shared__ unsigned long long s_wmask;
+    // cells of this tile queued for the dense cooling pass: for the network
+    // (filled from the front of the tile's segment) and for the integrator
+    // (from the back)
+    __shared__ unsigned s_nq, s_ni;
     // FP32 cooling table (one-step cooling in FP32 mode)
     __shared__ TabRow sTab[sizeof(R) == 4 ? MAX_TABLE : 1];
 
     const Problem &P = *Pp;
     const Grid &g = P.g;
-    int bx, by, bz;
-    if (!tile_of_block(T, bx, by, bz)) return;
+    int bx, by, bz, L;
+    if (!tile_of_block(T, bx, by, bz, L)) return;
     const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
     // The first waves also form the halo face states (phase 2, threads
     // below 2 (TX + TY)) and the tile's upper-edge faces (phase 3): issuing
@@ -525,10 +538,13 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
     const R gam = (R)P.gamma, gr = (R)(P.gamma / (P.gamma - 1.0));
     const int lim = P.limiter;
     const bool grav = P.nmasses > 0 || P.phi != nullptr;
-    // Cooling modes 2 and 3 run in a pass of their own after this kernel,
-    // which then also forms the CFL maxima.
+    // Cooling mode 1: the reference's one-step formula on every cell.  Modes
+    // 2 and 3: cells with dt <= 0.01 t_cool take the one substep the
+    // converged integrator would take; the others are queued for the dense
+    // pass after this kernel, which also forms their CFL maxima.
     const bool cool = P.cooling == 1 && P.ntable > 1;
-    const bool maxima_here = P.cooling < 2;
+    const bool coolpass = P.cooling >= 2 && P.ntable > 1;
+    unsigned *const qseg = coolpass ? cqueue + (long long)L * (TX * TY * ZC) : nullptr;
     const double g1 = P.gamma - 1.0;
     const double eta = P.eta;
     if (blockIdx.x == 0 && tid == 0) {
@@ -536,11 +552,14 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
         st->dt = dt;
         st->it = st->it + 1;
     }
-    if (tid == 0)
+    if (tid == 0) {
         s_wmask = P.nwinds ? winds_mask(P, x0 - 2, x0 + TX + 1, y0 - 2, y0 + TY + 1, z0 - 2, z1 + 2)
                            : 0ull;
+        s_nq = 0;
+        s_ni = 0;
+    }
     if constexpr (sizeof(R) == 4) {
-        if (cool) stage_table(P, sTab, tid, NT);  // visible after the plane's first barrier
+        if (cool || coolpass) stage_table(P, sTab, tid, NT);  // visible after the plane's first barrier
     }
 
     // z pipeline of U^{1/2}: primitives at k, k+1, conserved at k, k+1, the
@@ -738,24 +757,35 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
                                   sN[e][ty][tx + 1] + np[e] + nm[e] - R(6) * nc[e];
                     oq[e] = (R)(o[e] + eta * (double)lap);
                 }
-                // calc_prims + cooling (cooling.f90:64-117) + time_step maxima
+                // calc_prims + cooling (cooling.f90:64-117) + time_step maxima.
+                // Cooling modes 2 and 3: a cell with dt > 0.01 t_cool is queued
+                // for the dense pass, which cools it and forms its maxima; the
+                // others take the one step of apply_cooling, the converged
+                // integrator's single substep for them.
+                bool queued = false;
+                float qdtc = 0.f;  // dt / t_cool of a queued cell
                 if constexpr (sizeof(R) == 8) {
                     const double rho = oq[0];
                     const double vx = oq[1] / oq[0], vy = oq[2] / oq[0], vz = oq[3] / oq[0];
                     double p = pressure64(P.gamma, rho, oq[1], oq[2], oq[3], oq[4]);
-                    if (cool) {
+                    if (cool || coolpass) {
                         bool cooled;
-                        const double pn = cool_exact(P, rho, p, dt, &cooled);
-                        if (cooled) {
+                        double cdt = 0.0;
+                        const double pn = cool_exact(P, rho, p, dt, &cooled, &cdt);
+                        queued = coolpass && cooled && cdt > (double)COOL_EPS;
+                        qdtc = (float)cdt;
+                        if (cooled && !queued) {
                             const double ek = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
                             oq[4] = ek + pn / g1;
                             p = pn;
                         }
                     }
-                    const double cs = sqrt(P.gamma * p / rho);
-                    mxd = fmax(mxd, fabs(vx) + cs);
-                    myd = fmax(myd, fabs(vy) + cs);
-                    mzd = fmax(mzd, fabs(vz) + cs);
+                    if (!queued) {
+                        const double cs = sqrt(P.gamma * p / rho);
+                        mxd = fmax(mxd, fabs(vx) + cs);
+                        myd = fmax(myd, fabs(vy) + cs);
+                        mzd = fmax(mzd, fabs(vz) + cs);
+                    }
                 } else {
                     const float rinv = __builtin_amdgcn_rcpf(oq[0]);
                     const float vx = oq[1] * rinv, vy = oq[2] * rinv, vz = oq[3] * rinv;
@@ -763,20 +793,47 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
                     const double ek = 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) * inv64(oq[0]);
                     double pd = g1 * ((double)oq[4] - ek);
                     float p = pd > 1e-20 ? (float)pd : 1e-20f;
-                    if (cool) {
+                    if (cool || coolpass) {
                         bool cooled;
-                        const float pn = cool_fast_tab(P, sTab, oq[0], p, (float)dt, &cooled);
-                        if (cooled) {
+                        float cdt = 0.f;
+                        const float pn = cool_fast_tab(P, sTab, oq[0], p, (float)dt, &cooled, &cdt);
+                        queued = coolpass && cooled && cdt > COOL_EPS;
+                        qdtc = cdt;
+                        if (cooled && !queued) {
                             oq[4] = (float)(ek + (double)pn * (double)P.f_inv_gm1);
                             p = pn;
                         }
                     }
-                    const float cs = __builtin_amdgcn_sqrtf((float)P.gamma * p * rinv);
-                    mx = fmaxf(mx, fabsf(vx) + cs);
-                    my = fmaxf(my, fabsf(vy) + cs);
-                    mz = fmaxf(mz, fabsf(vz) + cs);
+                    if (!queued) {
+                        const float cs = __builtin_amdgcn_sqrtf((float)P.gamma * p * rinv);
+                        mx = fmaxf(mx, fabsf(vx) + cs);
+                        my = fmaxf(my, fabsf(vy) + cs);
+                        mz = fmaxf(mz, fabsf(vz) + cs);
+                    }
                 }
                 store_col(Unew, pl, cc, ku, oq);
+                if (coolpass) {  // append to the tile's queue segment: one LDS atomic per wave
+                    const bool net = queued && P.cooling == 2 && qdtc >= P.net_min_dtc;
+                    const bool itg = queued && !net;
+                    const int lane = tid & 63;
+                    const unsigned long long nm = __ballot(net), im = __ballot(itg);
+                    if (nm) {
+                        const int leader = __ffsll((long long)nm) - 1;
+                        unsigned qb = 0;
+                        if (lane == leader) qb = atomicAdd(&s_nq, (unsigned)__popcll(nm));
+                        qb = __shfl(qb, leader);
+                        if (net) qseg[qb + __popcll(nm & ((1ull << lane) - 1ull))] = (unsigned)g.idx(i, j, k);
+                    }
+                    if (im) {
+                        const int leader = __ffsll((long long)im) - 1;
+                        unsigned qb = 0;
+                        if (lane == leader) qb = atomicAdd(&s_ni, (unsigned)__popcll(im));
+                        qb = __shfl(qb, leader);
+                        if (itg)
+                            qseg[TX * TY * ZC - 1 - (qb + __popcll(im & ((1ull << lane) - 1ull)))] =
+                                (unsigned)g.idx(i, j, k);
+                    }
+                }
             }
         }
 #pragma unroll
@@ -791,9 +848,13 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
             nc[e] = np[e];
         }
     }
-    if (maxima_here) {
-        if constexpr (sizeof(R) == 8) block_max3<NT>(mxd, myd, mzd, st->maxima[parity ^ 1]);
-        else block_max3<NT>((double)mx, (double)my, (double)mz, st->maxima[parity ^ 1]);
+    if constexpr (sizeof(R) == 8) block_max3<NT>(mxd, myd, mzd, st->maxima[parity ^ 1]);
+    else block_max3<NT>((double)mx, (double)my, (double)mz, st->maxima[parity ^ 1]);
+    // block_max3 ends with a block barrier: every append precedes it.  cseg
+    // holds the network counts of the tiles, then their integrator counts.
+    if (coolpass && tid == 0) {
+        cseg[L] = s_nq;
+        cseg[T.ntiles + L] = s_ni;
     }
 }
 

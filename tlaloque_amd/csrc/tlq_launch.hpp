@@ -73,9 +73,11 @@ template <class R> struct Launch {
                             long long ninterior);
     static hipError_t stage1(hipStream_t s, const Problem *P, const Tiling &T, const R *U, R *Uh,
                              StepState *st, int parity, int solver);
+    // cqueue/cseg: the cooling queue (modes 2 and 3), one segment of
+    // TX*TY*ZC cells per tile, and the per-tile counts; null otherwise.
     static hipError_t stage2(hipStream_t s, const Problem *P, const Tiling &T, const R *U,
                              const R *Uh, R *Unew, StepState *st, int parity, int limiter,
-                             int solver);
+                             int solver, unsigned *cqueue, unsigned *cseg);
 };
 
 }  // namespace tlq

@@ -113,7 +113,7 @@ hipError_t Launch<R>::stage1(hipStream_t s, const Problem *P, const Tiling &T, c
 template <class R>
 hipError_t Launch<R>::stage2(hipStream_t s, const Problem *P, const Tiling &T, const R *U,
                              const R *Uh, R *Unew, StepState *st, int parity, int limiter,
-                             int solver) {
+                             int solver, unsigned *cqueue, unsigned *cseg) {
     using TS = typename TileShape<R>::S2;
     const dim3 grid(8 * T.per_xcd), block(TS::TX * TS::TY);
     // van Leer (the reference default) is compiled in; other limiters are
@@ -122,17 +122,17 @@ hipError_t Launch<R>::stage2(hipStream_t s, const Problem *P, const Tiling &T, c
     if (solver == SOLVER_HLL) {
         if (vl)
             hipLaunchKernelGGL((k_stage2<R, TS::TX, TS::TY, TS::ZC, LIM_VANLEER, SOLVER_HLL, TS::WAVES>), grid,
-                               block, 0, s, P, T, U, Uh, Unew, st, parity);
+                               block, 0, s, P, T, U, Uh, Unew, st, parity, cqueue, cseg);
         else
             hipLaunchKernelGGL((k_stage2<R, TS::TX, TS::TY, TS::ZC, 0, SOLVER_HLL, TS::WAVES>), grid, block, 0,
-                               s, P, T, U, Uh, Unew, st, parity);
+                               s, P, T, U, Uh, Unew, st, parity, cqueue, cseg);
     } else {
         if (vl)
             hipLaunchKernelGGL((k_stage2<R, TS::TX, TS::TY, TS::ZC, LIM_VANLEER, SOLVER_HLLC, TS::WAVES>),
-                               grid, block, 0, s, P, T, U, Uh, Unew, st, parity);
+                               grid, block, 0, s, P, T, U, Uh, Unew, st, parity, cqueue, cseg);
         else
             hipLaunchKernelGGL((k_stage2<R, TS::TX, TS::TY, TS::ZC, 0, SOLVER_HLLC, TS::WAVES>), grid, block, 0,
-                               s, P, T, U, Uh, Unew, st, parity);
+                               s, P, T, U, Uh, Unew, st, parity, cqueue, cseg);
     }
     return hipGetLastError();
 }

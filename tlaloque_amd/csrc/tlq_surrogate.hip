@@ -1,6 +1,7 @@
 // tlq_surrogate.hip -- kernels of the learned cooling surrogate (see
 // tlq_surrogate.hpp): training data, MFMA training gradient, Adam, inference,
 // and the cooling pass of the fused step.
+#include "tlq_cooling.hpp"
 #include "tlq_kernels.hpp"
 #include "tlq_surrogate.hpp"
 
@@ -44,100 +45,6 @@ __device__ __forceinline__ float gelu_d(float x) {
 
 __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
-}
-
-// ---------------------------------------------------------------------------
-// Cooling integrated to convergence: apply_cooling's per-cell formula
-// (cooling.f90:67-112) in substeps h <= eps t_cool with Lambda re-read at the
-// current temperature, until dt is used or T < T_min (the reference's gate).
-__device__ double cool_sub_d(const Problem &P, double n, double T, double dt, double eps,
-                             int max_sub) {
-    double left = dt;
-    for (int s = 0; left > 0.0 && T >= P.tmin && s < max_sub; ++s) {
-        const double lam = find_coef(P, T);
-        const double eth = n * KB * T / (P.gamma - 1.0);
-        const double ce = lam * (n * n) / eth;
-        double h = eps / ce;
-        if (h > left) h = left;
-        double f = exp(-ce * h);
-        if (f < 1.0 - P.cooling_limit) f = 1.0 - P.cooling_limit;
-        if (T * f < 10.0) f = 10.0 / T;
-        T = T * f;
-        left -= h;
-    }
-    return T;
-}
-
-// Table segment of temperature T (first row with T_i > T, as find_coef),
-// clamped to the last segment for T >= T_max.
-__device__ __forceinline__ int table_row(const Problem &P, float temp, float lg) {
-    if (!(temp < P.f_tmax)) return P.ntable - 1;
-    int hi;
-    if (P.tab_uniform) {
-        hi = (int)((lg - P.f_lgT0) * P.f_inv_dlg) + 1;
-        hi = hi < 1 ? 1 : (hi > P.ntable - 1 ? P.ntable - 1 : hi);
-        if (!(P.tab_Tf[hi] > temp) && hi < P.ntable - 1) ++hi;
-        if (hi > 1 && P.tab_Tf[hi - 1] > temp) --hi;
-    } else {
-        int lo = 0;
-        hi = P.ntable - 1;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (P.tab_Tf[mid] > temp) hi = mid;
-            else lo = mid;
-        }
-    }
-    return hi;
-}
-
-// Lambda(T) from the table in FP32 (find_coef's interpolation).
-__device__ __forceinline__ float lambda_f(const Problem &P, float temp) {
-    const float lg = __builtin_amdgcn_logf(temp) * 0.301029995663981f;
-    if (!(temp < P.f_tmax)) return __builtin_amdgcn_exp2f(P.tab_Lf[P.ntable - 1] * 3.32192809488736f);
-    const int hi = table_row(P, temp, lg);
-    const float lgl = P.tab_Lf[hi - 1] + P.tab_slopef[hi] * (lg - P.tab_lgTf[hi - 1]);
-    return __builtin_amdgcn_exp2f(lgl * 3.32192809488736f);
-}
-
-// Y(T) = int_{T_0}^{T} dT'/Lambda(T'): on each table segment Lambda is a
-// power law (linear in log-log), so Y is closed form.  The time to cool from
-// T0 to T1 at density n is t = k_B / ((gamma-1) n) (Y(T0) - Y(T1)).  Y is
-// split into its value at the segment's lower node (double) and the part
-// inside the segment (float), so that Y(T0) - Y(T1) keeps its accuracy when
-// T1 is close to T0.
-struct YSplit {
-    double node;
-    float part;
-};
-__device__ __forceinline__ YSplit cool_Y(const Problem &P, float temp) {
-    if (!(temp > P.f_tmin)) return {0.0, 0.f};
-    const float l2 = __builtin_amdgcn_logf(temp);
-    const int hi = table_row(P, temp, l2 * 0.301029995663981f);
-    const float a = 1.f - P.tab_slopef[hi];
-    const float lr = l2 - __builtin_amdgcn_logf(P.tab_Tf[hi - 1]);  // log2(T/T_{hi-1})
-    const float seg = fabsf(a) > 1e-4f ? (__builtin_amdgcn_exp2f(a * lr) - 1.f) / a
-                                       : lr * 0.6931471805599453f;
-    return {P.tab_Ycum[hi - 1], P.tab_ycoef[hi] * seg};
-}
-__device__ __forceinline__ float cool_dY(const Problem &P, float T0, float T1) {
-    const YSplit a = cool_Y(P, T0), b = cool_Y(P, T1);
-    return (float)(a.node - b.node) + (a.part - b.part);
-}
-
-__device__ float cool_sub_f(const Problem &P, float n, float T, float dt, float eps, int max_sub) {
-    const float cgk = (float)((P.gamma - 1.0) / KB);
-    const float floor_f = P.f_floor_f;
-    float left = dt;
-    for (int s = 0; left > 0.f && T >= P.f_tmin && s < max_sub; ++s) {
-        const float ce = lambda_f(P, T) * n * cgk * __builtin_amdgcn_rcpf(T);
-        const float h = fminf(left, eps * __builtin_amdgcn_rcpf(ce));
-        float f = __builtin_amdgcn_exp2f(-1.4426950408889634f * ce * h);
-        if (f < floor_f) f = floor_f;
-        if (T * f < 10.f) f = 10.f * __builtin_amdgcn_rcpf(T);
-        T = T * f;
-        left -= h;
-    }
-    return T;
 }
 
 // ---------------------------------------------------------------------------
@@ -529,16 +436,17 @@ __global__ __launch_bounds__(64) void k_mlp_eval(const unsigned short *__restric
 }
 
 // ---------------------------------------------------------------------------
-// Cooling of the fused step in modes 2 and 3, after stage 2, in two passes.
+// Cooling of the fused step in modes 2 and 3.
 //
-// k_cool_classify visits every interior cell.  Cells below T_min are left
-// alone; cells with dt <= eps t_cool take one substep (all the converged
-// integrator would take); their CFL maxima are formed here.  Every other
-// cell goes to a queue (one atomic per wavefront).
+// k_stage2 classifies every interior cell as it writes it.  Cells below T_min
+// are left alone; cells with dt <= eps t_cool take one substep (all the
+// converged integrator would take); their CFL maxima are formed there.  Every
+// other cell goes to the queue segment of its stage-2 block (one LDS atomic
+// per wavefront).
 //
-// k_cool_dense runs the queued cells in full wavefronts: mode 3 integrates
-// them to convergence; mode 2 evaluates the surrogate and keeps its answer T1
-// when
+// k_cool_dense then runs the queued cells in full wavefronts: mode 3
+// integrates them to convergence; mode 2 evaluates the surrogate and keeps
+// its answer T1 when
 //   * the inputs lie in the training box,
 //   * 10 K <= T1 <= T0 (cooling only lowers T, never below the floor), and
 //   * the exact cooling time from T0 to T1, t = k_B (Y(T0) - Y(T1)) / ((gamma-1) n),
@@ -549,184 +457,90 @@ __global__ __launch_bounds__(64) void k_mlp_eval(const unsigned short *__restric
 //     the gate);
 // other cells go to a fallback queue that a mode-3 dense pass integrates.
 // Then the CFL maxima of the cells finished here.
-//
-constexpr float COOL_EPS = 0.01f;
-constexpr int COOL_MAXSUB = 100000;
-// A surrogate answer at the gate is set just below it, in the 1% band where
-// the converged integrator's last substep ends (so the cell stays gated).
-constexpr double SNAP = 0.995;
 
-struct CellState {
-    double rho, ek, p, nd, T;
-    float rinv;  // 1/rho (FP32 state)
-};
-
-// Primitive state of one cell for cooling: FP64 arithmetic for an FP64
-// state; for an FP32 state the guarded fast path of calc_prims (pressure
-// recovery in FP64 with a refined reciprocal, the rest in FP32).
-template <class R>
-__device__ __forceinline__ CellState cell_state(const Problem &P, const R *__restrict__ U,
-                                                long long nc, long long c, double u[5]) {
-#pragma unroll
-    for (int e = 0; e < 5; ++e) u[e] = (double)U[e * nc + c];
-    CellState s;
-    s.rho = u[0];
-    if constexpr (sizeof(R) == 8) {
-        s.ek = 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) / s.rho;
-        s.p = (P.gamma - 1.0) * (u[4] - s.ek);
-        if (!(s.p > 1e-20)) s.p = 1e-20;
-        s.nd = s.rho / (P.mu0 * AMU);
-        s.T = s.p * P.mu0 * AMU / (s.rho * KB);
-        s.rinv = 0.f;
-    } else {
-        s.rinv = __builtin_amdgcn_rcpf((float)u[0]);
-        s.ek = 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) * inv64((float)u[0]);
-        s.p = (P.gamma - 1.0) * (u[4] - s.ek);
-        if (!(s.p > 1e-20)) s.p = 1e-20;
-        s.nd = (double)((float)u[0] * P.f_inv_mu_amu);
-        s.T = (double)((float)s.p * P.f_cT * s.rinv);
-    }
-    return s;
-}
-
-// dt / t_cool of a cell at T >= T_min
-template <class R>
-__device__ __forceinline__ double dt_over_tcool(const Problem &P, const CellState &s, double dt) {
-    if constexpr (sizeof(R) == 8) {
-        return find_coef(P, s.T) * s.nd * s.nd / (s.p / (P.gamma - 1.0)) * dt;
-    } else {
-        const float ce = lambda_f(P, (float)s.T) * (float)s.nd * (float)s.nd *
-                         (float)(P.gamma - 1.0) * __builtin_amdgcn_rcpf((float)s.p);
-        return (double)(ce * (float)dt);
-    }
-}
-
-template <class R>
-__device__ __forceinline__ double integrate(const Problem &P, const CellState &s, double dt,
-                                            int max_sub) {
-    if constexpr (sizeof(R) == 8) return cool_sub_d(P, s.nd, s.T, dt, COOL_EPS, max_sub);
-    else return cool_sub_f(P, (float)s.nd, (float)s.T, (float)dt, COOL_EPS, max_sub);
-}
-
-template <class R>
-__device__ __forceinline__ void finish_cell(const Problem &P, R *__restrict__ U, long long nc,
-                                            long long c, const double u[5], const CellState &s,
-                                            double Tn, double &mx, double &my, double &mz) {
-    if constexpr (sizeof(R) == 8) {
-        const double pn = s.p * (Tn / s.T);
-        U[4 * nc + c] = (R)(s.ek + pn / (P.gamma - 1.0));
-        const double cs = sqrt(P.gamma * pn / s.rho);
-        mx = fmax(mx, fabs(u[1] / s.rho) + cs);
-        my = fmax(my, fabs(u[2] / s.rho) + cs);
-        mz = fmax(mz, fabs(u[3] / s.rho) + cs);
-    } else {
-        const float pn = (float)s.p * ((float)Tn * __builtin_amdgcn_rcpf((float)s.T));
-        U[4 * nc + c] = (R)(s.ek + (double)pn * (double)P.f_inv_gm1);
-        const float cs = __builtin_amdgcn_sqrtf((float)P.gamma * pn * s.rinv);
-        mx = fmax(mx, (double)(fabsf((float)u[1] * s.rinv) + cs));
-        my = fmax(my, (double)(fabsf((float)u[2] * s.rinv) + cs));
-        mz = fmax(mz, (double)(fabsf((float)u[3] * s.rinv) + cs));
-    }
-}
-
-// CFL maxima of a cell left as it is (below T_min)
-template <class R>
-__device__ __forceinline__ void cell_maxima(const Problem &P, const double u[5],
-                                            const CellState &s, double &mx, double &my,
-                                            double &mz) {
-    if constexpr (sizeof(R) == 8) {
-        const double cs = sqrt(P.gamma * s.p / s.rho);
-        mx = fmax(mx, fabs(u[1] / s.rho) + cs);
-        my = fmax(my, fabs(u[2] / s.rho) + cs);
-        mz = fmax(mz, fabs(u[3] / s.rho) + cs);
-    } else {
-        const float cs = __builtin_amdgcn_sqrtf((float)P.gamma * (float)s.p * s.rinv);
-        mx = fmax(mx, (double)(fabsf((float)u[1] * s.rinv) + cs));
-        my = fmax(my, (double)(fabsf((float)u[2] * s.rinv) + cs));
-        mz = fmax(mz, (double)(fabsf((float)u[3] * s.rinv) + cs));
-    }
-}
-
-// log10 through the hardware log2 (FP32 inputs of the surrogate and its check)
-__device__ __forceinline__ float lg10(float x) {
-    return __builtin_amdgcn_logf(x) * 0.301029995663981f;
-}
-
-// Block b of the classifying pass owns the contiguous interior cells
-// [b C, (b+1) C) (flat index, x fastest) and the queue segment of the same
-// range; it fills the segment with an LDS counter and leaves its length in
-// seg[b].  No global atomics besides the CFL maxima.
-template <class R>
-__global__ __launch_bounds__(256) void k_cool_classify(const Problem *__restrict__ Pp,
-                                                       R *__restrict__ U,
-                                                       StepState *__restrict__ st, int parity,
-                                                       unsigned *__restrict__ queue,
-                                                       unsigned *__restrict__ seg, long long C) {
-    __shared__ unsigned s_n;
-    const Problem &P = *Pp;
-    const Grid &g = P.g;
-    const double dt = dt_from_maxima(P, st->maxima[parity]);
-    const int lane = threadIdx.x & 63;
-    const long long ncell = (long long)g.nx * g.ny * g.nz;
-    const long long first = (long long)blockIdx.x * C;
-    unsigned *qseg = queue + first;
-    if (threadIdx.x == 0) s_n = 0;
+// Exclusive prefix sums of the per-tile queue counts: off[b] = cnt[0] + ... +
+// cnt[b-1], off[n] = the total; also zeroes the fallback counter.  One block.
+__global__ __launch_bounds__(1024) void k_queue_scan(const unsigned *__restrict__ cnt,
+                                                     unsigned *__restrict__ off, int n,
+                                                     unsigned *__restrict__ fcount) {
+    __shared__ unsigned part[1024];
+    const int t = threadIdx.x, per = (n + 1023) / 1024;
+    const int b0 = min(t * per, n), b1 = min(b0 + per, n);
+    unsigned sum = 0;
+    for (int b = b0; b < b1; ++b) sum += cnt[b];
+    part[t] = sum;
     __syncthreads();
-    double mx = 0.0, my = 0.0, mz = 0.0;
-    // C is a multiple of the block size: every wavefront runs whole
-    // iterations, so its ballot sees all 64 lanes
-    for (long long t = first + threadIdx.x; t < first + C; t += blockDim.x) {
-        const bool in = t < ncell;
-        const long long tt = in ? t : ncell - 1;
-        const int i = (int)(tt % g.nx) + 1;
-        const long long r = tt / g.nx;
-        const int j = (int)(r % g.ny) + 1, k = (int)(r / g.ny) + 1;
-        const long long c = g.idx(i, j, k);
-        double u[5];
-        const CellState s = cell_state(P, U, g.nc, c, u);
-        const bool hot = in && s.T >= P.tmin;
-        const bool big = hot && dt_over_tcool<R>(P, s, dt) > (double)COOL_EPS;
-        if (in && !big) {
-            if (hot) finish_cell(P, U, g.nc, c, u, s, integrate<R>(P, s, dt, 1), mx, my, mz);
-            else cell_maxima<R>(P, u, s, mx, my, mz);
-        }
-        const unsigned long long mask = __ballot(big);
-        if (mask) {
-            const int leader = __ffsll((long long)mask) - 1;
-            unsigned base = 0;
-            if (lane == leader) base = atomicAdd(&s_n, (unsigned)__popcll(mask));
-            base = __shfl(base, leader);
-            if (big) qseg[base + __popcll(mask & ((1ull << lane) - 1ull))] = (unsigned)c;
-        }
+    for (int d = 1; d < 1024; d <<= 1) {  // inclusive scan of the partial sums
+        const unsigned v = t >= d ? part[t - d] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
     }
-    __syncthreads();
-    if (threadIdx.x == 0) seg[blockIdx.x] = s_n;
-    block_max3<256>(mx, my, mz, st->maxima[parity ^ 1]);
+    unsigned run = part[t] - sum;
+    for (int b = b0; b < b1; ++b) {
+        off[b] = run;
+        run += cnt[b];
+    }
+    if (t == 1023) off[n] = part[1023];
+    if (t == 0) *fcount = 0;
 }
 
-// Block b of the dense pass runs queue segment b, 64 cells per wavefront
-// step; fall[b] receives the segment's fallback count.
+// Segment b of the queue holding position p: off[b] <= p < off[b+1] (off
+// non-decreasing; empty segments repeat a value).
+__device__ __forceinline__ int queue_segment(const unsigned *__restrict__ off, int nseg,
+                                             unsigned p) {
+    int lo = 0, hi = nseg;  // off[lo] <= p < off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= p) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// The dense passes: wavefronts take 64 consecutive queue positions at a
+// time, grid-stride, so the work is spread evenly whatever the tiles' counts.
+// MODE 2, the network: the cells k_stage2 put at the front of each tile's
+// segment (offn: their scanned counts); rejected cells go to the fallback
+// queue through a per-wavefront LDS buffer, flushed with one global atomic
+// per FBUF_FLUSH cells (a global atomic per wavefront serialises on the
+// counter when a tenth of the cells fall back).  MODE 3, the converged
+// integrator: the cells at the back of each segment (offi), then the
+// *fcount fallbacks.
+constexpr int FBUF = 1024, FBUF_FLUSH = FBUF - 64;
+
+// Copies a wavefront's n buffered fallback cells to the fallback queue.
+__device__ __forceinline__ void flush_fallbacks(const unsigned *buf, unsigned n, int lane,
+                                                unsigned *__restrict__ fqueue,
+                                                unsigned *__restrict__ fcount) {
+    __builtin_amdgcn_wave_barrier();  // the wavefront's LDS writes of buf come first
+    unsigned base = 0;
+    if (lane == 0) base = atomicAdd(fcount, n);
+    base = __shfl(base, 0);
+    for (unsigned q = lane; q < n; q += 64) fqueue[base + q] = buf[q];
+}
+
 template <class R, int MODE>
 __global__ __launch_bounds__(128) void k_cool_dense(const Problem *__restrict__ Pp,
                                                     R *__restrict__ U,
                                                     const unsigned short *__restrict__ thq,
                                                     StepState *__restrict__ st, int parity,
                                                     const unsigned *__restrict__ queue,
-                                                    const unsigned *__restrict__ seg,
-                                                    unsigned *__restrict__ fqueue,
-                                                    unsigned *__restrict__ fall, long long C) {
+                                                    const unsigned *__restrict__ offn,
+                                                    const unsigned *__restrict__ offi, int nseg,
+                                                    long long C, unsigned *__restrict__ fqueue,
+                                                    unsigned *__restrict__ fcount) {
     __shared__ MlpSmall sm;
-    __shared__ unsigned s_fall;
-    unsigned *fseg = MODE == 2 ? fqueue + (long long)blockIdx.x * C : nullptr;
+    __shared__ unsigned s_fbuf[MODE == 2 ? 2 : 1][MODE == 2 ? FBUF : 1];
     const Problem &P = *Pp;
     const Grid &g = P.g;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const long long nq = seg[blockIdx.x];
-    const unsigned *qseg = queue + (long long)blockIdx.x * C;
+    const unsigned nseg_cells = MODE == 2 ? offn[nseg] : offi[nseg];
+    const unsigned nq = MODE == 2 ? nseg_cells : nseg_cells + *fcount;
     const double dt = dt_from_maxima(P, st->maxima[parity]);
+    unsigned nfb = 0;  // this wavefront's buffered fallbacks
     bf16x4 w2a[4][4];
     float b3 = 0.f;
-    if (threadIdx.x == 0) s_fall = 0;
     if (MODE == 2 && nq > 0) {
         load_small_f(sm, thq, threadIdx.x, blockDim.x);
         b3 = bf2f(thq[OB3]);
@@ -734,10 +548,21 @@ __global__ __launch_bounds__(128) void k_cool_dense(const Problem *__restrict__ 
     }
     __syncthreads();
     double mx = 0.0, my = 0.0, mz = 0.0;
-    for (long long base = (long long)wv * 64; base < nq; base += 128) {
-        const long long qi = base + lane;
+    const unsigned stride = gridDim.x * 128u;
+    for (unsigned base = (blockIdx.x * 2u + wv) * 64u; base < nq; base += stride) {
+        const unsigned qi = base + lane;
         const bool valid = qi < nq;
-        const long long c = (long long)qseg[valid ? qi : base];
+        const unsigned pos = valid ? qi : base;
+        long long c;
+        if (MODE == 2) {
+            const int b = queue_segment(offn, nseg, pos);
+            c = queue[(long long)b * C + (pos - offn[b])];
+        } else if (pos < nseg_cells) {
+            const int b = queue_segment(offi, nseg, pos);
+            c = queue[(long long)b * C + (C - 1 - (pos - offi[b]))];
+        } else {
+            c = fqueue[pos - nseg_cells];
+        }
         double u[5];
         const CellState s = cell_state(P, U, g.nc, c, u);
         bool need_exact = valid;
@@ -780,16 +605,16 @@ __global__ __launch_bounds__(128) void k_cool_dense(const Problem *__restrict__ 
                 Tn = snap ? (double)SNAP * P.tmin : (double)T1;
                 need_exact = false;
             }
-            // fallbacks go to a queue of their own, integrated by a dense
-            // mode-3 pass: a lane that needs the integrator does not hold up
-            // the other 63
+            // fallbacks go to the integrator's pass: a lane that needs the
+            // integrator does not hold up the other 63
             const unsigned long long fm = __ballot(need_exact);
             if (fm) {
-                const int leader = __ffsll((long long)fm) - 1;
-                unsigned fb = 0;
-                if (lane == leader) fb = atomicAdd(&s_fall, (unsigned)__popcll(fm));
-                fb = __shfl(fb, leader);
-                if (need_exact) fseg[fb + __popcll(fm & ((1ull << lane) - 1ull))] = (unsigned)c;
+                if (need_exact) s_fbuf[wv][nfb + __popcll(fm & ((1ull << lane) - 1ull))] = (unsigned)c;
+                nfb += (unsigned)__popcll(fm);
+                if (nfb > FBUF_FLUSH) {  // the buffer is this wavefront's alone: no barrier
+                    flush_fallbacks(s_fbuf[wv], nfb, lane, fqueue, fcount);
+                    nfb = 0;
+                }
             }
             if (valid && !need_exact) finish_cell(P, U, g.nc, c, u, s, Tn, mx, my, mz);
         } else {
@@ -797,8 +622,7 @@ __global__ __launch_bounds__(128) void k_cool_dense(const Problem *__restrict__ 
                                    my, mz);
         }
     }
-    __syncthreads();
-    if (MODE == 2 && threadIdx.x == 0) fall[blockIdx.x] = s_fall;
+    if (MODE == 2 && nfb) flush_fallbacks(s_fbuf[wv], nfb, lane, fqueue, fcount);
     if (nq > 0) block_max3<128>(mx, my, mz, st->maxima[parity ^ 1]);
 }
 
@@ -841,31 +665,27 @@ hipError_t sur_eval(hipStream_t s, const unsigned short *thq, const float *X, lo
 }
 
 template <class R>
-hipError_t sur_cool_pass(hipStream_t s, const Problem *P, const Grid &g, R *U,
+hipError_t sur_cool_pass(hipStream_t s, const Problem *P, int nseg, long long C, R *U,
                          const unsigned short *thq, StepState *st, int parity, unsigned *queue,
-                         unsigned *seg, unsigned *fqueue, unsigned *fall, int mode) {
-    const long long ncell = (long long)g.nx * g.ny * g.nz;
-    const long long C = cool_segment(ncell);
-    const unsigned nb = (unsigned)((ncell + C - 1) / C);
-    hipLaunchKernelGGL(k_cool_classify<R>, dim3(nb), dim3(256), 0, s, P, U, st, parity, queue,
-                       seg, C);
-    if (mode == 2) {
+                         const unsigned *seg, unsigned *off, unsigned *fqueue, unsigned *fcount,
+                         int mode) {
+    constexpr unsigned nb = 4096;  // 8192 wavefronts, grid-stride over the queue
+    unsigned *offn = off, *offi = off + nseg + 1;
+    hipLaunchKernelGGL(k_queue_scan, dim3(1), dim3(1024), 0, s, seg, offn, nseg, fcount);
+    hipLaunchKernelGGL(k_queue_scan, dim3(1), dim3(1024), 0, s, seg + nseg, offi, nseg, fcount);
+    if (mode == 2)
         hipLaunchKernelGGL((k_cool_dense<R, 2>), dim3(nb), dim3(128), 0, s, P, U, thq, st, parity,
-                           queue, seg, fqueue, fall, C);
-        hipLaunchKernelGGL((k_cool_dense<R, 3>), dim3(nb), dim3(128), 0, s, P, U, thq, st, parity,
-                           fqueue, fall, nullptr, nullptr, C);
-    } else {
-        hipLaunchKernelGGL((k_cool_dense<R, 3>), dim3(nb), dim3(128), 0, s, P, U, thq, st, parity,
-                           queue, seg, nullptr, nullptr, C);
-    }
+                           queue, offn, offi, nseg, C, fqueue, fcount);
+    hipLaunchKernelGGL((k_cool_dense<R, 3>), dim3(nb), dim3(128), 0, s, P, U, thq, st, parity,
+                       queue, offn, offi, nseg, C, fqueue, fcount);
     return hipGetLastError();
 }
 
-template hipError_t sur_cool_pass<float>(hipStream_t, const Problem *, const Grid &, float *,
+template hipError_t sur_cool_pass<float>(hipStream_t, const Problem *, int, long long, float *,
                                          const unsigned short *, StepState *, int, unsigned *,
-                                         unsigned *, unsigned *, unsigned *, int);
-template hipError_t sur_cool_pass<double>(hipStream_t, const Problem *, const Grid &, double *,
+                                         const unsigned *, unsigned *, unsigned *, unsigned *, int);
+template hipError_t sur_cool_pass<double>(hipStream_t, const Problem *, int, long long, double *,
                                           const unsigned short *, StepState *, int, unsigned *,
-                                          unsigned *, unsigned *, unsigned *, int);
+                                          const unsigned *, unsigned *, unsigned *, unsigned *, int);
 
 }  // namespace tlq

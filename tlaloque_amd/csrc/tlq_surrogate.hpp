@@ -53,31 +53,20 @@ hipError_t sur_eval(hipStream_t s, const unsigned short *thq, const float *X, lo
                     float *out);
 
 // Cooling after the fused corrector in modes 2 (surrogate with per-cell
-// fallback) and 3 (the converged integrator): a classifying pass over the
-// interior that queues the cells with dt > 0.01 t_cool, then a dense pass over
-// the queue (mode 2: the network, whose rejected cells go to fqueue for a
-// second dense pass with the integrator).  Also produces the next step's CFL
-// maxima.  The interior (flat index) is cut into segments of
-// cool_segment(ncell) cells, one block of each pass per segment; queue and
-// fqueue hold one 32-bit cell offset per segment slot (cool_queue_len
-// entries), seg[b] the number queued in segment b and fall[b] the number of
-// those that took the fallback.
-inline long long cool_segment(long long ncell) {
-    const long long nb = 2048;  // blocks per pass
-    long long C = (ncell + nb - 1) / nb;
-    C = (C + 255) / 256 * 256;
-    return C < 256 ? 256 : C;
-}
-inline long long cool_blocks(long long ncell) {
-    const long long C = cool_segment(ncell);
-    return (ncell + C - 1) / C;
-}
-inline long long cool_queue_len(long long ncell) { return cool_blocks(ncell) * cool_segment(ncell); }
-
+// fallback) and 3 (the converged integrator): k_stage2 queues the cells with
+// dt > 0.01 t_cool in one segment of C slots per stage-2 tile (nseg
+// segments): from the front the cells for the network (mode 2, dt >=
+// net_min_dtc t_cool), from the back those for the integrator; seg[b] and
+// seg[nseg + b] hold the two counts of segment b.  This pass scans them into
+// off (network: off[0..nseg], integrator: off[nseg+1..2 nseg+1]; the last of
+// each is its total), runs the network on its cells (mode 2; rejected cells
+// go to fqueue, *fcount of them) and the integrator on its cells and the
+// fallbacks, and adds those cells' CFL maxima.
 struct StepState;
 template <class R>
-hipError_t sur_cool_pass(hipStream_t s, const Problem *P, const Grid &g, R *U,
+hipError_t sur_cool_pass(hipStream_t s, const Problem *P, int nseg, long long C, R *U,
                          const unsigned short *thq, StepState *st, int parity, unsigned *queue,
-                         unsigned *seg, unsigned *fqueue, unsigned *fall, int mode);
+                         const unsigned *seg, unsigned *off, unsigned *fqueue, unsigned *fcount,
+                         int mode);
 
 }  // namespace tlq

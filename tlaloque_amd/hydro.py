@@ -48,7 +48,7 @@ EXPORTS = (
     "tlq_kernel_stats", "tlq_layout", "tlq_copy_probe", "tlq_surrogate_init",
     "tlq_surrogate_set_params", "tlq_surrogate_get_params", "tlq_surrogate_generate",
     "tlq_surrogate_samples", "tlq_surrogate_train", "tlq_surrogate_eval", "tlq_surrogate_grad",
-    "tlq_surrogate_grad_noalloc",
+    "tlq_surrogate_grad_noalloc", "tlq_surrogate_set_threshold",
 )
 MLP_NPARAMS = 4481
 
@@ -135,6 +135,7 @@ def load_library(path: Optional[str] = None):
         "tlq_surrogate_train": [vp, ip, ip, ctypes.c_double, ctypes.c_double, dp],
         "tlq_surrogate_eval": [vp, fp, ctypes.c_longlong, fp],
         "tlq_surrogate_grad": [vp, fp, fp, ip, dp, dp],
+        "tlq_surrogate_set_threshold": [vp, ctypes.c_double],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -378,6 +379,11 @@ class HydroCore:
         out = np.empty(MLP_NPARAMS, dtype=np.float32)
         _check(_lib.tlq_surrogate_get_params(self._ctx, _fptr(out), out.size))
         return out
+
+    def surrogate_threshold(self, dtc_min: float) -> None:
+        """Surrogate mode: cells with 0.01 <= dt/t_cool < dtc_min take the
+        converged integrator instead of the network (include/tlq.h)."""
+        _check(_lib.tlq_surrogate_set_threshold(self._ctx, float(dtc_min)))
 
     def surrogate_generate(self, n: int, seed: int = 7) -> None:
         _check(_lib.tlq_surrogate_generate(self._ctx, int(n), int(seed)))

@@ -20,7 +20,7 @@ included), the form ``user_initconds`` fills (user/user.f90:22-52).
   wind         one stellar wind (radius 5 cells, 1000 km/s, 1e-6 Msun/yr, wind
                T = 1e4 K, mu = 1) at the box centre (a cell corner for even n,
                so no cell sits at r = 0) in a uniform n = 1 cm^-3, T = 1e4 K
-               medium, cgs, tabulated cooling.
+               medium (n_amb, t_amb), cgs, tabulated cooling.
   clumpy       lognormal density (mean n = 1 cm^-3, sigma_ln rho = 1) from a
                seeded Gaussian random field with P(k) ~ k^-4, velocities of the
                same spectrum at RMS Mach 2, T = 1e4 K, a central 1e4 Msun point
@@ -181,7 +181,8 @@ def clumpy(n=512, precision=P.PREC_FP32, seed=2024, box_pc=10.0, cooling=P.COOL_
     return p, cons_from_prims(p, rho, v, pres)
 
 
-def wind(n=256, precision=P.PREC_FP32, box_pc=1.0, cooling=P.COOL_EXACT, temp_wind=1e4):
+def wind(n=256, precision=P.PREC_FP32, box_pc=1.0, cooling=P.COOL_EXACT, temp_wind=1e4,
+         n_amb=1.0, t_amb=1e4):
     L = box_pc * P.PC
     dx = L / n
     # the wind state is singular at r = 0 (winds.f90:72): the centre must not
@@ -192,7 +193,7 @@ def wind(n=256, precision=P.PREC_FP32, box_pc=1.0, cooling=P.COOL_EXACT, temp_wi
     p = P.HydroParams(nx=n, ny=n, nz=n, xl=-L / 2, xr=L / 2, yl=-L / 2, yr=L / 2, zl=-L / 2,
                       zr=L / 2, gamma=5.0 / 3.0, cfl=0.4, bc=(P.BC_OUTFLOW,) * 6,
                       precision=precision, cooling=cooling, winds=[w])
-    return p, uniform_state(p, 1.0 * p.mu0 * P.AMU, 1.0 * P.KB * 1e4)
+    return p, uniform_state(p, n_amb * p.mu0 * P.AMU, n_amb * P.KB * t_amb)
 
 
 def cluster(n=1024, precision=P.PREC_FP32, seed=7, box_pc=4.0, cooling=P.COOL_SURROGATE,
