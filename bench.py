@@ -1,27 +1,32 @@
 """Benchmark of the HydroCore time step on one MI355X.
 
 Workload (the configuration the headline metric is quoted on): the clumpy
-self-gravitating medium on a 512^3 mesh -- lognormal density (mean n = 1
-cm^-3, sigma_ln rho = 1) from a seeded k^-4 Gaussian random field, RMS Mach 2
-velocities, T = 1e4 K, a central 1e4 Msun point mass softened over 2 cells,
-periodic boundaries -- advanced with HLLC + van Leer, FP32 sweeps with FP64
-energy recovery and accumulation, artificial viscosity, exact tabulated
-cooling (Z1.0.dat) and point-mass gravity.  A step is one iteration of the
-reference's main loop (source/mainHD.f90:74-99): two fused kernels.
+medium on a 512^3 mesh -- lognormal density (mean n = 1 cm^-3, sigma_ln rho =
+1) from a seeded k^-4 Gaussian random field, RMS Mach 2 velocities, T = 1e4 K,
+a central 1e4 Msun point mass softened over 2 cells, periodic boundaries --
+advanced with HLLC + van Leer, FP32 sweeps with FP64 energy recovery and
+accumulation, artificial viscosity, exact tabulated cooling (Z1.0.dat) and
+point-mass gravity.  A step is one iteration of the reference's main loop
+(source/mainHD.f90:74-99): two fused kernels.  (The configuration's
+self-gravity runs in a leg of its own.)
 
 Prints one JSON line: zone-updates/s over K timed steps (inputs resident in
-HBM), the HBM roofline of the dominant kernel (stage2), the CPU baseline (the
-FP64 C oracle with OpenMP on a bounded sample of the same workload), the L1
-error of the FP32 path against the FP64 path on the same problem, and the
-cooling paths compared on the same mesh: the reference's one-step formula
-fused into stage 2 (the headline value), the learned surrogate with its
-per-cell fallback (trained in this run on the GPU), and the converged
-integrator (the surrogate's target and fallback), on the workload as given,
-at T = 3e5 K, and at T = 3e5 K with every density times 100 (strongly
-cooling: dt/t_cool of order 1-10, hundreds of integrator substeps per cell).
-A last leg runs the same workload with self-gravity (the multigrid Poisson
-solve of MultiGrid, multipole boundary values every 10 steps) and reports
-its V-cycles and time per step.
+HBM; the median of 5 repetitions, timed with HIP events around the steps
+only), the HBM roofline of the dominant kernel (stage2, from a separate run
+with per-launch events), the CPU baselines (the FP64 C oracle with OpenMP on
+all host threads and on one, on the planar shock tube, the blast and a
+bounded sample of the clumpy workload), the L1 error of the FP32 path
+against the FP64 path on the same problem, and the cooling paths compared on
+the same mesh: the reference's one-step formula fused into stage 2 (the
+headline value), the learned surrogate with its per-cell fallback (trained in
+this run on the GPU; with its default cost threshold, and with every cell
+above 0.01 t_cool handed to the network), and the converged integrator (the
+surrogate's target and fallback), on the workload as given, at T = 3e5 K,
+and at T = 3e5 K with every density times 100 (strongly cooling).  Further
+legs: the cluster-wind bubble on a 1024^3 mesh (200 steps, surrogate
+cooling, per-step fallback fraction), and the workload with self-gravity
+(the multigrid Poisson solve of MultiGrid, multipole boundary values every
+10 steps).
 """
 from __future__ import annotations
 
@@ -56,13 +61,15 @@ def sg_traffic():
     k = read_traffic(SG_TRAFFIC_REPORT)["kernels"].get("k_sg_rbz")
     return k["bytes"] if k else None
 COPY_PROBE_BYTES = 2 << 30
+NET_THRESHOLD = 0.3         # TLQ_SURROGATE_MIN_DTC (include/tlq.h)
+REPEATS = 5                 # timed repetitions of the headline; value = their median
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def timed_run(p, u0, table, warmup, steps, theta=None):
+def timed_run(p, u0, table, warmup, steps, theta=None, threshold=None):
     """Advance a fresh context over `steps` timed steps; returns
     (zone-updates/s, ms per step, per-kernel ms, (queued cells, fallbacks) of
     the last step, bad cells)."""
@@ -70,6 +77,8 @@ def timed_run(p, u0, table, warmup, steps, theta=None):
     with HydroCore(p, cooling_table=table) as h:
         if theta is not None:
             h.surrogate_params(theta)
+        if threshold is not None:
+            h.surrogate_threshold(threshold)
         h.set_state(u0)
         h.advance(warmup)
         h.sync()
@@ -117,22 +126,30 @@ def cooling_modes(p, u0, table, warmup, steps):
     for label, make in variants:
         u = make()
         res = {}
-        for name, mode in (("reference_one_step", P.COOL_EXACT),
-                           ("surrogate", P.COOL_SURROGATE),
-                           ("converged_integrator", P.COOL_SUBCYCLED)):
+        for name, mode, thr in (("reference_one_step", P.COOL_EXACT, None),
+                                ("surrogate", P.COOL_SURROGATE, None),
+                                ("surrogate_all_cells", P.COOL_SURROGATE, 0.01),
+                                ("converged_integrator", P.COOL_SUBCYCLED, None)):
             if label == "T=1e4K" and mode == P.COOL_EXACT:
                 continue  # the headline run
             zps, ms, kms, (nq, nf), bad = timed_run(p.replace(cooling=mode), u, table, warmup,
                                                      steps,
-                                                     theta if mode == P.COOL_SURROGATE else None)
+                                                     theta if mode == P.COOL_SURROGATE else None,
+                                                     threshold=thr)
             res[name] = {"zone_updates_per_s": zps, "ms_per_step": ms,
                          "cooling_pass_ms": kms.get("cooling"), "nonfinite_cells": bad}
             if mode != P.COOL_EXACT:
+                # mode 2: cells handed to the network; mode 3: to the integrator
                 res[name]["queued_fraction"] = nq / p.zones
             if mode == P.COOL_SURROGATE:
+                res[name]["network_threshold_dtc"] = thr if thr is not None else NET_THRESHOLD
                 res[name]["fallback_fraction"] = nf / nq if nq else 0.0
             log(f"bench: {label} {name}: {zps:.4g} zone-updates/s, {ms:.3f} ms/step, "
                 f"kernels {kms}, queued {nq}, fallbacks {nf}")
+        if "surrogate_all_cells" in res and "converged_integrator" in res:
+            res["surrogate_all_cells_speedup_vs_converged"] = (
+                res["surrogate_all_cells"]["zone_updates_per_s"] /
+                res["converged_integrator"]["zone_updates_per_s"])
         if "surrogate" in res and "converged_integrator" in res:
             res["surrogate_speedup_vs_converged"] = (res["surrogate"]["zone_updates_per_s"] /
                                                      res["converged_integrator"]["zone_updates_per_s"])
@@ -141,7 +158,7 @@ def cooling_modes(p, u0, table, warmup, steps):
                 res["cooling_pass_speedup"] = ci / cs
         out[label] = res
         del u
-    return out
+    return out, theta
 
 
 def self_gravity_leg(p, u0, table, warmup, steps):
@@ -199,13 +216,10 @@ def host_cpu() -> dict:
     return {"cpu_model": model, "logical_cpus": os.cpu_count()}
 
 
-def cpu_baseline(n: int, steps: int, threads: int):
-    """The FP64 C oracle on the same workload at n^3 (bounded sample)."""
+def cpu_run(p, u0, table, steps: int, threads: int):
+    """The FP64 C oracle on (p, u0): zone-updates/s over `steps` steps after
+    one warm-up step, and the seconds they took."""
     import oracle
-    from tlaloque_amd import params as P
-    from tlaloque_amd import problems
-    p, u0 = problems.clumpy(n=n, precision=P.PREC_FP64)
-    table = P.load_cooling_table(os.path.join(ROOT, "tlaloque_amd", "data", "Z1.0.dat"))
     o = oracle.Oracle(p, table=table)
     o.set_cons(u0)
     oracle.set_threads(threads)
@@ -214,7 +228,81 @@ def cpu_baseline(n: int, steps: int, threads: int):
     for _ in range(steps):
         o.advance()
     el = time.perf_counter() - t0
-    return n ** 3 * steps / el, el
+    return p.zones * steps / el, el
+
+
+def cpu_baselines(threads: int) -> dict:
+    """The FP64 C oracle (the reference's algorithm restated, bit for bit equal
+    to the reference's runs) on the host cores: configuration 1 (planar shock
+    tube 128x32x32), configuration 2 (blast 256^3, HLLC) and a bounded sample
+    of the clumpy workload, each on all `threads` host threads and on one."""
+    from tlaloque_amd import params as P
+    from tlaloque_amd import problems
+    table = P.load_cooling_table(os.path.join(ROOT, "tlaloque_amd", "data", "Z1.0.dat"))
+    cases = (
+        ("C1 planar shock tube 128x32x32 FP64", lambda: problems.sod_planar(), None, (40, 10)),
+        ("C2 blast 256^3 HLLC FP64", lambda: problems.blast(n=256), None, (5, 1)),
+        ("C4 clumpy 256^3 FP64 (sample of the 512^3 workload)",
+         lambda: problems.clumpy(n=256, precision=P.PREC_FP64), table, (10, 1)),
+    )
+    out = {}
+    for name, make, tab, (steps_n, steps_1) in cases:
+        p, u0 = make()
+        zn, en = cpu_run(p, u0, tab, steps_n, threads)
+        z1, e1 = cpu_run(p, u0, tab, steps_1, 1)
+        out[name] = {"all_threads": {"value": zn, "cores": threads, "steps": steps_n,
+                                     "seconds": en},
+                     "one_thread": {"value": z1, "cores": 1, "steps": steps_1, "seconds": e1}}
+        log(f"bench: CPU {name}: {zn:.4g} zone-updates/s on {threads} threads, {z1:.4g} on 1")
+        del u0
+    return out
+
+
+def cluster_leg(theta, table, steps: int = 200, n: int = 1024) -> dict:
+    """Configuration 5: the cluster-wind bubble on a 1024^3 mesh (8 winds,
+    point masses, surrogate cooling with the per-cell fallback), `steps` steps
+    enqueued at once (one synchronisation), then 20 more steps one at a time
+    for the per-step counts of the cooling pass."""
+    from tlaloque_amd import problems
+    from tlaloque_amd.hydro import HydroCore
+    t0 = time.perf_counter()
+    p, u0 = problems.cluster(n=n)
+    log(f"bench: cluster {n}^3 initial state built in {time.perf_counter() - t0:.1f}s")
+    with HydroCore(p, cooling_table=table) as h:
+        h.surrogate_params(theta)
+        h.surrogate_threshold(0.01)  # every cell above 0.01 t_cool to the network
+        h.set_state(u0)
+        del u0
+        h.advance(3)
+        h.sync()
+        h.timer_start()
+        h.advance(steps)
+        ms = h.timer_stop()
+        bad = h.check_state()
+        h.kernel_timing(True)
+        fracs, queued = [], []
+        for _ in range(20):
+            h.advance(1)
+            nq, nf = h.cooling_counts()
+            queued.append(nq)
+            fracs.append(nf / nq if nq else 0.0)
+        ks = h.kernel_stats()
+        h.kernel_timing(False)
+    zps = p.zones * steps / (ms / 1e3)
+    s2 = ks["stage2"][0] / max(ks["stage2"][1], 1)
+    ach = BYTES_STAGE2 * p.zones / (s2 / 1e3) / 1e9
+    out = {"zone_updates_per_s": zps, "ms_per_step": ms / steps, "steps": steps,
+           "mesh": [n, n, n], "nonfinite_cells": bad,
+           "stage2_roofline": {"achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": ach / HBM_PEAK_GBS, "avg_launch_ms": s2},
+           "step_roofline_frac": zps * BYTES_STEP / 1e9 / HBM_PEAK_GBS,
+           "kernels_ms": {k: v[0] / max(v[1], 1) for k, v in ks.items() if v[1]},
+           "network_cells_per_step": {"mean": float(np.mean(queued)), "max": int(max(queued))},
+           "fallback_fraction_per_step": {"mean": float(np.mean(fracs)),
+                                          "max": float(np.max(fracs))},
+           "network_threshold_dtc": 0.01}
+    log(f"bench: cluster {n}^3: {out}")
+    return out
 
 
 def l1_fp32_vs_fp64(n: int, steps: int):
@@ -251,6 +339,7 @@ def main():
     ap.add_argument("--no-l1", action="store_true")
     ap.add_argument("--no-modes", action="store_true", help="skip the cooling-path comparison")
     ap.add_argument("--no-gravity", action="store_true", help="skip the self-gravity leg")
+    ap.add_argument("--no-cluster", action="store_true", help="skip the 1024^3 cluster leg")
     ap.add_argument("--gravity-only", action="store_true",
                     help="run only the self-gravity leg and print its JSON (for profiles)")
     args = ap.parse_args()
@@ -279,10 +368,15 @@ def main():
     h.advance(args.warmup)
     h.sync()
     log(f"bench: {args.warmup} warm-up steps done")
+    reps = []
+    for _ in range(REPEATS):  # HIP events around the steps only
+        h.timer_start()
+        h.advance(args.steps)
+        reps.append(h.timer_stop())
+    ms = float(np.median(reps))
+    # per-kernel launch times from a separate instrumented window
     h.kernel_timing(True)
-    h.timer_start()
     h.advance(args.steps)
-    ms = h.timer_stop()
     ks = h.kernel_stats()
     h.kernel_timing(False)
     bad = h.check_state()
@@ -295,7 +389,8 @@ def main():
     s1_ms = ks["stage1"][0] / max(ks["stage1"][1], 1)
     achieved = BYTES_STAGE2 * zones / (s2_ms / 1e3) / 1e9
     copy_gbs = 2 * COPY_PROBE_BYTES / (copy_ms / 1e3) / 1e9
-    log(f"bench: {args.steps} steps in {ms:.1f} ms; stage1 {s1_ms:.3f} ms, stage2 {s2_ms:.3f} ms; "
+    log(f"bench: {args.steps} steps in {ms:.1f} ms (median of {REPEATS}: "
+        f"{', '.join(f'{r:.1f}' for r in reps)}); stage1 {s1_ms:.3f} ms, stage2 {s2_ms:.3f} ms; "
         f"bad cells {bad}; t={t:.4e} it={it}; copy probe {copy_gbs:.0f} GB/s")
     traffic, traffic_note = None, None
     if n == 512 and os.path.exists(TRAFFIC_REPORT):
@@ -319,9 +414,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 sweeps, f64 energy recovery/accumulation",
         "data": "synthetic (seeded k^-4 lognormal clumpy medium)",
-        "config": {"workload": f"clumpy self-gravitating medium {n}^3, HLLC+van Leer, exact "
-                               "Z1.0 cooling, 1e4 Msun softened point mass, periodic",
-                   "mesh": [n, n, n], "precision": "fp32+fp64-guard", "cooling": "exact"},
+        "config": {"workload": f"clumpy medium {n}^3 (configuration 4: lognormal density, "
+                               "Mach 2, T = 1e4 K), HLLC+van Leer, exact Z1.0 cooling, 1e4 Msun "
+                               "softened point mass, periodic; self-gravity in its own leg",
+                   "mesh": [n, n, n], "precision": "fp32+fp64-guard", "cooling": "exact",
+                   "repeats": REPEATS, "ms_per_step_repeats": [r / args.steps for r in reps]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_note,
@@ -333,8 +430,9 @@ def main():
         "kernels_ms": {"stage1": s1_ms, "stage2": s2_ms},
         "nonfinite_cells": bad,
     }
+    theta = None
     if not args.no_modes:
-        result["cooling_paths"] = cooling_modes(p, u0, table, args.warmup, args.steps)
+        result["cooling_paths"], theta = cooling_modes(p, u0, table, args.warmup, args.steps)
     if not args.no_gravity:
         result["self_gravity"] = self_gravity_leg(p, u0, table, args.warmup, args.steps)
     del u0
@@ -342,20 +440,28 @@ def main():
         t0 = time.perf_counter()
         result["l1_vs_fp64"] = l1_fp32_vs_fp64(128, 10)
         log(f"bench: L1 check done in {time.perf_counter() - t0:.1f}s: {result['l1_vs_fp64']}")
+    if not args.no_cluster:
+        if theta is None:
+            from tlaloque_amd import surrogate
+            theta, _ = surrogate.train()
+        result["cluster_1024"] = cluster_leg(theta, table)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         t0 = time.perf_counter()
-        zps, el = cpu_baseline(256, 20, threads)
-        zps1, el1 = cpu_baseline(128, 4, 1)
-        log(f"bench: CPU baseline done in {time.perf_counter() - t0:.1f}s")
-        result["cpu_baseline"] = {"value": zps, "unit": "zone-updates/s", "cores": threads,
-                                  "kind": "port",
-                                  "sample": f"FP64 C oracle (OpenMP, {threads} threads), same "
-                                            f"clumpy workload at 256^3, 20 steps ({el:.1f} s)",
+        cases = cpu_baselines(threads)
+        log(f"bench: CPU baselines done in {time.perf_counter() - t0:.1f}s")
+        c4 = cases["C4 clumpy 256^3 FP64 (sample of the 512^3 workload)"]
+        result["cpu_baseline"] = {"value": c4["all_threads"]["value"], "unit": "zone-updates/s",
+                                  "cores": threads, "kind": "port",
+                                  "sample": f"FP64 C oracle (OpenMP, {threads} threads), the same "
+                                            f"clumpy workload at 256^3, "
+                                            f"{c4['all_threads']['steps']} steps "
+                                            f"({c4['all_threads']['seconds']:.1f} s)",
                                   "host": host_cpu(),
-                                  "single_thread": {"value": zps1, "cores": 1,
-                                                    "sample": f"same oracle, 1 thread, clumpy 128^3, "
-                                                              f"4 steps ({el1:.1f} s)"}}
+                                  "single_thread": {"value": c4["one_thread"]["value"], "cores": 1,
+                                                    "sample": "same oracle and workload, 1 thread, "
+                                                              f"{c4['one_thread']['steps']} step"},
+                                  "configurations": cases}
     print(json.dumps(result), flush=True)
 
 
