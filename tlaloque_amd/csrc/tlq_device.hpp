@@ -223,13 +223,22 @@ template <class R> __device__ __forceinline__ R lim_avg(int opt, R a, R b) {
     }
 }
 
-// wavespeed, HLLC.f90:498-518 (Davis estimates and the contact speed).
+// Sound speed of a primitive state, sqrt(gamma p / rho) (csound,
+// HydroCore.f90:157-168, as wavespeed forms it).  A function of the state
+// alone: a kernel whose face states are cell states computes it once per
+// cell and hands it to the *_c solvers below with the same result.
+template <class R> __device__ __forceinline__ R sound_speed(R gamma, const R w[5]) {
+    return rsqrt_(rdiv(gamma * w[4], w[0]));
+}
+
+// wavespeed, HLLC.f90:498-518 (Davis estimates and the contact speed), from
+// the two states and their sound speeds.  (wavespeed below is the same with
+// the sound speeds formed in place: the source order the compiler schedules
+// best in the reconstructed-state kernel.)
 template <int D, class R>
-__device__ __forceinline__ void wavespeed(R gamma, const R wl[5], const R wr[5], R &sl, R &sr,
-                                          R &sst) {
+__device__ __forceinline__ void wavespeed_c(const R wl[5], const R wr[5], R csl, R csr, R &sl,
+                                            R &sr, R &sst) {
     constexpr int n = D + 1;
-    const R csl = rsqrt_(rdiv(gamma * wl[4], wl[0]));
-    const R csr = rsqrt_(rdiv(gamma * wr[4], wr[0]));
     if constexpr (sizeof(R) == 8) {
         sl = rmin(wl[n] - csl, wr[n] - csr);
         sr = rmax(wl[n] + csl, wr[n] + csr);
@@ -253,6 +262,14 @@ __device__ __forceinline__ void wavespeed(R gamma, const R wl[5], const R wr[5],
     }
 }
 
+template <int D, class R>
+__device__ __forceinline__ void wavespeed(R gamma, const R wl[5], const R wr[5], R &sl, R &sr,
+                                          R &sst) {
+    const R csl = rsqrt_(rdiv(gamma * wl[4], wl[0]));
+    const R csr = rsqrt_(rdiv(gamma * wr[4], wr[0]));
+    wavespeed_c<D>(wl, wr, csl, csr, sl, sr, sst);
+}
+
 // One HLLC face (HLLC.f90:99-135, 275-314): wl/wr reconstructed primitive
 // states, ul/ur the cell-centred conserved states of the two cells.  The
 // region is selected without branches (supersonic left, supersonic right,
@@ -262,12 +279,13 @@ __device__ __forceinline__ void wavespeed(R gamma, const R wl[5], const R wr[5],
 //   rs (E/rho + (S*-u)(S* + p/(rho (S-u))))
 //     = q E + rs (S*-u) S* + (S*-u) p / (S - S*),  q = (S-u)/(S-S*), rs = rho q
 // with one reciprocal.
-template <int D, class R>
-__device__ __forceinline__ void hllc_face(R gamma, R gratio, const R wl[5], const R wr[5],
-                                          const R ul[5], const R ur[5], R f[5]) {
+// `speeds(sl, sr, sst)` supplies the wave speeds.
+template <int D, class R, class WS>
+__device__ __forceinline__ void hllc_body(R gratio, const R wl[5], const R wr[5], const R ul[5],
+                                          const R ur[5], WS &&speeds, R f[5]) {
     constexpr int n = D + 1;
     R sl, sr, sst;
-    wavespeed<D>(gamma, wl, wr, sl, sr, sst);
+    speeds(sl, sr, sst);
     const bool supl = sl >= R(0), supr = sr <= R(0);
     const bool left = supl || (!supr && sst >= R(0));
     const bool star = !supl && !supr;
@@ -299,12 +317,27 @@ __device__ __forceinline__ void hllc_face(R gamma, R gratio, const R wl[5], cons
     for (int e = 0; e < 5; ++e) f[e] = star ? fs[e] + s * (us[e] - u[e]) : fs[e];
 }
 
-// One HLL face (HLL.f90:106-128, 197-223).
 template <int D, class R>
-__device__ __forceinline__ void hll_face(R gamma, R gratio, const R wl[5], const R wr[5],
-                                         const R ul[5], const R ur[5], R f[5]) {
+__device__ __forceinline__ void hllc_face(R gamma, R gratio, const R wl[5], const R wr[5],
+                                          const R ul[5], const R ur[5], R f[5]) {
+    hllc_body<D>(gratio, wl, wr, ul, ur,
+                 [&](R &sl, R &sr, R &sst) { wavespeed<D>(gamma, wl, wr, sl, sr, sst); }, f);
+}
+
+// The same with the sound speeds of the two states given.
+template <int D, class R>
+__device__ __forceinline__ void hllc_face_c(R gratio, const R wl[5], const R wr[5], const R ul[5],
+                                            const R ur[5], R csl, R csr, R f[5]) {
+    hllc_body<D>(gratio, wl, wr, ul, ur,
+                 [&](R &sl, R &sr, R &sst) { wavespeed_c<D>(wl, wr, csl, csr, sl, sr, sst); }, f);
+}
+
+// One HLL face (HLL.f90:106-128, 197-223).
+template <int D, class R, class WS>
+__device__ __forceinline__ void hll_body(R gratio, const R wl[5], const R wr[5], const R ul[5],
+                                         const R ur[5], WS &&speeds, R f[5]) {
     R sl, sr, sst;
-    wavespeed<D>(gamma, wl, wr, sl, sr, sst);
+    speeds(sl, sr, sst);
     if (sl >= R(0)) {
         prim2f<D>(gratio, wl, f);
     } else if (sr <= R(0)) {
@@ -321,6 +354,20 @@ __device__ __forceinline__ void hll_face(R gamma, R gratio, const R wl[5], const
             else f[e] = num * inv;
         }
     }
+}
+
+template <int D, class R>
+__device__ __forceinline__ void hll_face(R gamma, R gratio, const R wl[5], const R wr[5],
+                                         const R ul[5], const R ur[5], R f[5]) {
+    hll_body<D>(gratio, wl, wr, ul, ur,
+                [&](R &sl, R &sr, R &sst) { wavespeed<D>(gamma, wl, wr, sl, sr, sst); }, f);
+}
+
+template <int D, class R>
+__device__ __forceinline__ void hll_face_c(R gratio, const R wl[5], const R wr[5], const R ul[5],
+                                           const R ur[5], R csl, R csr, R f[5]) {
+    hll_body<D>(gratio, wl, wr, ul, ur,
+                [&](R &sl, R &sr, R &sst) { wavespeed_c<D>(wl, wr, csl, csr, sl, sr, sst); }, f);
 }
 
 template <int D, class R>
@@ -632,13 +679,14 @@ __device__ __forceinline__ bool winds_at_mask(const Problem &P, unsigned long lo
 // Replaces v by the state of the last wind of `mask` that covers (i, j, k).
 // The wind state passes through a private array only inside the branch, so
 // v itself stays in registers (no scratch traffic when no wind is near).
+template <class V>
 __device__ __forceinline__ bool wind_override(const Problem &P, unsigned long long mask, int i,
-                                              int j, int k, double v[5]) {
+                                              int j, int k, V v[5]) {
     if (!mask) return false;
     double o[5];
     if (!winds_at_mask(P, mask, i, j, k, o)) return false;
 #pragma unroll
-    for (int e = 0; e < 5; ++e) v[e] = o[e];
+    for (int e = 0; e < 5; ++e) v[e] = (V)o[e];
     return true;
 }
 

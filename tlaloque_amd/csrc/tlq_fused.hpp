@@ -79,10 +79,10 @@ __device__ __forceinline__ void ghost_images(int a, int n, int bcl, int bch, boo
 // the boundary fill copies from it, then lets the winds of `wmask` override
 // ghosts inside a wind sphere (boundaries, then user_boundaries:
 // boundaries.f90:11-24).
-template <class R>
+template <class R, class V>
 __device__ __forceinline__ void write_face_ghosts(const Problem &P, unsigned long long wmask,
                                                   R *__restrict__ out, int i, int j, int k,
-                                                  const double v[5], bool two) {
+                                                  const V v[5], bool two) {
     const Grid &g = P.g;
     auto put = [&](int gi, int gj, int gk, int comp, bool neg) {
         double w[5];
@@ -212,6 +212,14 @@ __device__ __forceinline__ void riemann_t(R gamma, R gratio, const R wl[5], cons
     else hllc_face<D>(gamma, gratio, wl, wr, ul, ur, f);
 }
 
+// The same with the sound speeds of the two states given.
+template <int D, int SOL, class R>
+__device__ __forceinline__ void riemann_c(R gratio, const R wl[5], const R wr[5], const R ul[5],
+                                          const R ur[5], R csl, R csr, R f[5]) {
+    if constexpr (SOL == SOLVER_HLL) hll_face_c<D>(gratio, wl, wr, ul, ur, csl, csr, f);
+    else hllc_face_c<D>(gratio, wl, wr, ul, ur, csl, csr, f);
+}
+
 // Godunov update of one cell: U - dtdx dF - dtdy dG - dtdz dH.  FP64 mode
 // keeps the reference's operation order (HLLC.f90:37-38); FP32 mode forms
 // the flux-difference sum in FP32 and accumulates it onto U in FP64.
@@ -223,6 +231,22 @@ __device__ __forceinline__ double godunov1(R u, R fp, R fm, R gp, R gm, R hp, R 
     } else {
         const float d = (float)dtdx * (fp - fm) + (float)dtdy * (gp - gm) + (float)dtdz * (hp - hm);
         return (double)u - (double)d;
+    }
+}
+
+// The same plus the time-scaled source src (when grav), in the sweep type:
+// FP64 mode adds the source after the update (the reference's order); FP32
+// mode forms the whole increment in FP32 and applies it with one rounding
+// (which is what the FP64 sum of the two FP32 values rounds to as well).
+template <class R>
+__device__ __forceinline__ R godunov_src(R u, R fp, R fm, R gp, R gm, R hp, R hm, double dtdx,
+                                         double dtdy, double dtdz, bool grav, R src) {
+    if constexpr (sizeof(R) == 8) {
+        const double v = godunov1<R>(u, fp, fm, gp, gm, hp, hm, dtdx, dtdy, dtdz);
+        return grav ? v + src : v;
+    } else {
+        const float d = (float)dtdx * (fp - fm) + (float)dtdy * (gp - gm) + (float)dtdz * (hp - hm);
+        return u - (d - src);
     }
 }
 
@@ -254,6 +278,21 @@ __device__ __forceinline__ void gravity_src(const Problem &P, int i, int j, int 
         s[2] = -rg1;
         s[3] = -rg2;
         s[4] = -(w[1] * rg0 + w[2] * rg1 + w[3] * rg2);
+    }
+}
+
+// The gravity source times the step h, in the sweep type.
+template <class R>
+__device__ __forceinline__ void gravity_src_h(const Problem &P, int i, int j, int k, const R w[5],
+                                              const float *gpre, double h, R s[5]) {
+    double sd[5];
+    gravity_src<R>(P, i, j, k, w, gpre, sd);
+    if constexpr (sizeof(R) == 8) {
+#pragma unroll
+        for (int e = 0; e < 5; ++e) s[e] = h * sd[e];
+    } else {
+#pragma unroll
+        for (int e = 0; e < 5; ++e) s[e] = (float)h * (float)sd[e];
     }
 }
 
@@ -325,6 +364,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
     constexpr int LX = TX + 2, LY = TY + 2;
     __shared__ R sW[5][LY][LX];
     __shared__ R sU[5][LY][LX];
+    __shared__ R sC[LY][LX];  // sound speed of each cell, formed once
     __shared__ R sF[5][TY][TX + 1];
     __shared__ R sG[5][TY + 1][TX];
     __shared__ unsigned long long s_wmask;
@@ -362,6 +402,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
     float pf_g[3] = {};
     load_col(U, pl, cc, z0 - 1, uc);
     prims(P, uc, wc);
+    R csc = sound_speed(gam, wc);
     load_col(U, pl, cc, z0, pf_n);
     if (ht.kind) load_col(U, pl, ch, z0 - 1, pf_h);
 
@@ -382,7 +423,8 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
             load_gacc(gacc, pl, cc, ku + 1, pf_g);
         }
         prims(P, un, wn);
-        riemann_t<2, SOL>(gam, gr, wc, wn, uc, un, hp);
+        const R csn = sound_speed(gam, wn);
+        riemann_c<2, SOL>(gr, wc, wn, uc, un, csc, csn, hp);
         // No barrier before the plane's stores: sW and sU were last read by
         // the previous plane's faces, which its flux barrier closed, and sF
         // and sG are written only after this plane's next barrier.
@@ -391,6 +433,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
             sW[e][ty + 1][tx + 1] = wc[e];
             sU[e][ty + 1][tx + 1] = uc[e];
         }
+        sC[ty + 1][tx + 1] = csc;
         if (ht.kind) {
             R wh[5];
             prims(P, hr, wh);
@@ -399,6 +442,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
                 sW[e][ht.ly + 1][ht.lx + 1] = wh[e];
                 sU[e][ht.ly + 1][ht.lx + 1] = hr[e];
             }
+            sC[ht.ly + 1][ht.lx + 1] = sound_speed(gam, wh);
         }
         lds_barrier();
         // Faces: the left x face and the lower y face of every tile cell,
@@ -407,25 +451,25 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
         // directions in one pass.  The cell's own state comes from registers
         // (it is what this thread stored), and so do the fluxes of its own
         // two faces in the update.
-        auto xface = [&](int fx, int fy, const R *wr, const R *ur, R f[5]) {  // cells fx-1 | fx
+        auto xface = [&](int fx, int fy, const R *wr, const R *ur, R csr, R f[5]) {  // cells fx-1 | fx
             R wl[5], ul[5];
 #pragma unroll
             for (int e = 0; e < 5; ++e) {
                 wl[e] = sW[e][fy + 1][fx];
                 ul[e] = sU[e][fy + 1][fx];
             }
-            riemann_t<0, SOL>(gam, gr, wl, wr, ul, ur, f);
+            riemann_c<0, SOL>(gr, wl, wr, ul, ur, sC[fy + 1][fx], csr, f);
 #pragma unroll
             for (int e = 0; e < 5; ++e) sF[e][fy][fx] = f[e];
         };
-        auto yface = [&](int fx, int fy, const R *wr, const R *ur, R f[5]) {  // rows fy-1 | fy
+        auto yface = [&](int fx, int fy, const R *wr, const R *ur, R csr, R f[5]) {  // rows fy-1 | fy
             R wl[5], ul[5];
 #pragma unroll
             for (int e = 0; e < 5; ++e) {
                 wl[e] = sW[e][fy][fx + 1];
                 ul[e] = sU[e][fy][fx + 1];
             }
-            riemann_t<1, SOL>(gam, gr, wl, wr, ul, ur, f);
+            riemann_c<1, SOL>(gr, wl, wr, ul, ur, sC[fy][fx + 1], csr, f);
 #pragma unroll
             for (int e = 0; e < 5; ++e) sG[e][fy][fx] = f[e];
         };
@@ -437,36 +481,30 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
                 wr[e] = sW[e][fy + 1][fx + 1];
                 ur[e] = sU[e][fy + 1][fx + 1];
             }
-            if (xdir) xface(fx, fy, wr, ur, f);
-            else yface(fx, fy, wr, ur, f);
+            const R csr = sC[fy + 1][fx + 1];
+            if (xdir) xface(fx, fy, wr, ur, csr, f);
+            else yface(fx, fy, wr, ur, csr, f);
         };
         R fxm[5], gym[5];
-        xface(tx, ty, wc, uc, fxm);
-        yface(tx, ty, wc, uc, gym);
+        xface(tx, ty, wc, uc, csc, fxm);
+        yface(tx, ty, wc, uc, csc, gym);
         if (tid < TX) edge(tid, TY, false);
         else if (tid >= XW && tid < XW + TY) edge(TX, tid - XW, true);
         lds_barrier();
         if (active && k >= z0) {
-            double v[5];
+            R s[5] = {};
+            if (grav) gravity_src_h<R>(P, i, j, k, wc, gacc ? gk : nullptr, dth, s);
+            R v[5];
 #pragma unroll
             for (int e = 0; e < 5; ++e)
-                v[e] = godunov1<R>(uc[e], sF[e][ty][tx + 1], fxm[e], sG[e][ty + 1][tx], gym[e],
-                                   hp[e], hm[e], dtdx, dtdy, dtdz);
-            if (grav) {
-                double s[5];
-                gravity_src<R>(P, i, j, k, wc, gacc ? gk : nullptr, s);
-#pragma unroll
-                for (int e = 0; e < 5; ++e) v[e] = v[e] + dth * s[e];
-            }
+                v[e] = godunov_src<R>(uc[e], sF[e][ty][tx + 1], fxm[e], sG[e][ty + 1][tx], gym[e],
+                                      hp[e], hm[e], dtdx, dtdy, dtdz, grav, s[e]);
             const unsigned long long wmask = s_wmask;
-            double o[5];
+            R o[5];
 #pragma unroll
             for (int e = 0; e < 5; ++e) o[e] = v[e];
             wind_override(P, wmask, i, j, k, o);
-            R oh[5];
-#pragma unroll
-            for (int e = 0; e < 5; ++e) oh[e] = (R)o[e];
-            store_col(Uh, pl, cc, ku, oh);
+            store_col(Uh, pl, cc, ku, o);
             write_face_ghosts(P, wmask, Uh, i, j, k, v, true);
         }
 #pragma unroll
@@ -475,6 +513,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
             uc[e] = un[e];
             wc[e] = wn[e];
         }
+        csc = csn;
     }
 }
 
