@@ -261,6 +261,13 @@ int tlq_kernel_stats(tlq_ctx *ctx, int index, char *name, int name_len, double *
  * (calibrates HBM counters and bandwidth; no counterpart in the reference). */
 int tlq_copy_probe(tlq_ctx *ctx, long long bytes, int iters, double *ms_per_copy);
 
+/* The same copy in one configuration: `unroll` (1, 2, 4 or 8) independent
+ * 16-byte loads in flight per thread, `blocks` blocks of 256 threads, and
+ * nontemporal loads and stores when `nontemporal` is nonzero (a sweep over
+ * these finds the practical HBM ceiling; no counterpart in the reference). */
+int tlq_copy_probe_config(tlq_ctx *ctx, long long bytes, int iters, int unroll, int blocks,
+                          int nontemporal, double *ms_per_copy);
+
 /* Sizes: bytes of one device field and the number of cells per field. */
 int tlq_layout(tlq_ctx *ctx, long long *cells_per_field, int *pitch_x, int *origin_x);
 

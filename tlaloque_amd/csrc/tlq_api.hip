@@ -1246,9 +1246,56 @@ __global__ void k_stream_copy(const float4 *__restrict__ a, float4 *__restrict__
         b[t] = a[t];
 }
 
-int tlq_copy_probe(tlq_ctx *c, long long bytes, int iters, double *ms) {
+// The same copy with UNR independent 16-byte loads in flight per thread
+// (all issued before the first store) and, when NT, nontemporal loads and
+// stores (the streamed data bypass the caches' normal retention).
+extern "C++" {
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+template <int UNR, bool NT>
+__global__ void k_stream_copy_x(const f4v *__restrict__ a, f4v *__restrict__ b, long long n) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (; t + (UNR - 1) * stride < n; t += UNR * stride) {
+        f4v v[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            if constexpr (NT) v[u] = __builtin_nontemporal_load(a + t + u * stride);
+            else v[u] = a[t + u * stride];
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            if constexpr (NT) __builtin_nontemporal_store(v[u], b + t + u * stride);
+            else b[t + u * stride] = v[u];
+        }
+    }
+    for (; t < n; t += stride) b[t] = a[t];
+}
+
+static hipError_t launch_copy(hipStream_t s, int unroll, bool nt, unsigned blocks, const float4 *a4,
+                              float4 *b4, long long n) {
+    const dim3 g(blocks), t(256);
+    const f4v *a = reinterpret_cast<const f4v *>(a4);
+    f4v *b = reinterpret_cast<f4v *>(b4);
+    if (unroll == 1 && !nt) hipLaunchKernelGGL(k_stream_copy, g, t, 0, s, a4, b4, n);
+    else if (unroll == 1) hipLaunchKernelGGL((k_stream_copy_x<1, true>), g, t, 0, s, a, b, n);
+    else if (unroll == 2 && !nt) hipLaunchKernelGGL((k_stream_copy_x<2, false>), g, t, 0, s, a, b, n);
+    else if (unroll == 2) hipLaunchKernelGGL((k_stream_copy_x<2, true>), g, t, 0, s, a, b, n);
+    else if (unroll == 4 && !nt) hipLaunchKernelGGL((k_stream_copy_x<4, false>), g, t, 0, s, a, b, n);
+    else if (unroll == 4) hipLaunchKernelGGL((k_stream_copy_x<4, true>), g, t, 0, s, a, b, n);
+    else if (unroll == 8 && !nt) hipLaunchKernelGGL((k_stream_copy_x<8, false>), g, t, 0, s, a, b, n);
+    else hipLaunchKernelGGL((k_stream_copy_x<8, true>), g, t, 0, s, a, b, n);
+    return hipGetLastError();
+}
+}  // extern "C++"
+
+int tlq_copy_probe_config(tlq_ctx *c, long long bytes, int iters, int unroll, int blocks,
+                          int nontemporal, double *ms) {
     if (int rc = check_ctx(c)) return rc;
     if (bytes < 16 || iters < 1 || !ms) return fail(TLQ_ERR_ARG, "bad probe arguments");
+    if (unroll != 1 && unroll != 2 && unroll != 4 && unroll != 8)
+        return fail(TLQ_ERR_ARG, "probe unroll must be 1, 2, 4 or 8");
+    if (blocks < 1 || blocks > (1 << 22)) return fail(TLQ_ERR_ARG, "probe blocks out of range");
     const long long n = bytes / 16;
     float4 *a = nullptr, *b = nullptr;
     if (hipMalloc(&a, n * 16) != hipSuccess) return fail(TLQ_ERR_NOMEM, "probe alloc");
@@ -1256,20 +1303,35 @@ int tlq_copy_probe(tlq_ctx *c, long long bytes, int iters, double *ms) {
         (void)hipFree(a);
         return fail(TLQ_ERR_NOMEM, "probe alloc");
     }
-    HIPCHK(hipMemsetAsync(a, 0, n * 16, c->stream));
-    const unsigned nb = 256 * 8 * 4;
-    hipLaunchKernelGGL(k_stream_copy, dim3(nb), dim3(256), 0, c->stream, a, b, n);  // warm-up
-    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    auto release = [&]() {
+        (void)hipFree(a);
+        (void)hipFree(b);
+    };
+    const bool nt = nontemporal != 0;
+    if (hipMemsetAsync(a, 0, n * 16, c->stream) != hipSuccess ||
+        launch_copy(c->stream, unroll, nt, (unsigned)blocks, a, b, n) != hipSuccess ||  // warm-up
+        hipEventRecord(c->ev0, c->stream) != hipSuccess) {
+        release();
+        return fail(TLQ_ERR_HIP, "probe launch");
+    }
     for (int q = 0; q < iters; ++q)
-        hipLaunchKernelGGL(k_stream_copy, dim3(nb), dim3(256), 0, c->stream, a, b, n);
-    HIPCHK(hipEventRecord(c->ev1, c->stream));
-    HIPCHK(hipEventSynchronize(c->ev1));
+        if (launch_copy(c->stream, unroll, nt, (unsigned)blocks, a, b, n) != hipSuccess) {
+            release();
+            return fail(TLQ_ERR_HIP, "probe launch");
+        }
     float f = 0.f;
-    HIPCHK(hipEventElapsedTime(&f, c->ev0, c->ev1));
+    const bool ok = hipEventRecord(c->ev1, c->stream) == hipSuccess &&
+                    hipEventSynchronize(c->ev1) == hipSuccess &&
+                    hipEventElapsedTime(&f, c->ev0, c->ev1) == hipSuccess;
+    release();
+    if (!ok) return fail(TLQ_ERR_HIP, "probe timing");
     *ms = f / iters;
-    HIPCHK(hipFree(a));
-    HIPCHK(hipFree(b));
     return TLQ_OK;
+}
+
+int tlq_copy_probe(tlq_ctx *c, long long bytes, int iters, double *ms) {
+    // the plain one-load-per-iteration copy over 8192 blocks of 256 threads
+    return tlq_copy_probe_config(c, bytes, iters, 1, 256 * 8 * 4, 0, ms);
 }
 
 int tlq_layout(tlq_ctx *c, long long *cells, int *pitch_x, int *origin_x) {

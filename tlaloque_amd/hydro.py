@@ -45,7 +45,7 @@ EXPORTS = (
     "tlq_advance", "tlq_advance_n", "tlq_get_time", "tlq_sync", "tlq_check_state",
     "tlq_fallback_fraction", "tlq_cooling_counts", "tlq_resume", "tlq_upload_phi",
     "tlq_download_phi", "tlq_gravity_solve", "tlq_gravity_info", "tlq_timer_start", "tlq_timer_stop", "tlq_kernel_timing",
-    "tlq_kernel_stats", "tlq_layout", "tlq_copy_probe", "tlq_surrogate_init",
+    "tlq_kernel_stats", "tlq_layout", "tlq_copy_probe", "tlq_copy_probe_config", "tlq_surrogate_init",
     "tlq_surrogate_set_params", "tlq_surrogate_get_params", "tlq_surrogate_generate",
     "tlq_surrogate_samples", "tlq_surrogate_train", "tlq_surrogate_eval", "tlq_surrogate_grad",
     "tlq_surrogate_grad_noalloc", "tlq_surrogate_set_threshold",
@@ -127,6 +127,7 @@ def load_library(path: Optional[str] = None):
         "tlq_layout": [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ip),
                        ctypes.POINTER(ip)],
         "tlq_copy_probe": [vp, ctypes.c_longlong, ip, dp],
+        "tlq_copy_probe_config": [vp, ctypes.c_longlong, ip, ip, ip, ip, dp],
         "tlq_surrogate_init": [vp, ctypes.c_ulonglong],
         "tlq_surrogate_set_params": [vp, fp, ip],
         "tlq_surrogate_get_params": [vp, fp, ip],
@@ -364,6 +365,16 @@ class HydroCore:
         """Mean ms of one device-to-device copy of nbytes (streaming kernel)."""
         ms = ctypes.c_double()
         _check(_lib.tlq_copy_probe(self._ctx, int(nbytes), int(iters), ctypes.byref(ms)))
+        return ms.value
+
+    def copy_probe_config(self, nbytes: int, iters: int, unroll: int, blocks: int,
+                          nontemporal: bool) -> float:
+        """Mean ms of one copy in a given configuration: `unroll` 16-byte loads
+        in flight per thread, `blocks` blocks of 256 threads, nontemporal
+        loads and stores or not."""
+        ms = ctypes.c_double()
+        _check(_lib.tlq_copy_probe_config(self._ctx, int(nbytes), int(iters), int(unroll),
+                                          int(blocks), int(bool(nontemporal)), ctypes.byref(ms)))
         return ms.value
 
     # learned cooling surrogate -------------------------------------------------
