@@ -142,3 +142,38 @@ def test_cluster_1024_surrogate_200_steps(theta):
           f"steps with any {sum(q > 0 for q in queued)}")
     assert min(queued) > 0                   # the network ran at every step
     assert 0.0 <= np.max(fracs) < 0.5
+
+
+def test_clumpy_512_surrogate(theta):
+    """The headline workload (clumpy medium 512^3, periodic, point mass) with
+    surrogate cooling, every cell with dt > 0.01 t_cool on the network, five
+    steps: the network runs, the state stays finite, the total mass is
+    conserved (no source changes it and the box is periodic), and the energy
+    agrees with the converged integrator's run to the network's accuracy."""
+    out = {}
+    for mode in (P.COOL_SUBCYCLED, P.COOL_SURROGATE):
+        p, u0 = problems.clumpy(n=512)
+        p = p.replace(cooling=mode)
+        with HydroCore(p, cooling_table=cooling_table()) as h:
+            if mode == P.COOL_SURROGATE:
+                h.surrogate_params(theta)
+                h.surrogate_threshold(0.01)
+            h.set_state(u0)
+            del u0
+            m0 = _mass(h, p)
+            h.advance(5)
+            nq, nf = h.cooling_counts()
+            assert h.check_state() == 0
+            m1 = _mass(h, p)
+            out[mode] = h.download("U")[2:-2, 2:-2, 2:-2, 4].astype(np.float64)
+        print(f"clumpy 512^3 mode {mode}, 5 steps: last step queued {nq} "
+              f"({nq / p.zones:.4f} of the cells), fallbacks {nf} "
+              f"(fraction {nf / max(nq, 1):.4f}); mass change {abs(m1 - m0) / m0:.2e}")
+        assert nq > 0
+        assert abs(m1 - m0) <= 1e-6 * m0
+        if mode == P.COOL_SURROGATE:
+            assert nf < 0.5 * nq
+    a, b = out[P.COOL_SURROGATE], out[P.COOL_SUBCYCLED]
+    e = float(np.abs(a - b).sum() / np.abs(b).sum())
+    print(f"clumpy 512^3: surrogate vs converged integrator L1 energy {e:.3e}")
+    assert e <= 1e-2
