@@ -161,6 +161,49 @@ def cooling_modes(p, u0, table, warmup, steps):
     return out, theta
 
 
+def cooling_case(p, u0, table, spec, warmup, steps, theta_file=None):
+    """One cooling path on one variant of the workload, step by step, with the
+    cells queued for the dense pass (network or integrator) and the fallbacks
+    of every step: the per-cell denominators of a counter profile of the
+    cooling-pass kernels.  spec = 'VARIANT:PATH', VARIANT one of T=1e4K,
+    T=3e5K, T=3e5K,n*100, PATH one of surrogate, surrogate_all_cells,
+    converged_integrator."""
+    from tlaloque_amd import params as P
+    from tlaloque_amd import surrogate
+    from tlaloque_amd.hydro import HydroCore
+    label, name = spec.split(":")
+    u = {"T=1e4K": lambda: u0, "T=3e5K": lambda: heated(p, u0, 3e5),
+         "T=3e5K,n*100": lambda: heated(p, u0, 3e5, 100.0)}[label]()
+    mode, thr = {"surrogate": (P.COOL_SURROGATE, None),
+                 "surrogate_all_cells": (P.COOL_SURROGATE, 0.01),
+                 "converged_integrator": (P.COOL_SUBCYCLED, None)}[name]
+    theta = None
+    if mode == P.COOL_SURROGATE:
+        if theta_file and os.path.exists(theta_file):
+            theta = np.load(theta_file)
+        else:
+            theta = surrogate.train()[0]
+            if theta_file:
+                np.save(theta_file, theta)
+    q = p.replace(cooling=mode)
+    with HydroCore(q, cooling_table=table) as h:
+        if theta is not None:
+            h.surrogate_params(theta)
+        if thr is not None:
+            h.surrogate_threshold(thr)
+        h.set_state(u)
+        del u
+        per_step = []
+        for _ in range(warmup + steps):
+            h.advance(1)
+            per_step.append(h.cooling_counts())
+        bad = h.check_state()
+    log(f"bench: cooling case {spec}: queued/fallbacks per step {per_step}")
+    return {"cooling_case": spec, "zones": q.zones, "steps": warmup + steps,
+            "queued_per_step": [c[0] for c in per_step],
+            "fallbacks_per_step": [c[1] for c in per_step], "nonfinite_cells": bad}
+
+
 def self_gravity_leg(p, u0, table, warmup, steps):
     """The workload with self-gravity (TLQ_SG_MULTIPOLE): zone-updates/s of
     whole steps, multigrid ms and V-cycles per step."""
@@ -342,6 +385,12 @@ def main():
     ap.add_argument("--no-cluster", action="store_true", help="skip the 1024^3 cluster leg")
     ap.add_argument("--gravity-only", action="store_true",
                     help="run only the self-gravity leg and print its JSON (for profiles)")
+    ap.add_argument("--cooling-case", default=None,
+                    help="run one cooling path on one variant of the workload and print the "
+                         "cells queued per step (for profiles), e.g. 'T=3e5K,n*100:surrogate_all_cells'")
+    ap.add_argument("--theta", default=None,
+                    help="with --cooling-case: network parameters (.npy) to load, or to write "
+                         "after training if the file does not exist")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -357,6 +406,10 @@ def main():
     p, u0 = problems.clumpy(n=n, precision=P.PREC_FP32)
     log(f"bench: clumpy {n}^3 initial state built in {time.perf_counter() - t0:.1f}s")
     table = P.load_cooling_table(os.path.join(ROOT, "tlaloque_amd", "data", "Z1.0.dat"))
+    if args.cooling_case:
+        print(json.dumps(cooling_case(p, u0, table, args.cooling_case, args.warmup, args.steps,
+                                      args.theta)))
+        return
     if args.gravity_only:
         print(json.dumps({"self_gravity": self_gravity_leg(p, u0, table, args.warmup, args.steps),
                           "config": {"workload": f"clumpy self-gravitating medium {n}^3 with "

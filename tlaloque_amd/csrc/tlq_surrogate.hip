@@ -541,6 +541,8 @@ __global__ __launch_bounds__(128) void k_cool_dense(const Problem *__restrict__ 
     unsigned nfb = 0;  // this wavefront's buffered fallbacks
     bf16x4 w2a[4][4];
     float b3 = 0.f;
+    // T_min / Lambda(T_min): k_B T_min / ((gamma-1) n Lambda) is the cooling time at the gate
+    const float gate_tc = MODE == 2 ? P.f_tmin / lambda_f(P, P.f_tmin) : 0.f;
     if (MODE == 2 && nq > 0) {
         load_small_f(sm, thq, threadIdx.x, blockDim.x);
         b3 = bf2f(thq[OB3]);
@@ -568,41 +570,51 @@ __global__ __launch_bounds__(128) void k_cool_dense(const Problem *__restrict__ 
         bool need_exact = valid;
         double Tn = s.T;
         if constexpr (MODE == 2) {
-            const float lam = lambda_f(P, (float)s.T);
-            const float ce = lam * (float)s.nd * (float)s.nd * (float)(P.gamma - 1.0) *
-                             __builtin_amdgcn_rcpf((float)s.p);
-            float x[3] = {lg10((float)s.nd), lg10((float)s.T), lg10(ce * (float)dt)};
-            float xn[3];
-            normalise(x, xn);
-            const float o = mlp_wave(sm, w2a, b3, lane, xn);
-            const bool inbox = x[0] >= X0LO && x[0] <= X0HI && x[1] >= X1LO && x[1] <= X1HI &&
-                               x[2] <= X2HI;
-            const float d = __builtin_amdgcn_exp2f(x[2] * 3.32192809488736f);
-            const float rh = -0.4342944819032518f * d / (1.f + d) * o;  // log10(T1/T0)
-            const float rmin = lg10(10.f * __builtin_amdgcn_rcpf((float)s.T));
-            bool ok = valid && inbox && rh <= 1e-3f && rh >= rmin - 1e-3f;
-            bool snap = false;
             const float T0 = (float)s.T;
-            const float T1 = T0 * __builtin_amdgcn_exp2f(fminf(rh, 0.f) * 3.32192809488736f);
-            if (ok) {
-                const float kn = (float)(KB / (P.gamma - 1.0)) / (float)s.nd;
-                const float fdt = (float)dt;
-                // a time error dt' shifts ln T1 by dt'/t_cool(T1): allow 5% of
-                // the change in ln T, at most 0.05
-                const float Te = fmaxf(T1, P.f_tmin);
-                const float tol = 0.05f * fminf(fdt, kn * Te / lambda_f(P, Te));
-                if (T1 <= 1.1f * P.f_tmin) {
-                    // the network says the gas reaches the gate: accepted when
-                    // the exact cooling time to T_min fits in dt, and the answer
-                    // is the gate itself (where the integrator stops)
-                    ok = kn * cool_dY(P, T0, P.f_tmin) <= fdt + tol;
-                    snap = true;
-                } else {
-                    ok = fabsf(kn * cool_dY(P, T0, T1) - fdt) <= tol;
+            const float kn = (float)(KB / (P.gamma - 1.0)) / (float)s.nd;
+            const float fdt = (float)dt;
+            // A cell whose exact cooling time to the gate (closed form) fits in
+            // dt ends at the gate, where the integrator stops: the answer the
+            // check below accepts when the network predicts the gate, given
+            // without the network.
+            const bool gate =
+                valid && kn * cool_dY(P, T0, P.f_tmin) <= fdt + 0.05f * fminf(fdt, kn * gate_tc);
+            const bool net = valid && !gate;
+            bool ok = false, snap = false;
+            float T1 = T0;
+            if (__ballot(net)) {  // wavefront-uniform: the network runs unless every cell is gated
+                const float lam = lambda_f(P, T0);
+                const float ce = lam * (float)s.nd * (float)s.nd * (float)(P.gamma - 1.0) *
+                                 __builtin_amdgcn_rcpf((float)s.p);
+                float x[3] = {lg10((float)s.nd), lg10(T0), lg10(ce * fdt)};
+                float xn[3];
+                normalise(x, xn);
+                const float o = mlp_wave(sm, w2a, b3, lane, xn);
+                const bool inbox = x[0] >= X0LO && x[0] <= X0HI && x[1] >= X1LO &&
+                                   x[1] <= X1HI && x[2] <= X2HI;
+                const float d = __builtin_amdgcn_exp2f(x[2] * 3.32192809488736f);
+                const float rh = -0.4342944819032518f * d / (1.f + d) * o;  // log10(T1/T0)
+                const float rmin = lg10(10.f * __builtin_amdgcn_rcpf(T0));
+                ok = net && inbox && rh <= 1e-3f && rh >= rmin - 1e-3f;
+                T1 = T0 * __builtin_amdgcn_exp2f(fminf(rh, 0.f) * 3.32192809488736f);
+                if (ok) {
+                    // a time error dt' shifts ln T1 by dt'/t_cool(T1): allow 5% of
+                    // the change in ln T, at most 0.05
+                    const float Te = fmaxf(T1, P.f_tmin);
+                    const float tol = 0.05f * fminf(fdt, kn * Te / lambda_f(P, Te));
+                    if (T1 <= 1.1f * P.f_tmin) {
+                        // the network says the gas reaches the gate: accepted when
+                        // the exact cooling time to T_min fits in dt, and the answer
+                        // is the gate itself (where the integrator stops)
+                        ok = kn * cool_dY(P, T0, P.f_tmin) <= fdt + tol;
+                        snap = true;
+                    } else {
+                        ok = fabsf(kn * cool_dY(P, T0, T1) - fdt) <= tol;
+                    }
                 }
             }
-            if (ok) {
-                Tn = snap ? (double)SNAP * P.tmin : (double)T1;
+            if (gate || ok) {
+                Tn = gate || snap ? (double)SNAP * P.tmin : (double)T1;
                 need_exact = false;
             }
             // fallbacks go to the integrator's pass: a lane that needs the
