@@ -129,7 +129,8 @@ def cooling_modes(p, u0, table, warmup, steps):
         for name, mode, thr in (("reference_one_step", P.COOL_EXACT, None),
                                 ("surrogate", P.COOL_SURROGATE, None),
                                 ("surrogate_all_cells", P.COOL_SURROGATE, 0.01),
-                                ("converged_integrator", P.COOL_SUBCYCLED, None)):
+                                ("converged_integrator", P.COOL_SUBCYCLED, None),
+                                ("exact_integration", P.COOL_INTEGRATED, None)):
             if label == "T=1e4K" and mode == P.COOL_EXACT:
                 continue  # the headline run
             zps, ms, kms, (nq, nf), bad = timed_run(p.replace(cooling=mode), u, table, warmup,
@@ -156,6 +157,10 @@ def cooling_modes(p, u0, table, warmup, steps):
             cs, ci = res["surrogate"]["cooling_pass_ms"], res["converged_integrator"]["cooling_pass_ms"]
             if cs and ci:
                 res["cooling_pass_speedup"] = ci / cs
+        if "exact_integration" in res and "converged_integrator" in res:
+            res["exact_integration_speedup_vs_converged"] = (
+                res["exact_integration"]["zone_updates_per_s"] /
+                res["converged_integrator"]["zone_updates_per_s"])
         out[label] = res
         del u
     return out, theta
@@ -176,7 +181,8 @@ def cooling_case(p, u0, table, spec, warmup, steps, theta_file=None):
          "T=3e5K,n*100": lambda: heated(p, u0, 3e5, 100.0)}[label]()
     mode, thr = {"surrogate": (P.COOL_SURROGATE, None),
                  "surrogate_all_cells": (P.COOL_SURROGATE, 0.01),
-                 "converged_integrator": (P.COOL_SUBCYCLED, None)}[name]
+                 "converged_integrator": (P.COOL_SUBCYCLED, None),
+                 "exact_integration": (P.COOL_INTEGRATED, None)}[name]
     theta = None
     if mode == P.COOL_SURROGATE:
         if theta_file and os.path.exists(theta_file):

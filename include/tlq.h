@@ -56,7 +56,7 @@ extern "C" {
 #define TLQ_FP32 1
 
 /* Cooling: off, exact (cooling.f90:47-119), or the learned surrogate with a
- * per-cell fallback to the exact integrator. */
+ * per-cell fallback to the exact integration (TLQ_COOL_INTEGRATED's). */
 #define TLQ_COOL_OFF 0
 #define TLQ_COOL_EXACT 1
 #define TLQ_COOL_SURROGATE 2
@@ -64,6 +64,14 @@ extern "C" {
  * 0.01 t_cool with Lambda re-read every substep (the surrogate's target and
  * fallback; one substep is exactly cooling.f90:67-112). */
 #define TLQ_COOL_SUBCYCLED 3
+/* The cooling equation integrated exactly for the tabulated Lambda (a power
+ * law on each table segment; Townsend 2009, ApJS 181, 391): the limit of
+ * TLQ_COOL_SUBCYCLED as its substeps shrink, in the same dense pass.  Cells
+ * with dt <= 0.01 t_cool take cooling.f90:67-112's one step, as in
+ * TLQ_COOL_SUBCYCLED.  The surrogate's integrator cells and fallbacks use the
+ * same integration.  (An addition: the reference has only the one-step
+ * formula.) */
+#define TLQ_COOL_INTEGRATED 4
 
 /* Self-gravity (the -DGRAV build, sources.f90:99-897): off; on with the
  * potential's ghost faces kept as uploaded (the serial build, whose neighbour

@@ -114,6 +114,9 @@ def lib():
                                          ctypes.c_double, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_int)]
         L.orc_cool_subcycled.restype = ctypes.c_double
+        L.orc_cool_exact_int.argtypes = [pp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double]
+        L.orc_cool_exact_int.restype = ctypes.c_double
         L.orc_mlp_nparams.restype = ctypes.c_int
         L.orc_mlp.argtypes = [ctypes.POINTER(ctypes.c_float), dp, dp, ctypes.c_int, dp, dp]
         L.orc_mlp.restype = ctypes.c_double
@@ -302,6 +305,11 @@ class Oracle:
         T1 = lib().orc_cool_subcycled(self.P, float(n), float(T), float(dt), float(eps),
                                       int(max_sub), ctypes.byref(ns))
         return T1, ns.value
+
+    def cool_exact_int(self, n: float, T: float, dt: float, gate_frac: float = 0.995) -> float:
+        """Cooling integrated exactly over dt (closed form on the power-law
+        table segments); gate_frac * T_min when the gate is reached."""
+        return lib().orc_cool_exact_int(self.P, float(n), float(T), float(dt), float(gate_frac))
 
     def hllc_face(self, wl, wr, ul, ur, dim, solver=None):
         f = np.full(5, np.nan)

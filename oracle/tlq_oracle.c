@@ -962,6 +962,64 @@ double orc_cool_subcycled(const orc_params *p, double n, double T, double dt, do
 }
 
 /* ------------------------------------------------------------------------
+ * Exact integration of the cooling equation dT/dt = -(gamma-1) n Lambda(T)/k_B
+ * (the per-cell loss of apply_cooling, cooling.f90:67-112, as a rate) over dt,
+ * for Lambda as find_coef interpolates it: a power law on each table segment
+ * and the last row's value above the table.  Then the time to cool from T0 to
+ * T1 is k_B/((gamma-1) n) (Y(T0) - Y(T1)) with Y(T) = int_{T_min}^{T} dT'/Lambda
+ * in closed form, and T(dt) follows by inverting Y on the segment where
+ * Y(T0) - dt (gamma-1) n / k_B falls (Townsend 2009, ApJS 181, 391, "exact
+ * integration").  It is the limit of orc_cool_subcycled as eps -> 0.  When
+ * the cooling time to the gate T_min is at most dt the gas stops at the gate,
+ * as the reference's loop does (cooling.f90:71): the result is then
+ * gate_frac T_min (the integrator's last substep ends within 1% below it).
+ * Returns T0 unchanged below T_min. */
+static double orc_seg_slope(const orc_params *p, int i) {
+    return (p->table_L[i] - p->table_L[i - 1]) /
+           (log10(p->table_T[i]) - log10(p->table_T[i - 1]));
+}
+
+/* Y inside segment i (from T_{i-1} to T <= T_i) */
+static double orc_seg_Y(const orc_params *p, int i, double T) {
+    const double a = 1.0 - orc_seg_slope(p, i);
+    const double coef = p->table_T[i - 1] / pow(10.0, p->table_L[i - 1]);
+    const double r = T / p->table_T[i - 1];
+    return fabs(a) > 1e-12 ? coef * (pow(r, a) - 1.0) / a : coef * log(r);
+}
+
+double orc_cool_exact_int(const orc_params *p, double n, double T0, double dt, double gate_frac) {
+    const int nt = p->n_table;
+    const double Tmin = p->table_T[0], Tmax = p->table_T[nt - 1];
+    if (!(T0 >= Tmin)) return T0;
+    /* Y at the nodes */
+    double ycum[4096];
+    if (nt > 4096) return NAN;
+    ycum[0] = 0.0;
+    for (int i = 1; i < nt; ++i) ycum[i] = ycum[i - 1] + orc_seg_Y(p, i, p->table_T[i]);
+    const double lam_top = pow(10.0, p->table_L[nt - 1]);
+    /* Y(T0): the segment as find_coef picks it (first row with T_i > T0) */
+    double y0;
+    if (T0 >= Tmax) {
+        y0 = ycum[nt - 1] + (T0 - Tmax) / lam_top;
+    } else {
+        int i = 1;
+        while (!(p->table_T[i] > T0)) ++i;
+        y0 = ycum[i - 1] + orc_seg_Y(p, i, T0);
+    }
+    const double y1 = y0 - dt * (p->gamma - 1.0) * n / ORC_KB;
+    if (!(y1 > 0.0)) return gate_frac * Tmin;
+    if (y1 >= ycum[nt - 1]) return Tmax + (y1 - ycum[nt - 1]) * lam_top;
+    int i = 1;
+    while (!(ycum[i] > y1)) ++i;
+    /* invert Y on segment i: (T/T_{i-1})^a = 1 + a (y1 - Y_{i-1}) / coef */
+    const double a = 1.0 - orc_seg_slope(p, i);
+    const double coef = p->table_T[i - 1] / pow(10.0, p->table_L[i - 1]);
+    const double s = (y1 - ycum[i - 1]) / coef;
+    const double r = fabs(a) > 1e-12 ? pow(1.0 + a * s, 1.0 / a) : exp(s);
+    return p->table_T[i - 1] * r;
+}
+
+/* ------------------------------------------------------------------------
  * The cooling surrogate: an MLP 3 -> 64 -> 64 -> 1 with GELU (tanh form),
  * inputs (log10 n, log10 T, log10(dt/t_cool)) normalised to [-1, 1] over the
  * training box, output r = log10(T_new / T).  Parameters, flat float array:

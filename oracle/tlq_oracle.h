@@ -105,6 +105,9 @@ double orc_advance_sg(const orc_params *p, double *u, double *up, double *upp, d
 /* Cooling integrated to convergence (the reference formula sub-cycled). */
 double orc_cool_subcycled(const orc_params *p, double n, double T, double dt, double eps,
                           int max_sub, int *nsub);
+/* Cooling integrated exactly (closed form for the power-law table segments);
+ * gate_frac T_min when the gas reaches the gate within dt. */
+double orc_cool_exact_int(const orc_params *p, double n, double T, double dt, double gate_frac);
 /* Cooling surrogate MLP: parameter count, forward, loss and gradient. */
 int orc_mlp_nparams(void);
 double orc_mlp(const float *theta, const double *x, const double *y, int B, double *out,

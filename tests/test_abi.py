@@ -55,7 +55,7 @@ _BAD = [
     ("limiter", dict(limiter=7), -1, "limiter"),
     ("solver", dict(solver=2), -1, "solver"),
     ("precision", dict(precision=2), -1, "precision"),
-    ("cooling", dict(cooling=4), -1, "cooling"),
+    ("cooling", dict(cooling=5), -1, "cooling"),
     ("gamma", dict(gamma=1.0), -1, "physical"),
     ("domain", dict(xr=-2.0), -1, "physical"),
     ("self_gravity id", dict(self_gravity=3), -1, "self_gravity"),

@@ -508,7 +508,7 @@ int tlq_create(const tlq_params *p, tlq_ctx **out) {
         return fail(TLQ_ERR_ARG, "bad solver id");
     if (p->precision != TLQ_FP64 && p->precision != TLQ_FP32)
         return fail(TLQ_ERR_ARG, "bad precision id");
-    if (p->cooling < TLQ_COOL_OFF || p->cooling > TLQ_COOL_SUBCYCLED)
+    if (p->cooling < TLQ_COOL_OFF || p->cooling > TLQ_COOL_INTEGRATED)
         return fail(TLQ_ERR_ARG, "bad cooling id");
     if (!(p->gamma > 1.0) || !(p->cfl > 0.0) || !(p->xr > p->xl) || !(p->yr > p->yl) ||
         !(p->zr > p->zl) || !(p->mu0 > 0.0))

@@ -106,12 +106,94 @@ static __device__ float cool_sub_f(const Problem &P, float n, float T, float dt,
 }
 
 // ---------------------------------------------------------------------------
+// Exact integration of the cooling equation dT/dt = -(gamma-1) n Lambda(T)/k_B
+// over dt for Lambda as find_coef interpolates it (a power law on each table
+// segment, the last row's value above the table): the time to cool from T0 to
+// T1 is k_B/((gamma-1) n) (Y(T0) - Y(T1)) with Y closed form (cool_Y), so T(dt)
+// follows by inverting Y on the segment where Y(T0) - dt (gamma-1) n/k_B falls
+// (Townsend 2009, ApJS 181, 391).  It is the eps -> 0 limit of the substepped
+// integrator.  At the gate (the cooling time to T_min at most dt) the result is
+// SNAP T_min, where the substepped integrator's last substep ends.
+constexpr double SNAP = 0.995;
+
+// FP64: the node values of Y from the host (tab_Ycum), the rest with the
+// device's pow/log/exp.
+static __device__ double cool_int_d(const Problem &P, double n, double T0, double dt) {
+    if (!(T0 >= P.tmin)) return T0;
+    const int last = P.ntable - 1;
+    const double lam_top = pow(10.0, P.tab_L[last]);
+    auto seg = [&](int i, double &a, double &coef) {
+        a = 1.0 - (P.tab_L[i] - P.tab_L[i - 1]) / (P.tab_logT[i] - P.tab_logT[i - 1]);
+        coef = P.tab_T[i - 1] / pow(10.0, P.tab_L[i - 1]);
+    };
+    double y0;
+    if (T0 >= P.tab_T[last]) {
+        y0 = P.tab_Ycum[last] + (T0 - P.tab_T[last]) / lam_top;
+    } else {  // first row with T_i > T0 (find_coef's segment)
+        int lo = 0, hi = last;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (P.tab_T[mid] > T0) hi = mid;
+            else lo = mid;
+        }
+        double a, coef;
+        seg(hi, a, coef);
+        const double r = T0 / P.tab_T[hi - 1];
+        y0 = P.tab_Ycum[hi - 1] + (fabs(a) > 1e-12 ? coef * (pow(r, a) - 1.0) / a : coef * log(r));
+    }
+    const double y1 = y0 - dt * (P.gamma - 1.0) * n / KB;
+    if (!(y1 > 0.0)) return SNAP * P.tmin;
+    if (y1 >= P.tab_Ycum[last]) return P.tab_T[last] + (y1 - P.tab_Ycum[last]) * lam_top;
+    int lo = 0, hi = last;  // Ycum[lo] <= y1 < Ycum[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (P.tab_Ycum[mid] > y1) hi = mid;
+        else lo = mid;
+    }
+    double a, coef;
+    seg(hi, a, coef);
+    const double s = (y1 - P.tab_Ycum[hi - 1]) / coef;
+    return P.tab_T[hi - 1] * (fabs(a) > 1e-12 ? pow(1.0 + a * s, 1.0 / a) : exp(s));
+}
+
+// FP32: Y's node values in FP64, the segment parts in FP32 (cool_Y).
+static __device__ float cool_int_f(const Problem &P, float n, float T0, float dt) {
+    if (!(T0 >= P.f_tmin)) return T0;
+    const int last = P.ntable - 1;
+    const float lam_top = __builtin_amdgcn_exp2f(P.tab_Lf[last] * 3.32192809488736f);
+    double y0;
+    if (!(T0 < P.f_tmax)) {
+        y0 = P.tab_Ycum[last] + (double)((T0 - P.f_tmax) / lam_top);
+    } else {
+        const YSplit q = cool_Y(P, T0);
+        y0 = q.node + (double)q.part;
+    }
+    const double y1 = y0 - (double)dt * (double)((float)((P.gamma - 1.0) / KB) * n);
+    if (!(y1 > 0.0)) return (float)(SNAP * P.tmin);
+    if (!(y1 < P.tab_Ycum[last])) return P.f_tmax + (float)(y1 - P.tab_Ycum[last]) * lam_top;
+    int lo = 0, hi = last;  // Ycum[lo] <= y1 < Ycum[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (P.tab_Ycum[mid] > y1) hi = mid;
+        else lo = mid;
+    }
+    const float a = 1.f - P.tab_slopef[hi];
+    const float s = (float)(y1 - P.tab_Ycum[hi - 1]) / P.tab_ycoef[hi];
+    const float r = fabsf(a) > 1e-4f
+                        ? __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(fmaxf(1.f + a * s, 1e-30f)) / a)
+                        : __builtin_amdgcn_exp2f(s * 1.4426950408889634f);
+    return P.tab_Tf[hi - 1] * r;
+}
+
+template <class R> __device__ __forceinline__ double cool_int(const Problem &P, double n, double T, double dt) {
+    if constexpr (sizeof(R) == 8) return cool_int_d(P, n, T, dt);
+    else return (double)cool_int_f(P, (float)n, (float)T, (float)dt);
+}
+
+// ---------------------------------------------------------------------------
 // Per-cell state and the cooling pass's bookkeeping.
 constexpr float COOL_EPS = 0.01f;
 constexpr int COOL_MAXSUB = 100000;
-// A surrogate answer at the gate is set just below it, in the 1% band where
-// the converged integrator's last substep ends (so the cell stays gated).
-constexpr double SNAP = 0.995;
 
 struct CellState {
     double rho, ek, p, nd, T;

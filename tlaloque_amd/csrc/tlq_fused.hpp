@@ -578,7 +578,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
     const int lim = P.limiter;
     const bool grav = P.nmasses > 0 || P.phi != nullptr;
     // Cooling mode 1: the reference's one-step formula on every cell.  Modes
-    // 2 and 3: cells with dt <= 0.01 t_cool take the one substep the
+    // 2, 3 and 4: cells with dt <= 0.01 t_cool take the one substep the
     // converged integrator would take; the others are queued for the dense
     // pass after this kernel, which also forms their CFL maxima.
     const bool cool = P.cooling == 1 && P.ntable > 1;

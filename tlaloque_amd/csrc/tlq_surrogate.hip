@@ -630,8 +630,13 @@ __global__ __launch_bounds__(128) void k_cool_dense(const Problem *__restrict__ 
             }
             if (valid && !need_exact) finish_cell(P, U, g.nc, c, u, s, Tn, mx, my, mz);
         } else {
-            if (valid) finish_cell(P, U, g.nc, c, u, s, integrate<R>(P, s, dt, COOL_MAXSUB), mx,
-                                   my, mz);
+            // mode 3: the substepped integrator; mode 4 and the surrogate's
+            // integrator cells and fallbacks: the exact integration
+            if (valid)
+                finish_cell(P, U, g.nc, c, u, s,
+                            P.cooling == 3 ? integrate<R>(P, s, dt, COOL_MAXSUB)
+                                           : cool_int<R>(P, s.nd, s.T, dt),
+                            mx, my, mz);
         }
     }
     if (MODE == 2 && nfb) flush_fallbacks(s_fbuf[wv], nfb, lane, fqueue, fcount);

@@ -29,7 +29,7 @@ module tlq_c
   integer(c_int), parameter :: TLQ_SOLVER_HLLC = 0, TLQ_SOLVER_HLL = 1
   integer(c_int), parameter :: TLQ_FP64 = 0, TLQ_FP32 = 1
   integer(c_int), parameter :: TLQ_COOL_OFF = 0, TLQ_COOL_EXACT = 1, TLQ_COOL_SURROGATE = 2
-  integer(c_int), parameter :: TLQ_COOL_SUBCYCLED = 3
+  integer(c_int), parameter :: TLQ_COOL_SUBCYCLED = 3, TLQ_COOL_INTEGRATED = 4
   integer(c_int), parameter :: TLQ_SG_OFF = 0, TLQ_SG_FIXED = 1, TLQ_SG_MULTIPOLE = 2
 
   integer(c_int), parameter :: TLQ_BUF_U = 0, TLQ_BUF_UP = 1, TLQ_BUF_UPP = 2, TLQ_BUF_UPRIM = 3

@@ -32,7 +32,7 @@ LIMITER_NONE, LIMITER_VANLEER, LIMITER_MINMOD, LIMITER_UMIST, LIMITER_WOODWARD, 
 
 SOLVER_HLLC, SOLVER_HLL = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1          # FP32: FP32 storage and sweeps, FP64 guards
-COOL_OFF, COOL_EXACT, COOL_SURROGATE, COOL_SUBCYCLED = 0, 1, 2, 3
+COOL_OFF, COOL_EXACT, COOL_SURROGATE, COOL_SUBCYCLED, COOL_INTEGRATED = 0, 1, 2, 3, 4
 # Self-gravity (source/sources.f90:99-205): off; phi's ghost faces kept as set
 # (the serial build); multipole values on them every 10 steps (the MPI build
 # at the domain edge).
