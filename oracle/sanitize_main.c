@@ -210,7 +210,8 @@ int main(int argc, char **argv) {
     free(phi); free(dens); free(work);
     free_state(&t);
 
-    /* 4. Cooling to convergence and the surrogate network (forward, loss, gradient). */
+    /* 4. Cooling to convergence, its exact integration and the surrogate network
+     * (forward, loss, gradient). */
     p.n_table = nt; p.table_T = tT; p.table_L = tL; p.cooling = 1;
     for (int q = 0; q < 20; ++q) {
         int ns = 0;
@@ -218,6 +219,10 @@ int main(int argc, char **argv) {
                                        pow(10.0, 4.0 + 4.5 * urand()), 1e12 * urand(), 0.01,
                                        100000, &ns);
         check_finite("subcycled", &T1, 1);
+        /* exact integration: below, inside and above the table, to the gate */
+        double T2 = orc_cool_exact_int(&p, pow(10.0, 4.0 * urand() - 2.0),
+                                       pow(10.0, 3.5 + 5.5 * urand()), 1e14 * urand(), 0.995);
+        check_finite("exact_int", &T2, 1);
     }
     {
         double lam = orc_find_coef(&p, 3e5);
