@@ -179,19 +179,15 @@ template <class R> __device__ __forceinline__ R lim_avg(int opt, R a, R b) {
             if (a * b <= 0.0) return 0.0;
             return a * b * (a + b) / (a * a + b * b);
         } else {
-            // a*b (a+b)/(a^2+b^2) evaluated on a 2^-x, b 2^-x with 2^x the
-            // binade of m = max(|a|,|b|) (exact scaling, |a|, |b| 2^-x < 1):
-            // no a*b product to underflow for cgs density differences (~1e-25)
-            // and no overflow.
-            // The slope is 0 where a b <= 0: max(a b, 0) gives that without
-            // a select, and the 1e-30 keeps the denominator (>= 1/4 when
-            // m > 0) nonzero when a = b = 0.
-            const R m = fmaxf(__builtin_fabsf(a), __builtin_fabsf(b));
-            const int x = __builtin_amdgcn_frexp_expf(m);
-            const R as = __builtin_ldexpf(a, -x), bs = __builtin_ldexpf(b, -x);
-            const R ab = fmaxf(as * bs, 0.f);
-            return ab * (a + b) *
-                   __builtin_amdgcn_rcpf(__builtin_fmaf(as, as, __builtin_fmaf(bs, bs, 1e-30f)));
+            // a b (a+b)/(a^2+b^2) = (a+b) r/(1+r^2) = (a+b) / (r + 1/r) with
+            // r = a/b: a ratio, so no product underflows for cgs density
+            // differences (~1e-25) and none overflows.  r = max(a/b, 0)
+            // gives the zero slope where a b <= 0 without a select (a NaN
+            // ratio, a = b = 0, goes to 0 as well), and r = 0 or r = inf
+            // (b = 0, or a quotient past the FP32 range) gives 1/(r + 1/r)
+            // = 0.  Three hardware reciprocals, no division.
+            const R r = fmaxf(a * __builtin_amdgcn_rcpf(b), 0.f);
+            return (a + b) * __builtin_amdgcn_rcpf(r + __builtin_amdgcn_rcpf(r));
         }
     case LIM_MINMOD:
         s = __builtin_copysign(R(1), a);
@@ -527,48 +523,72 @@ __device__ __forceinline__ void stage_table(const Problem &P, TabRow *tab, int t
         tab[q] = TabRow{P.tab_Tf[q], P.tab_lgTf[q], P.tab_Lf[q], P.tab_slopef[q]};
 }
 
+// The FP32 cooling constants of a Problem, read once by a kernel.  The fused
+// kernels' block barriers clobber memory, so a Problem field read inside
+// their plane loop is reloaded (a scalar load and its wait) on every plane;
+// held in a local, it is read once.
+struct CoolF32 {
+    float cT, inv_mu_amu, inv_gm1, tmin, tmax, floor_f, lgT0, inv_dlg;
+    int ntable, uniform;
+};
+
+__device__ __forceinline__ CoolF32 cool_consts(const Problem &P) {
+    return CoolF32{P.f_cT,   P.f_inv_mu_amu, P.f_inv_gm1, P.f_tmin,  P.f_tmax,
+                   P.f_floor_f, P.f_lgT0,    P.f_inv_dlg, P.ntable, P.tab_uniform};
+}
+
 // apply_cooling in FP32 with the table rows in LDS (see cool_fast); *ce_dt
 // (when given) receives dt / t_cool.
-__device__ __forceinline__ float cool_fast_tab(const Problem &P, const TabRow *tab, float rho,
+__device__ __forceinline__ float cool_fast_tab(const CoolF32 &C, const TabRow *tab, float rho,
                                                float p, float dt, bool *cooled,
                                                float *ce_dt = nullptr) {
     const float rinv = __builtin_amdgcn_rcpf(rho);
-    const float temp = p * P.f_cT * rinv;
-    if (!(temp >= P.f_tmin)) {
+    const float temp = p * C.cT * rinv;
+    if (!(temp >= C.tmin)) {
         *cooled = false;
         return p;
     }
     *cooled = true;
     const float lg = __builtin_amdgcn_logf(temp) * 0.301029995663981f;  // log10 T
     float lgl;
-    if (!(temp < P.f_tmax)) {
-        lgl = tab[P.ntable - 1].L;
+    if (!(temp < C.tmax)) {
+        lgl = tab[C.ntable - 1].L;
     } else {
-        int hi;
-        if (P.tab_uniform) {
-            hi = (int)((lg - P.f_lgT0) * P.f_inv_dlg) + 1;
-            hi = hi < 1 ? 1 : (hi > P.ntable - 1 ? P.ntable - 1 : hi);
-            if (!(tab[hi].T > temp) && hi < P.ntable - 1) ++hi;
-            if (hi > 1 && tab[hi - 1].T > temp) --hi;
+        if (C.uniform) {
+            // The row index h0 from log10 T is corrected by at most one row
+            // either way (rounding of the index): hi = h0 + 1 when T[h0] <=
+            // temp (and h0 < n-1), else h0 - 1 when T[h0-1] > temp (and h0 >
+            // 1), else h0.  The four rows that (hi-1, hi) can be are read at
+            // once, so the lookup costs one LDS round trip instead of three
+            // dependent ones.
+            int h0 = (int)((lg - C.lgT0) * C.inv_dlg) + 1;
+            h0 = h0 < 1 ? 1 : (h0 > C.ntable - 1 ? C.ntable - 1 : h0);
+            const TabRow r0 = tab[h0 > 1 ? h0 - 2 : 0], r1 = tab[h0 - 1], r2 = tab[h0],
+                         r3 = tab[h0 < C.ntable - 1 ? h0 + 1 : h0];
+            const bool up = !(r2.T > temp) && h0 < C.ntable - 1;
+            const bool down = !up && h0 > 1 && r1.T > temp;
+            const float aL = up ? r2.L : (down ? r0.L : r1.L);
+            const float alg = up ? r2.lgT : (down ? r0.lgT : r1.lgT);
+            const float bs = up ? r3.slope : (down ? r1.slope : r2.slope);
+            lgl = aL + bs * (lg - alg);
         } else {
-            int lo = 0;
-            hi = P.ntable - 1;
+            int lo = 0, hi = C.ntable - 1;
             while (hi - lo > 1) {
                 const int mid = (lo + hi) >> 1;
                 if (tab[mid].T > temp) hi = mid;
                 else lo = mid;
             }
+            const TabRow a = tab[hi - 1];
+            lgl = a.L + tab[hi].slope * (lg - a.lgT);
         }
-        const TabRow a = tab[hi - 1];
-        lgl = a.L + tab[hi].slope * (lg - a.lgT);
     }
     const float lam = __builtin_amdgcn_exp2f(lgl * 3.32192809488736f);  // 10^lgl
-    const float n = rho * P.f_inv_mu_amu;
-    const float eth = p * P.f_inv_gm1;
+    const float n = rho * C.inv_mu_amu;
+    const float eth = p * C.inv_gm1;
     const float ce = lam * n * n * __builtin_amdgcn_rcpf(eth);
     if (ce_dt) *ce_dt = ce * dt;
     float f = __builtin_amdgcn_exp2f(-ce * dt * 1.44269504088896f);  // exp(-ce dt)
-    if (f < P.f_floor_f) f = P.f_floor_f;
+    if (f < C.floor_f) f = C.floor_f;
     if (temp * f < 10.f) f = 10.f * __builtin_amdgcn_rcpf(temp);
     return p * f;
 }

@@ -102,11 +102,17 @@ __device__ __forceinline__ void write_face_ghosts(const Problem &P, unsigned lon
 }
 
 // calc_prims of one cell: the reference's arithmetic in FP64 mode, the
-// guarded fast path in FP32 mode.
+// guarded fast path in FP32 mode.  pg is prims_param's value, read from the
+// Problem once per kernel (a field read inside a plane loop would be
+// reloaded after every block barrier, whose memory clobber covers it).
 template <class R>
-__device__ __forceinline__ void prims(const Problem &P, const R u[5], R w[5]) {
-    if constexpr (sizeof(R) == 8) cons2prim(P.gamma, u, w);
-    else cons2prim_fast(P.gamma - 1.0, u, w);
+__device__ __forceinline__ double prims_param(const Problem &P) {
+    return sizeof(R) == 8 ? P.gamma : P.gamma - 1.0;
+}
+template <class R>
+__device__ __forceinline__ void prims(double pg, const R u[5], R w[5]) {
+    if constexpr (sizeof(R) == 8) cons2prim(pg, u, w);  // pg = gamma
+    else cons2prim_fast(pg, u, w);                      // pg = gamma - 1
 }
 
 // Addressing of the z march.  A thread computes the byte offset of its
@@ -252,10 +258,11 @@ __device__ __forceinline__ R godunov_src(R u, R fp, R fm, R gp, R gm, R hp, R hm
 
 // Gravity source S (sources.f90:886-893) from the cell's primitives.  In
 // FP32 mode `gpre`, when not null, holds the cell's point-mass acceleration
-// as gravity_fast gives it (read from the precomputed field).
+// as gravity_fast gives it (read from the precomputed field), and `phi` is
+// the Problem's self-gravity potential (null without one).
 template <class R>
 __device__ __forceinline__ void gravity_src(const Problem &P, int i, int j, int k, const R w[5],
-                                            const float *gpre, double s[5]) {
+                                            const float *gpre, const double *phi, double s[5]) {
     if constexpr (sizeof(R) == 8) {
         gravity_source(P, i, j, k, w[0], w[1], w[2], w[3], s);
     } else {
@@ -267,7 +274,7 @@ __device__ __forceinline__ void gravity_src(const Problem &P, int i, int j, int 
         } else {
             gravity_fast(P, i, j, k, g);
         }
-        if (P.phi) {  // central differences of the self-gravity potential (FP64)
+        if (phi) {  // central differences of the self-gravity potential (FP64)
             g[0] += (float)((phi_at(P, i + 1, j, k) - phi_at(P, i - 1, j, k)) / (2.0 * P.dx));
             g[1] += (float)((phi_at(P, i, j + 1, k) - phi_at(P, i, j - 1, k)) / (2.0 * P.dy));
             g[2] += (float)((phi_at(P, i, j, k + 1) - phi_at(P, i, j, k - 1)) / (2.0 * P.dz));
@@ -284,9 +291,10 @@ __device__ __forceinline__ void gravity_src(const Problem &P, int i, int j, int 
 // The gravity source times the step h, in the sweep type.
 template <class R>
 __device__ __forceinline__ void gravity_src_h(const Problem &P, int i, int j, int k, const R w[5],
-                                              const float *gpre, double h, R s[5]) {
+                                              const float *gpre, const double *phi, double h,
+                                              R s[5]) {
     double sd[5];
-    gravity_src<R>(P, i, j, k, w, gpre, sd);
+    gravity_src<R>(P, i, j, k, w, gpre, phi, sd);
     if constexpr (sizeof(R) == 8) {
 #pragma unroll
         for (int e = 0; e < 5; ++e) s[e] = h * sd[e];
@@ -401,7 +409,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
     R uc[5], wc[5], hm[5] = {}, pf_n[5], pf_h[5];
     float pf_g[3] = {};
     load_col(U, pl, cc, z0 - 1, uc);
-    prims(P, uc, wc);
+    prims(prims_param<R>(P), uc, wc);
     R csc = sound_speed(gam, wc);
     load_col(U, pl, cc, z0, pf_n);
     if (ht.kind) load_col(U, pl, ch, z0 - 1, pf_h);
@@ -422,7 +430,9 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
             if (ht.kind) load_col(U, pl, ch, ku + 1, pf_h);
             load_gacc(gacc, pl, cc, ku + 1, pf_g);
         }
-        prims(P, un, wn);
+        // (Stage 1 reads its few Problem fields inside the loop: held in
+        // registers they cost spills at its three waves per SIMD.)
+        prims(prims_param<R>(P), un, wn);
         const R csn = sound_speed(gam, wn);
         riemann_c<2, SOL>(gr, wc, wn, uc, un, csc, csn, hp);
         // No barrier before the plane's stores: sW and sU were last read by
@@ -436,7 +446,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
         sC[ty + 1][tx + 1] = csc;
         if (ht.kind) {
             R wh[5];
-            prims(P, hr, wh);
+            prims(prims_param<R>(P), hr, wh);
 #pragma unroll
             for (int e = 0; e < 5; ++e) {
                 sW[e][ht.ly + 1][ht.lx + 1] = wh[e];
@@ -493,7 +503,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage1(const Problem *__restr
         lds_barrier();
         if (active && k >= z0) {
             R s[5] = {};
-            if (grav) gravity_src_h<R>(P, i, j, k, wc, gacc ? gk : nullptr, dth, s);
+            if (grav) gravity_src_h<R>(P, i, j, k, wc, gacc ? gk : nullptr, P.phi, dth, s);
             R v[5];
 #pragma unroll
             for (int e = 0; e < 5; ++e)
@@ -571,12 +581,22 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
     const Column cc = column_of<R>(g, i, j);
     const Column ch = column_of<R>(g, x0 + ht.lx, y0 + ht.ly);
     const float *const gacc = sizeof(R) == 4 ? P.gacc : nullptr;
+    // Problem fields the plane loop uses, read once (see prims_param).
+    const double pg = prims_param<R>(P);
+    const double gamma_d = P.gamma;
+    const double *const phi = P.phi;
+    const CoolF32 cc32 = cool_consts(P);
+    const bool netmode = P.cooling == 2;
+    const float net_min = P.net_min_dtc;
+    const int nz = g.nz;
+    const bool gedge = i <= 2 || i >= g.nx - 1 || j <= 2 || j >= g.ny - 1;
+    const long long cell0 = g.idx(i, j, 0), sxy = g.sxy;
 
     const double dt = dt_from_maxima(P, st->maxima[parity]);
     const double dtdx = dt / P.dx, dtdy = dt / P.dy, dtdz = dt / P.dz;
     const R gam = (R)P.gamma, gr = (R)(P.gamma / (P.gamma - 1.0));
     const int lim = P.limiter;
-    const bool grav = P.nmasses > 0 || P.phi != nullptr;
+    const bool grav = P.nmasses > 0 || phi != nullptr;
     // Cooling mode 1: the reference's one-step formula on every cell.  Modes
     // 2, 3 and 4: cells with dt <= 0.01 t_cool take the one substep the
     // converged integrator would take; the others are queued for the dense
@@ -613,11 +633,11 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
     {
         R wmm[5], t5[5];
         load_col(Uh, pl, cc, z0 - 2, t5);
-        prims(P, t5, wmm);
+        prims(pg, t5, wmm);
         load_col(Uh, pl, cc, z0 - 1, uc);
-        prims(P, uc, wc);
+        prims(pg, uc, wc);
         load_col(Uh, pl, cc, z0, up);
-        prims(P, up, wp);
+        prims(pg, up, wp);
 #pragma unroll
         for (int e = 0; e < 5; ++e) hz[e] = avg<LIM>(lim, wc[e] - wmm[e], wp[e] - wc[e]);
         load_col(U, pl, cc, z0 - 1, nc);
@@ -645,7 +665,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
             if (ht.kind) load_col(hsrc, pl, ch, ku + 1, pf_h);
             load_gacc(gacc, pl, cc, ku + 1, pf_g);
         }
-        prims(P, uq, wq);
+        prims(pg, uq, wq);
         R hzp[5];
         {
             R wl[5], wr[5];
@@ -673,7 +693,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
             }
             if (ht.kind == 1) {
                 R wh[5];
-                prims(P, hr, wh);
+                prims(pg, hr, wh);
 #pragma unroll
                 for (int e = 0; e < 5; ++e) sW[e][ht.ly + 2][ht.lx + 2] = wh[e];
                 if (ht.lx >= -1 && ht.lx <= TX && ht.ly >= -1 && ht.ly <= TY) {
@@ -777,13 +797,13 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
                                        hm[e], dtdx, dtdy, dtdz);
                 if (grav) {
                     double s[5];
-                    gravity_src<R>(P, i, j, k, wc, gacc ? gk : nullptr, s);
+                    gravity_src<R>(P, i, j, k, wc, gacc ? gk : nullptr, phi, s);
 #pragma unroll
                     for (int e = 0; e < 5; ++e) v[e] = v[e] + dt * s[e];
                 }
                 const unsigned long long wmask = s_wmask;
                 // ghosts of U^{n+1} = boundary fill of the pre-wind UP, then winds
-                write_face_ghosts(P, wmask, Unew, i, j, k, v, false);
+                if (gedge || k <= 2 || k >= nz - 1) write_face_ghosts(P, wmask, Unew, i, j, k, v, false);
                 double o[5];
 #pragma unroll
                 for (int e = 0; e < 5; ++e) o[e] = v[e];
@@ -806,7 +826,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
                 if constexpr (sizeof(R) == 8) {
                     const double rho = oq[0];
                     const double vx = oq[1] / oq[0], vy = oq[2] / oq[0], vz = oq[3] / oq[0];
-                    double p = pressure64(P.gamma, rho, oq[1], oq[2], oq[3], oq[4]);
+                    double p = pressure64(gamma_d, rho, oq[1], oq[2], oq[3], oq[4]);
                     if (cool || coolpass) {
                         bool cooled;
                         double cdt = 0.0;
@@ -820,7 +840,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
                         }
                     }
                     if (!queued) {
-                        const double cs = sqrt(P.gamma * p / rho);
+                        const double cs = sqrt(gamma_d * p / rho);
                         mxd = fmax(mxd, fabs(vx) + cs);
                         myd = fmax(myd, fabs(vy) + cs);
                         mzd = fmax(mzd, fabs(vz) + cs);
@@ -835,16 +855,16 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
                     if (cool || coolpass) {
                         bool cooled;
                         float cdt = 0.f;
-                        const float pn = cool_fast_tab(P, sTab, oq[0], p, (float)dt, &cooled, &cdt);
+                        const float pn = cool_fast_tab(cc32, sTab, oq[0], p, (float)dt, &cooled, &cdt);
                         queued = coolpass && cooled && cdt > COOL_EPS;
                         qdtc = cdt;
                         if (cooled && !queued) {
-                            oq[4] = (float)(ek + (double)pn * (double)P.f_inv_gm1);
+                            oq[4] = (float)(ek + (double)pn * (double)cc32.inv_gm1);
                             p = pn;
                         }
                     }
                     if (!queued) {
-                        const float cs = __builtin_amdgcn_sqrtf((float)P.gamma * p * rinv);
+                        const float cs = __builtin_amdgcn_sqrtf((float)gamma_d * p * rinv);
                         mx = fmaxf(mx, fabsf(vx) + cs);
                         my = fmaxf(my, fabsf(vy) + cs);
                         mz = fmaxf(mz, fabsf(vz) + cs);
@@ -852,7 +872,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
                 }
                 store_col(Unew, pl, cc, ku, oq);
                 if (coolpass) {  // append to the tile's queue segment: one LDS atomic per wave
-                    const bool net = queued && P.cooling == 2 && qdtc >= P.net_min_dtc;
+                    const bool net = queued && netmode && qdtc >= net_min;
                     const bool itg = queued && !net;
                     const int lane = tid & 63;
                     const unsigned long long nm = __ballot(net), im = __ballot(itg);
@@ -861,7 +881,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
                         unsigned qb = 0;
                         if (lane == leader) qb = atomicAdd(&s_nq, (unsigned)__popcll(nm));
                         qb = __shfl(qb, leader);
-                        if (net) qseg[qb + __popcll(nm & ((1ull << lane) - 1ull))] = (unsigned)g.idx(i, j, k);
+                        if (net) qseg[qb + __popcll(nm & ((1ull << lane) - 1ull))] = (unsigned)(cell0 + sxy * k);
                     }
                     if (im) {
                         const int leader = __ffsll((long long)im) - 1;
@@ -870,7 +890,7 @@ __global__ __launch_bounds__(TX *TY, WAVES) void k_stage2(const Problem *__restr
                         qb = __shfl(qb, leader);
                         if (itg)
                             qseg[TX * TY * ZC - 1 - (qb + __popcll(im & ((1ull << lane) - 1ull)))] =
-                                (unsigned)g.idx(i, j, k);
+                                (unsigned)(cell0 + sxy * k);
                     }
                 }
             }
