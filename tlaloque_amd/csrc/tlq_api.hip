@@ -334,7 +334,7 @@ template <class R> int do_advance(tlq_ctx *c, int nsteps) {
             HIPCHK(sur_cool_pass<R>(c->stream, c->dprob, (int)cool_segments(c), cool_segment(c),
                                     mp<R>(c->u[par ^ 1]), c->sur_thq, c->st, par, c->cool_queue,
                                     c->cool_seg, c->cool_off, c->cool_fqueue, c->cool_fall,
-                                    c->host.cooling));
+                                    c->host.cooling, c->host.ntable));
         }
         c->cur ^= 1;
         ++c->it_host;

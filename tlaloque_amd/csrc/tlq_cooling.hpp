@@ -33,22 +33,63 @@ static __device__ double cool_sub_d(const Problem &P, double n, double T, double
     return T;
 }
 
+// The FP32 cooling table as the dense cooling passes hold it in LDS: the rows
+// of cool_fast_tab (TabRow: T, log10 T, log10 Lambda, slope of the segment
+// ending at the row) and, for the closed-form cooling time, the node values
+// of Y (FP64) with the segment coefficients.  The values are the Problem's
+// FP32 table (and tab_Ycum, tab_ycoef), so a lookup gives what a lookup in
+// global memory gives, at LDS latency.
+struct YRow {
+    double Ycum;  // Y at the row's temperature
+    float ycoef;  // T_{i-1} / Lambda_{i-1} of the segment ending at the row
+    float pad;
+};
+
+struct CoolTab {
+    CoolF32 C;
+    float cgk;     // (gamma - 1) / k_B
+    float snap_T;  // SNAP T_min: where an integration that reaches the gate ends
+    const TabRow *row;
+    const YRow *y;  // null when only the rows are staged
+};
+
+// LDS bytes of a staged table of n rows (rows only, or rows and Y).
+__host__ __device__ constexpr unsigned cool_tab_bytes(int n, bool with_y) {
+    return (unsigned)n * (unsigned)(sizeof(TabRow) + (with_y ? sizeof(YRow) : 0));
+}
+
+// Fills row[] (and y[] when not null) from the Problem; the caller
+// synchronises the block before use.
+__device__ __forceinline__ void stage_cool_tab(const Problem &P, TabRow *row, YRow *y, int tid,
+                                               int nthreads) {
+    stage_table(P, row, tid, nthreads);
+    if (y)
+        for (int q = tid; q < P.ntable; q += nthreads) y[q] = YRow{P.tab_Ycum[q], P.tab_ycoef[q], 0.f};
+}
+
+constexpr double SNAP = 0.995;
+
+__device__ __forceinline__ CoolTab cool_tab(const Problem &P, const TabRow *row, const YRow *y) {
+    return CoolTab{cool_consts(P), (float)((P.gamma - 1.0) / KB), (float)(SNAP * P.tmin), row, y};
+}
+
 // Table segment of temperature T (first row with T_i > T, as find_coef),
 // clamped to the last segment for T >= T_max.
-__device__ __forceinline__ int table_row(const Problem &P, float temp, float lg) {
-    if (!(temp < P.f_tmax)) return P.ntable - 1;
+__device__ __forceinline__ int table_row(const CoolTab &t, float temp, float lg) {
+    const int n = t.C.ntable;
+    if (!(temp < t.C.tmax)) return n - 1;
     int hi;
-    if (P.tab_uniform) {
-        hi = (int)((lg - P.f_lgT0) * P.f_inv_dlg) + 1;
-        hi = hi < 1 ? 1 : (hi > P.ntable - 1 ? P.ntable - 1 : hi);
-        if (!(P.tab_Tf[hi] > temp) && hi < P.ntable - 1) ++hi;
-        if (hi > 1 && P.tab_Tf[hi - 1] > temp) --hi;
+    if (t.C.uniform) {
+        hi = (int)((lg - t.C.lgT0) * t.C.inv_dlg) + 1;
+        hi = hi < 1 ? 1 : (hi > n - 1 ? n - 1 : hi);
+        if (!(t.row[hi].T > temp) && hi < n - 1) ++hi;
+        if (hi > 1 && t.row[hi - 1].T > temp) --hi;
     } else {
         int lo = 0;
-        hi = P.ntable - 1;
+        hi = n - 1;
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
-            if (P.tab_Tf[mid] > temp) hi = mid;
+            if (t.row[mid].T > temp) hi = mid;
             else lo = mid;
         }
     }
@@ -56,11 +97,11 @@ __device__ __forceinline__ int table_row(const Problem &P, float temp, float lg)
 }
 
 // Lambda(T) from the table in FP32 (find_coef's interpolation).
-__device__ __forceinline__ float lambda_f(const Problem &P, float temp) {
+__device__ __forceinline__ float lambda_f(const CoolTab &t, float temp) {
     const float lg = __builtin_amdgcn_logf(temp) * 0.301029995663981f;
-    if (!(temp < P.f_tmax)) return __builtin_amdgcn_exp2f(P.tab_Lf[P.ntable - 1] * 3.32192809488736f);
-    const int hi = table_row(P, temp, lg);
-    const float lgl = P.tab_Lf[hi - 1] + P.tab_slopef[hi] * (lg - P.tab_lgTf[hi - 1]);
+    if (!(temp < t.C.tmax)) return __builtin_amdgcn_exp2f(t.row[t.C.ntable - 1].L * 3.32192809488736f);
+    const int hi = table_row(t, temp, lg);
+    const float lgl = t.row[hi - 1].L + t.row[hi].slope * (lg - t.row[hi - 1].lgT);
     return __builtin_amdgcn_exp2f(lgl * 3.32192809488736f);
 }
 
@@ -74,30 +115,29 @@ struct YSplit {
     double node;
     float part;
 };
-__device__ __forceinline__ YSplit cool_Y(const Problem &P, float temp) {
-    if (!(temp > P.f_tmin)) return {0.0, 0.f};
+__device__ __forceinline__ YSplit cool_Y(const CoolTab &t, float temp) {
+    if (!(temp > t.C.tmin)) return {0.0, 0.f};
     const float l2 = __builtin_amdgcn_logf(temp);
-    const int hi = table_row(P, temp, l2 * 0.301029995663981f);
-    const float a = 1.f - P.tab_slopef[hi];
-    const float lr = l2 - __builtin_amdgcn_logf(P.tab_Tf[hi - 1]);  // log2(T/T_{hi-1})
+    const int hi = table_row(t, temp, l2 * 0.301029995663981f);
+    const float a = 1.f - t.row[hi].slope;
+    const float lr = l2 - __builtin_amdgcn_logf(t.row[hi - 1].T);  // log2(T/T_{hi-1})
     const float seg = fabsf(a) > 1e-4f ? (__builtin_amdgcn_exp2f(a * lr) - 1.f) / a
                                        : lr * 0.6931471805599453f;
-    return {P.tab_Ycum[hi - 1], P.tab_ycoef[hi] * seg};
+    return {t.y[hi - 1].Ycum, t.y[hi].ycoef * seg};
 }
-__device__ __forceinline__ float cool_dY(const Problem &P, float T0, float T1) {
-    const YSplit a = cool_Y(P, T0), b = cool_Y(P, T1);
+__device__ __forceinline__ float cool_dY(const CoolTab &t, float T0, float T1) {
+    const YSplit a = cool_Y(t, T0), b = cool_Y(t, T1);
     return (float)(a.node - b.node) + (a.part - b.part);
 }
 
-static __device__ float cool_sub_f(const Problem &P, float n, float T, float dt, float eps, int max_sub) {
-    const float cgk = (float)((P.gamma - 1.0) / KB);
-    const float floor_f = P.f_floor_f;
+static __device__ float cool_sub_f(const CoolTab &t, float n, float T, float dt, float eps,
+                                   int max_sub) {
     float left = dt;
-    for (int s = 0; left > 0.f && T >= P.f_tmin && s < max_sub; ++s) {
-        const float ce = lambda_f(P, T) * n * cgk * __builtin_amdgcn_rcpf(T);
+    for (int s = 0; left > 0.f && T >= t.C.tmin && s < max_sub; ++s) {
+        const float ce = lambda_f(t, T) * n * t.cgk * __builtin_amdgcn_rcpf(T);
         const float h = fminf(left, eps * __builtin_amdgcn_rcpf(ce));
         float f = __builtin_amdgcn_exp2f(-1.4426950408889634f * ce * h);
-        if (f < floor_f) f = floor_f;
+        if (f < t.C.floor_f) f = t.C.floor_f;
         if (T * f < 10.f) f = 10.f * __builtin_amdgcn_rcpf(T);
         T = T * f;
         left -= h;
@@ -114,7 +154,6 @@ static __device__ float cool_sub_f(const Problem &P, float n, float T, float dt,
 // (Townsend 2009, ApJS 181, 391).  It is the eps -> 0 limit of the substepped
 // integrator.  At the gate (the cooling time to T_min at most dt) the result is
 // SNAP T_min, where the substepped integrator's last substep ends.
-constexpr double SNAP = 0.995;
 
 // FP64: the node values of Y from the host (tab_Ycum), the rest with the
 // device's pow/log/exp.
@@ -157,37 +196,40 @@ static __device__ double cool_int_d(const Problem &P, double n, double T0, doubl
 }
 
 // FP32: Y's node values in FP64, the segment parts in FP32 (cool_Y).
-static __device__ float cool_int_f(const Problem &P, float n, float T0, float dt) {
-    if (!(T0 >= P.f_tmin)) return T0;
-    const int last = P.ntable - 1;
-    const float lam_top = __builtin_amdgcn_exp2f(P.tab_Lf[last] * 3.32192809488736f);
+static __device__ float cool_int_f(const CoolTab &t, float n, float T0, float dt) {
+    if (!(T0 >= t.C.tmin)) return T0;
+    const int last = t.C.ntable - 1;
+    const float lam_top = __builtin_amdgcn_exp2f(t.row[last].L * 3.32192809488736f);
+    const double ytop = t.y[last].Ycum;
     double y0;
-    if (!(T0 < P.f_tmax)) {
-        y0 = P.tab_Ycum[last] + (double)((T0 - P.f_tmax) / lam_top);
+    if (!(T0 < t.C.tmax)) {
+        y0 = ytop + (double)((T0 - t.C.tmax) / lam_top);
     } else {
-        const YSplit q = cool_Y(P, T0);
+        const YSplit q = cool_Y(t, T0);
         y0 = q.node + (double)q.part;
     }
-    const double y1 = y0 - (double)dt * (double)((float)((P.gamma - 1.0) / KB) * n);
-    if (!(y1 > 0.0)) return (float)(SNAP * P.tmin);
-    if (!(y1 < P.tab_Ycum[last])) return P.f_tmax + (float)(y1 - P.tab_Ycum[last]) * lam_top;
+    const double y1 = y0 - (double)dt * (double)(t.cgk * n);
+    if (!(y1 > 0.0)) return t.snap_T;
+    if (!(y1 < ytop)) return t.C.tmax + (float)(y1 - ytop) * lam_top;
     int lo = 0, hi = last;  // Ycum[lo] <= y1 < Ycum[hi]
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        if (P.tab_Ycum[mid] > y1) hi = mid;
+        if (t.y[mid].Ycum > y1) hi = mid;
         else lo = mid;
     }
-    const float a = 1.f - P.tab_slopef[hi];
-    const float s = (float)(y1 - P.tab_Ycum[hi - 1]) / P.tab_ycoef[hi];
+    const float a = 1.f - t.row[hi].slope;
+    const float s = (float)(y1 - t.y[hi - 1].Ycum) / t.y[hi].ycoef;
     const float r = fabsf(a) > 1e-4f
                         ? __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(fmaxf(1.f + a * s, 1e-30f)) / a)
                         : __builtin_amdgcn_exp2f(s * 1.4426950408889634f);
-    return P.tab_Tf[hi - 1] * r;
+    return t.row[hi - 1].T * r;
 }
 
-template <class R> __device__ __forceinline__ double cool_int(const Problem &P, double n, double T, double dt) {
+template <class R>
+__device__ __forceinline__ double cool_int(const Problem &P, const CoolTab &t, double n, double T,
+                                           double dt) {
     if constexpr (sizeof(R) == 8) return cool_int_d(P, n, T, dt);
-    else return (double)cool_int_f(P, (float)n, (float)T, (float)dt);
+    else return (double)cool_int_f(t, (float)n, (float)T, (float)dt);
 }
 
 // ---------------------------------------------------------------------------
@@ -236,23 +278,11 @@ __device__ __forceinline__ CellState cell_state(const Problem &P, const R *__res
     return cell_state_vals<R>(P, u);
 }
 
-// dt / t_cool of a cell at T >= T_min
 template <class R>
-__device__ __forceinline__ double dt_over_tcool(const Problem &P, const CellState &s, double dt) {
-    if constexpr (sizeof(R) == 8) {
-        return find_coef(P, s.T) * s.nd * s.nd / (s.p / (P.gamma - 1.0)) * dt;
-    } else {
-        const float ce = lambda_f(P, (float)s.T) * (float)s.nd * (float)s.nd *
-                         (float)(P.gamma - 1.0) * __builtin_amdgcn_rcpf((float)s.p);
-        return (double)(ce * (float)dt);
-    }
-}
-
-template <class R>
-__device__ __forceinline__ double integrate(const Problem &P, const CellState &s, double dt,
-                                            int max_sub) {
+__device__ __forceinline__ double integrate(const Problem &P, const CoolTab &t, const CellState &s,
+                                            double dt, int max_sub) {
     if constexpr (sizeof(R) == 8) return cool_sub_d(P, s.nd, s.T, dt, COOL_EPS, max_sub);
-    else return cool_sub_f(P, (float)s.nd, (float)s.T, (float)dt, COOL_EPS, max_sub);
+    else return cool_sub_f(t, (float)s.nd, (float)s.T, (float)dt, COOL_EPS, max_sub);
 }
 
 // A cell cooled to Tn at fixed density and momentum: its new total energy

@@ -529,12 +529,21 @@ __global__ __launch_bounds__(128) void k_cool_dense(const Problem *__restrict__ 
                                                     const unsigned *__restrict__ offn,
                                                     const unsigned *__restrict__ offi, int nseg,
                                                     long long C, unsigned *__restrict__ fqueue,
-                                                    unsigned *__restrict__ fcount) {
+                                                    unsigned *__restrict__ fcount, int with_y) {
     __shared__ MlpSmall sm;
     __shared__ unsigned s_fbuf[MODE == 2 ? 2 : 1][MODE == 2 ? FBUF : 1];
+    // The FP32 cooling table in LDS (cool_tab_bytes(ntable, with_y) bytes of
+    // dynamic LDS): for the network's inputs and check (MODE 2, either state
+    // precision) and the FP32 integrators; none in the FP64 integrator pass,
+    // whose arithmetic reads the FP64 table.
+    extern __shared__ __align__(16) unsigned char s_tab[];
     const Problem &P = *Pp;
     const Grid &g = P.g;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    TabRow *const trow = reinterpret_cast<TabRow *>(s_tab);
+    YRow *const ty = with_y ? reinterpret_cast<YRow *>(s_tab + P.ntable * sizeof(TabRow)) : nullptr;
+    if (sizeof(R) == 4 || MODE == 2) stage_cool_tab(P, trow, ty, threadIdx.x, blockDim.x);  // visible after the barrier below
+    const CoolTab tab = cool_tab(P, trow, ty);
     const unsigned nseg_cells = MODE == 2 ? offn[nseg] : offi[nseg];
     const unsigned nq = MODE == 2 ? nseg_cells : nseg_cells + *fcount;
     const double dt = dt_from_maxima(P, st->maxima[parity]);
@@ -542,13 +551,13 @@ __global__ __launch_bounds__(128) void k_cool_dense(const Problem *__restrict__ 
     bf16x4 w2a[4][4];
     float b3 = 0.f;
     // T_min / Lambda(T_min): k_B T_min / ((gamma-1) n Lambda) is the cooling time at the gate
-    const float gate_tc = MODE == 2 ? P.f_tmin / lambda_f(P, P.f_tmin) : 0.f;
     if (MODE == 2 && nq > 0) {
         load_small_f(sm, thq, threadIdx.x, blockDim.x);
         b3 = bf2f(thq[OB3]);
         load_w2_fwd(thq, lane, w2a);
     }
     __syncthreads();
+    const float gate_tc = MODE == 2 ? P.f_tmin / lambda_f(tab, P.f_tmin) : 0.f;
     double mx = 0.0, my = 0.0, mz = 0.0;
     const unsigned stride = gridDim.x * 128u;
     for (unsigned base = (blockIdx.x * 2u + wv) * 64u; base < nq; base += stride) {
@@ -578,12 +587,12 @@ __global__ __launch_bounds__(128) void k_cool_dense(const Problem *__restrict__ 
             // check below accepts when the network predicts the gate, given
             // without the network.
             const bool gate =
-                valid && kn * cool_dY(P, T0, P.f_tmin) <= fdt + 0.05f * fminf(fdt, kn * gate_tc);
+                valid && kn * cool_dY(tab, T0, P.f_tmin) <= fdt + 0.05f * fminf(fdt, kn * gate_tc);
             const bool net = valid && !gate;
             bool ok = false, snap = false;
             float T1 = T0;
             if (__ballot(net)) {  // wavefront-uniform: the network runs unless every cell is gated
-                const float lam = lambda_f(P, T0);
+                const float lam = lambda_f(tab, T0);
                 const float ce = lam * (float)s.nd * (float)s.nd * (float)(P.gamma - 1.0) *
                                  __builtin_amdgcn_rcpf((float)s.p);
                 float x[3] = {lg10((float)s.nd), lg10(T0), lg10(ce * fdt)};
@@ -601,15 +610,15 @@ __global__ __launch_bounds__(128) void k_cool_dense(const Problem *__restrict__ 
                     // a time error dt' shifts ln T1 by dt'/t_cool(T1): allow 5% of
                     // the change in ln T, at most 0.05
                     const float Te = fmaxf(T1, P.f_tmin);
-                    const float tol = 0.05f * fminf(fdt, kn * Te / lambda_f(P, Te));
+                    const float tol = 0.05f * fminf(fdt, kn * Te / lambda_f(tab, Te));
                     if (T1 <= 1.1f * P.f_tmin) {
                         // the network says the gas reaches the gate: accepted when
                         // the exact cooling time to T_min fits in dt, and the answer
                         // is the gate itself (where the integrator stops)
-                        ok = kn * cool_dY(P, T0, P.f_tmin) <= fdt + tol;
+                        ok = kn * cool_dY(tab, T0, P.f_tmin) <= fdt + tol;
                         snap = true;
                     } else {
-                        ok = fabsf(kn * cool_dY(P, T0, T1) - fdt) <= tol;
+                        ok = fabsf(kn * cool_dY(tab, T0, T1) - fdt) <= tol;
                     }
                 }
             }
@@ -634,8 +643,8 @@ __global__ __launch_bounds__(128) void k_cool_dense(const Problem *__restrict__ 
             // integrator cells and fallbacks: the exact integration
             if (valid)
                 finish_cell(P, U, g.nc, c, u, s,
-                            P.cooling == 3 ? integrate<R>(P, s, dt, COOL_MAXSUB)
-                                           : cool_int<R>(P, s.nd, s.T, dt),
+                            P.cooling == 3 ? integrate<R>(P, tab, s, dt, COOL_MAXSUB)
+                                           : cool_int<R>(P, tab, s.nd, s.T, dt),
                             mx, my, mz);
         }
     }
@@ -685,24 +694,32 @@ template <class R>
 hipError_t sur_cool_pass(hipStream_t s, const Problem *P, int nseg, long long C, R *U,
                          const unsigned short *thq, StepState *st, int parity, unsigned *queue,
                          const unsigned *seg, unsigned *off, unsigned *fqueue, unsigned *fcount,
-                         int mode) {
+                         int mode, int ntable) {
     constexpr unsigned nb = 4096;  // 8192 wavefronts, grid-stride over the queue
     unsigned *offn = off, *offi = off + nseg + 1;
+    // The network pass reads the FP32 table and Y (inputs and check); in FP32
+    // the integrator of mode 3 reads the rows, and the exact integration (mode
+    // 4, and the surrogate's integrator cells and fallbacks) also Y.
+    const int y3 = mode != 3;
+    const unsigned lds2 = cool_tab_bytes(ntable, true);
+    const unsigned lds3 = sizeof(R) == 4 ? cool_tab_bytes(ntable, y3) : 0u;
     hipLaunchKernelGGL(k_queue_scan, dim3(1), dim3(1024), 0, s, seg, offn, nseg, fcount);
     hipLaunchKernelGGL(k_queue_scan, dim3(1), dim3(1024), 0, s, seg + nseg, offi, nseg, fcount);
     if (mode == 2)
-        hipLaunchKernelGGL((k_cool_dense<R, 2>), dim3(nb), dim3(128), 0, s, P, U, thq, st, parity,
-                           queue, offn, offi, nseg, C, fqueue, fcount);
-    hipLaunchKernelGGL((k_cool_dense<R, 3>), dim3(nb), dim3(128), 0, s, P, U, thq, st, parity,
-                       queue, offn, offi, nseg, C, fqueue, fcount);
+        hipLaunchKernelGGL((k_cool_dense<R, 2>), dim3(nb), dim3(128), lds2, s, P, U, thq, st,
+                           parity, queue, offn, offi, nseg, C, fqueue, fcount, 1);
+    hipLaunchKernelGGL((k_cool_dense<R, 3>), dim3(nb), dim3(128), lds3, s, P, U, thq, st, parity,
+                       queue, offn, offi, nseg, C, fqueue, fcount, y3);
     return hipGetLastError();
 }
 
 template hipError_t sur_cool_pass<float>(hipStream_t, const Problem *, int, long long, float *,
                                          const unsigned short *, StepState *, int, unsigned *,
-                                         const unsigned *, unsigned *, unsigned *, unsigned *, int);
+                                         const unsigned *, unsigned *, unsigned *, unsigned *, int,
+                                         int);
 template hipError_t sur_cool_pass<double>(hipStream_t, const Problem *, int, long long, double *,
                                           const unsigned short *, StepState *, int, unsigned *,
-                                          const unsigned *, unsigned *, unsigned *, unsigned *, int);
+                                          const unsigned *, unsigned *, unsigned *, unsigned *, int,
+                                          int);
 
 }  // namespace tlq

@@ -67,6 +67,6 @@ template <class R>
 hipError_t sur_cool_pass(hipStream_t s, const Problem *P, int nseg, long long C, R *U,
                          const unsigned short *thq, StepState *st, int parity, unsigned *queue,
                          const unsigned *seg, unsigned *off, unsigned *fqueue, unsigned *fcount,
-                         int mode);
+                         int mode, int ntable);
 
 }  // namespace tlq
