@@ -6,6 +6,14 @@ Two reports, both markdown with the command that produced the data:
              launch resources (VGPR, AGPR, SGPR, LDS, scratch, grid, block),
              from a ``rocprofv3 --kernel-trace --stats --output-format csv`` run.
 
+``counters`` the mean per launch of every counter of one or more
+             ``rocprofv3 --pmc`` runs for the kernels whose names match a
+             pattern, and each counter per work unit (zones, or queued
+             cells) given on the command line: wave instructions per unit
+             times 64 for the SQ_INSTS_* counters (lane instructions per
+             unit, likewise SQ_ACTIVE_INST_* quad-cycles), the raw ratio
+             for the others.
+
 ``traffic``  measured HBM-side bytes per launch of each kernel from two
              ``rocprofv3 --pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE`` runs.  The
              counters report KiB per dispatch; each is scaled by what it reads
@@ -16,6 +24,8 @@ Two reports, both markdown with the command that produced the data:
 
 Usage (from the repository root):
     python -m tlaloque_amd.profiling stats TRACE.csv --cmd "..." --title "..."
+    python -m tlaloque_amd.profiling counters PMC.csv [PMC2.csv ...] --match k_stage \
+        --units N --cmd "..." --title "..."
     python -m tlaloque_amd.profiling traffic FETCH.csv WRITE.csv --copy-bytes B \
         --zones N --cmd-fetch "..." --cmd-write "..." --title "..."
 """
@@ -104,10 +114,10 @@ def traffic_report(fetch_csv: str, write_csv: str, copy_bytes: int, zones: int,
         cal_line = (f"Calibration: the factors of `{calibration}` (FETCH_SIZE x {f_fetch:.4f}, "
                     f"WRITE_SIZE x {f_write:.4f}, bytes per counted byte, from a streaming copy).")
     else:
-        ck = [k for k in fetch if COPY_KERNEL in k]
-        if not ck or not any(COPY_KERNEL in k for k in write):
+        # the plain copy kernel (not its variants of the copy-probe sweep)
+        if COPY_KERNEL not in fetch or COPY_KERNEL not in write:
             raise SystemExit("no copy-kernel dispatch in the counter runs: cannot calibrate")
-        ck = ck[0]
+        ck = COPY_KERNEL
         # the copy reads copy_bytes and writes copy_bytes
         f_fetch = copy_bytes / (fetch[ck] * 1024.0)
         f_write = copy_bytes / (write[ck] * 1024.0)
@@ -141,6 +151,33 @@ def traffic_report(fetch_csv: str, write_csv: str, copy_bytes: int, zones: int,
     return "\n".join(lines) + "\n"
 
 
+def counters_report(pmc_csvs: List[str], match: str, units: float, unit_name: str, cmd: str,
+                    title: str) -> str:
+    """Mean per launch of each counter of the kernels matching `match`, and
+    per work unit (SQ_INSTS_* as lane instructions: wave instructions x 64)."""
+    per: Dict[str, Dict[str, List[float]]] = collections.defaultdict(
+        lambda: collections.defaultdict(list))
+    for path in pmc_csvs:
+        for r in _rows(path):
+            name = short_name(r["Kernel_Name"])
+            if re.search(match, name):
+                per[name][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    lines = [f"# {title}", "", "Commands (one MI355X, from the repository root; one pass per command):",
+             "", "```", cmd, "```", "",
+             f"Values are means over each kernel's launches in the runs; per-unit figures divide by "
+             f"{units:.6g} {unit_name} per launch; SQ_INSTS_* (wave instructions) and SQ_ACTIVE_INST_* "
+             f"(quad-cycles a wave issues) are scaled by 64, the lanes of a wave.", "",
+             "| kernel | counter | per launch | per unit | launches |", "|---|---|---|---|---|"]
+    for name in sorted(per):
+        for cn in sorted(per[name]):
+            v = per[name][cn]
+            mean = sum(v) / len(v)
+            lane = cn.startswith(("SQ_INSTS_", "SQ_ACTIVE_INST_"))
+            pu = mean * (64.0 if lane else 1.0) / units if units else 0.0
+            lines.append(f"| `{name}` | {cn} | {mean:.4g} | {pu:.4g} | {len(v)} |")
+    return "\n".join(lines) + "\n"
+
+
 def read_traffic(path: str) -> dict:
     """The JSON line of a traffic report."""
     with open(path) as fh:
@@ -157,6 +194,13 @@ def main(argv=None):
     s.add_argument("trace_csv")
     s.add_argument("--cmd", required=True)
     s.add_argument("--title", required=True)
+    c = sub.add_parser("counters")
+    c.add_argument("pmc_csv", nargs="+")
+    c.add_argument("--match", required=True, help="regular expression on the kernel name")
+    c.add_argument("--units", type=float, required=True, help="work units per launch")
+    c.add_argument("--unit-name", default="zones")
+    c.add_argument("--cmd", required=True)
+    c.add_argument("--title", required=True)
     t = sub.add_parser("traffic")
     t.add_argument("fetch_csv")
     t.add_argument("write_csv")
@@ -169,6 +213,8 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.what == "stats":
         sys.stdout.write(stats_report(a.trace_csv, a.cmd, a.title))
+    elif a.what == "counters":
+        sys.stdout.write(counters_report(a.pmc_csv, a.match, a.units, a.unit_name, a.cmd, a.title))
     else:
         sys.stdout.write(traffic_report(a.fetch_csv, a.write_csv, a.copy_bytes, a.zones,
                                         a.cmd_fetch, a.cmd_write, a.title, a.calibration))
