@@ -193,10 +193,10 @@ int tlq_check_state(tlq_ctx *ctx, long long *bad);
  * integrator instead. */
 int tlq_fallback_fraction(tlq_ctx *ctx, double *frac);
 /* The same step's counts: cells handed to the network (mode 2: T >= T_min and
- * dt >= the threshold below, in units of t_cool) or to the converged
- * integrator (mode 3: T >= T_min and dt > 0.01 t_cool; below 0.01 t_cool one
- * substep of cooling.f90:67-112 is the converged answer), and, of the
- * network's cells, those that took the integrator (mode 2). */
+ * dt >= the threshold below, in units of t_cool) or to the integrator (modes
+ * 3 and 4: T >= T_min and dt > 0.01 t_cool; below 0.01 t_cool one substep of
+ * cooling.f90:67-112 is the converged answer), and, of the network's cells,
+ * those that took the exact integration instead (mode 2). */
 int tlq_cooling_counts(tlq_ctx *ctx, long long *queued, long long *fallbacks);
 
 /* Learned cooling surrogate (MLP 3 -> 64 -> 64 -> 1, GELU, bf16 weights,
@@ -211,10 +211,10 @@ int tlq_cooling_counts(tlq_ctx *ctx, long long *queued, long long *fallbacks);
 /* Random initial parameters (seeded) and fresh Adam moments. */
 int tlq_surrogate_init(tlq_ctx *ctx, unsigned long long seed);
 /* In TLQ_COOL_SURROGATE mode, cells with 0.01 < dt / t_cool < dtc_min take
- * the converged integrator instead of the network: there its few substeps
- * (one per 0.01 t_cool) cost less than the network's 128 activations.
- * Default TLQ_SURROGATE_MIN_DTC (about where the two cost the same on
- * MI355X); 0.01 or less hands every such cell to the network. */
+ * the exact integration (TLQ_COOL_INTEGRATED's) instead of the network:
+ * there the integrators cost less than the network's 128 activations.
+ * Default TLQ_SURROGATE_MIN_DTC; 0.01 or less hands every such cell to the
+ * network. */
 #define TLQ_SURROGATE_MIN_DTC 0.3
 int tlq_surrogate_set_threshold(tlq_ctx *ctx, double dtc_min);
 /* FP32 master parameters in / out (n = TLQ_MLP_NPARAMS); setting them also

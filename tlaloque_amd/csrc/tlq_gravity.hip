@@ -151,7 +151,7 @@ __global__ __launch_bounds__(ZX *ZY) void k_sg_rbz(
     double *__restrict__ partial) {
     constexpr int OX = ZX + 4, OY = ZY + 4, NX = ZX + 2, NY = ZY + 2, T = ZX * ZY;
     static_assert(OX * OY <= 2 * T && NX * NY <= 2 * T, "two ring cells per thread");
-    __shared__ double O[3][OY][OX];   // old planes, ring 2
+    __shared__ double O[4][OY][OX];   // old planes, ring 2 (four: see the loop's end)
     __shared__ double N[3][NY][NX];   // new planes, ring 1
     __shared__ double red[T / 64];
     const int tid = threadIdx.x;
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(ZX *ZY) void k_sg_rbz(
         }
     };
     auto store_plane = [&](int k, const double (&r)[2]) {
-        double(*o)[OX] = O[mod3(k)];
+        double(*o)[OX] = O[k & 3];
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             const int q = tid + s * T;
@@ -222,9 +222,9 @@ __global__ __launch_bounds__(ZX *ZY) void k_sg_rbz(
         // colour 0 of plane kr = t+1 on the tile and its ring of 1
         const int kr = t + 1;
         {
-            double(*om)[OX] = O[mod3(kr - 1)];
-            double(*oc)[OX] = O[mod3(kr)];
-            double(*op)[OX] = O[mod3(kr + 1)];
+            double(*om)[OX] = O[(kr - 1) & 3];
+            double(*oc)[OX] = O[kr & 3];
+            double(*op)[OX] = O[(kr + 1) & 3];
             double(*nn)[NX] = N[mod3(kr)];
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(ZX *ZY) void k_sg_rbz(
                 const int a = lx + 1, b = ly + 1;
                 double v = nc[b][a];
                 if (((i + j + t) & 1) == 1) {
-                    const double old = O[mod3(t)][ly + 2][lx + 2];
+                    const double old = O[t & 3][ly + 2][lx + 2];
                     v = ((1.0 / 6.0) * (nc[b][a + 1] + nc[b][a - 1] + nc[b + 1][a] + nc[b - 1][a] +
                                         N[mod3(t + 1)][b][a] + N[mod3(t - 1)][b][a] - sbc * h2)) * 1.0 +
                         (1.0 - 1.0) * old;
@@ -265,7 +265,11 @@ __global__ __launch_bounds__(ZX *ZY) void k_sg_rbz(
                 eout[sidx(nx, ny, i, j, t)] = v;
             }
         }
-        lds_barrier();
+        // No barrier before the next step: its store of old plane t+3 goes to
+        // the buffer of plane t-1, which no thread reads after the first
+        // barrier of this step, and its colour-0 writes of new plane t+2 (the
+        // buffer of t-1) come after its own first barrier, which every
+        // thread reaches only once it has read plane t-1 here.
     }
     if (partial) {
         const double bs = block_sum<T>(dsum, red);

@@ -329,8 +329,9 @@ class HydroCore:
         return bad.value
 
     def cooling_counts(self) -> Tuple[int, int]:
-        """(cells queued for the network or the converged integrator, fallbacks)
-        of the last step in cooling modes 2 and 3."""
+        """(cells queued for the dense pass, fallbacks) of the last step in
+        cooling modes 2-4: mode 2 counts the network's cells and those of them
+        that took the exact integration; modes 3 and 4 the integrator's cells."""
         q, f = ctypes.c_longlong(), ctypes.c_longlong()
         _check(_lib.tlq_cooling_counts(self._ctx, ctypes.byref(q), ctypes.byref(f)))
         return q.value, f.value
@@ -393,7 +394,7 @@ class HydroCore:
 
     def surrogate_threshold(self, dtc_min: float) -> None:
         """Surrogate mode: cells with 0.01 <= dt/t_cool < dtc_min take the
-        converged integrator instead of the network (include/tlq.h)."""
+        exact integration instead of the network (include/tlq.h)."""
         _check(_lib.tlq_surrogate_set_threshold(self._ctx, float(dtc_min)))
 
     def surrogate_generate(self, n: int, seed: int = 7) -> None:

@@ -9,7 +9,7 @@ convergence (cooling mode TLQ_COOL_SUBCYCLED), and it is trained on the GPU
 by the library's own MFMA forward/backward kernels with Adam
 (include/tlq.h: tlq_surrogate_*).  A trained context then runs the fused
 step with cooling mode TLQ_COOL_SURROGATE, falling back per cell to the
-converged integrator where the network's inputs leave the training box or
+exact integration (TLQ_COOL_INTEGRATED) where the network's inputs leave the training box or
 its answer fails the per-cell check (bounds, and the exact cooling time from
 T to T_new, which the piecewise power-law table gives in closed form, within
 5% of the change of ln T of dt).
