@@ -507,9 +507,6 @@ __device__ __forceinline__ void cons2prim_fast(double gm1, const float u[5], flo
     w[4] = p > 1e-20 ? (float)p : 1e-20f;
 }
 
-// apply_cooling for one cell in FP32 (cooling.f90:67-112): the table row is
-// found by direct indexing when the table is equally spaced in log10 T
-// (Z1.0.dat is: 0.01 dex), else by bisection; 10^x and exp via exp2/log2.
 // The FP32 cooling table as one float4 per row (T, log10 T, log10 Lambda,
 // slope of the segment ending at the row), the form a kernel stages in LDS.
 struct TabRow {
@@ -537,8 +534,10 @@ __device__ __forceinline__ CoolF32 cool_consts(const Problem &P) {
                    P.f_floor_f, P.f_lgT0,    P.f_inv_dlg, P.ntable, P.tab_uniform};
 }
 
-// apply_cooling in FP32 with the table rows in LDS (see cool_fast); *ce_dt
-// (when given) receives dt / t_cool.
+// apply_cooling for one cell in FP32 (cooling.f90:67-112) with the table
+// rows in LDS: the row is found by direct indexing when the table is equally
+// spaced in log10 T (Z1.0.dat is: 0.01 dex), else by bisection; 10^x and exp
+// via exp2/log2.  *ce_dt (when given) receives dt / t_cool.
 __device__ __forceinline__ float cool_fast_tab(const CoolF32 &C, const TabRow *tab, float rho,
                                                float p, float dt, bool *cooled,
                                                float *ce_dt = nullptr) {
@@ -589,47 +588,6 @@ __device__ __forceinline__ float cool_fast_tab(const CoolF32 &C, const TabRow *t
     if (ce_dt) *ce_dt = ce * dt;
     float f = __builtin_amdgcn_exp2f(-ce * dt * 1.44269504088896f);  // exp(-ce dt)
     if (f < C.floor_f) f = C.floor_f;
-    if (temp * f < 10.f) f = 10.f * __builtin_amdgcn_rcpf(temp);
-    return p * f;
-}
-
-__device__ __forceinline__ float cool_fast(const Problem &P, float rho, float p, float dt,
-                                           bool *cooled) {
-    const float rinv = __builtin_amdgcn_rcpf(rho);
-    const float temp = p * P.f_cT * rinv;
-    if (!(temp >= P.f_tmin)) {
-        *cooled = false;
-        return p;
-    }
-    *cooled = true;
-    const float lg = __builtin_amdgcn_logf(temp) * 0.301029995663981f;  // log10 T
-    float lgl;
-    if (!(temp < P.f_tmax)) {
-        lgl = P.tab_Lf[P.ntable - 1];
-    } else {
-        int hi;
-        if (P.tab_uniform) {
-            hi = (int)((lg - P.f_lgT0) * P.f_inv_dlg) + 1;
-            hi = hi < 1 ? 1 : (hi > P.ntable - 1 ? P.ntable - 1 : hi);
-            if (!(P.tab_Tf[hi] > temp) && hi < P.ntable - 1) ++hi;
-            if (hi > 1 && P.tab_Tf[hi - 1] > temp) --hi;
-        } else {
-            int lo = 0;
-            hi = P.ntable - 1;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (P.tab_Tf[mid] > temp) hi = mid;
-                else lo = mid;
-            }
-        }
-        lgl = P.tab_Lf[hi - 1] + P.tab_slopef[hi] * (lg - P.tab_lgTf[hi - 1]);
-    }
-    const float lam = __builtin_amdgcn_exp2f(lgl * 3.32192809488736f);  // 10^lgl
-    const float n = rho * P.f_inv_mu_amu;
-    const float eth = p * P.f_inv_gm1;
-    const float ce = lam * n * n * __builtin_amdgcn_rcpf(eth);
-    float f = __builtin_amdgcn_exp2f(-ce * dt * 1.44269504088896f);  // exp(-ce dt)
-    if (f < P.f_floor_f) f = P.f_floor_f;
     if (temp * f < 10.f) f = 10.f * __builtin_amdgcn_rcpf(temp);
     return p * f;
 }
