@@ -99,3 +99,47 @@ def test_integrated_cooling_matches_oracle(precision):
     else:
         assert l1_rel(w[..., 0], wr[..., 0]) <= 1e-3
         assert l1_rel(w[..., 4], wr[..., 4]) <= 1e-3
+
+
+def _refined_table(temps, logl):
+    """The table's rows plus one more inside every segment, at a fraction of
+    the segment's log T that varies from 0.25 to 0.75, on the segment's power
+    law: the same Lambda(T) as the table, in 1001 rows whose log T spacing is
+    not uniform (the rows are then found by bisection, not direct indexing)."""
+    lt, ll = np.log10(np.asarray(temps)), np.asarray(logl)
+    rows_t, rows_l = [lt[0]], [ll[0]]
+    for i in range(1, len(lt)):
+        f = 0.25 + 0.05 * ((7 * i) % 11)
+        rows_t += [lt[i - 1] + f * (lt[i] - lt[i - 1]), lt[i]]
+        rows_l += [ll[i - 1] + f * (ll[i] - ll[i - 1]), ll[i]]
+    return list(10.0 ** np.array(rows_t)), rows_l
+
+
+@pytest.mark.parametrize("mode", [P.COOL_SUBCYCLED, P.COOL_INTEGRATED])
+def test_dense_pass_large_nonuniform_table(mode):
+    """The FP32 dense pass stages the table in LDS (32 B per row).  With the
+    1001-row refinement of Z1.0 (31 KB of LDS, rows found by bisection) the
+    cooled states equal those with Z1.0 itself (501 rows, direct indexing) up
+    to the FP32 rounding of the inserted rows' log Lambda: measured L1 of rho
+    and p 2.7e-9 and 2.8e-8 (mode 3), 2.5e-8 and 1.7e-7 (mode 4) after three
+    steps; the bounds are about ten times those."""
+    p, u0, _, _ = load_ref("cooling")
+    p = p.replace(cooling=mode, precision=P.PREC_FP32)
+    base = cooling_table()
+    fine = _refined_table(*base)
+    assert len(fine[0]) == 1001
+    out = []
+    for table in (base, fine):
+        with HydroCore(p, cooling_table=table) as h:
+            h.set_state(u0)
+            h.advance(3)
+            queued, _ = h.cooling_counts()
+            out.append(interior(prims_from_cons(h.download("U"), p.gamma)))
+            assert h.check_state() == 0
+        assert queued > 50                      # the dense pass ran
+    a, b = out
+    l1r, l1p = l1_rel(b[..., 0], a[..., 0]), l1_rel(b[..., 4], a[..., 4])
+    print(f"mode {mode}: 1001-row table vs 501 rows: L1 rho {l1r:.2e} p {l1p:.2e}, "
+          f"max rel {max_rel(b, a):.2e}")
+    bound_r, bound_p = {P.COOL_SUBCYCLED: (3e-8, 3e-7), P.COOL_INTEGRATED: (3e-7, 2e-6)}[mode]
+    assert l1r <= bound_r and l1p <= bound_p
