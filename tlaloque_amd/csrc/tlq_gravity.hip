@@ -233,16 +233,19 @@ __global__ __launch_bounds__(ZX *ZY) void k_sg_rbz(
                 const int lx = q % NX - 1, ly = q / NX - 1;
                 const int i = x0 + 1 + lx, j = y0 + 1 + ly;
                 const int a = lx + 2, b = ly + 2;
-                double v = oc[b][a];
-                if (i >= 1 && i <= nx && j >= 1 && j <= ny && kr >= 1 && kr <= nz &&
-                    ((i + j + kr) & 1) == 0) {
-                    const double old = v;
-                    v = ((1.0 / 6.0) * (oc[b][a + 1] + oc[b][a - 1] + oc[b + 1][a] + oc[b - 1][a] +
-                                        op[b][a] + om[b][a] - (s ? sr1 : sr0) * h2)) * 1.0 +
-                        (1.0 - 1.0) * old;
-                    if (lx >= 0 && lx < ZX && ly >= 0 && ly < ZY && kr >= k0 && kr <= k1)
-                        dsum += fabs(v - old);
-                }
+                // every lane forms the update and keeps it on its colour's cells
+                // (a select instead of a branch: fewer exec-mask instructions on
+                // the scalar unit, the kernel's busiest)
+                const double old = oc[b][a];
+                const double nv =
+                    ((1.0 / 6.0) * (oc[b][a + 1] + oc[b][a - 1] + oc[b + 1][a] + oc[b - 1][a] +
+                                    op[b][a] + om[b][a] - (s ? sr1 : sr0) * h2)) * 1.0 +
+                    (1.0 - 1.0) * old;
+                const bool upd = i >= 1 && i <= nx && j >= 1 && j <= ny && kr >= 1 && kr <= nz &&
+                                 ((i + j + kr) & 1) == 0;
+                const double v = upd ? nv : old;
+                const bool in_tile = lx >= 0 && lx < ZX && ly >= 0 && ly < ZY && kr >= k0 && kr <= k1;
+                dsum += upd && in_tile ? fabs(v - old) : 0.0;
                 nn[ly + 1][lx + 1] = v;
             }
         }
@@ -254,14 +257,15 @@ __global__ __launch_bounds__(ZX *ZY) void k_sg_rbz(
             if (i <= nx && j <= ny) {
                 double(*nc)[NX] = N[mod3(t)];
                 const int a = lx + 1, b = ly + 1;
-                double v = nc[b][a];
-                if (((i + j + t) & 1) == 1) {
-                    const double old = O[t & 3][ly + 2][lx + 2];
-                    v = ((1.0 / 6.0) * (nc[b][a + 1] + nc[b][a - 1] + nc[b + 1][a] + nc[b - 1][a] +
-                                        N[mod3(t + 1)][b][a] + N[mod3(t - 1)][b][a] - sbc * h2)) * 1.0 +
-                        (1.0 - 1.0) * old;
-                    dsum += fabs(v - old);
-                }
+                const double cur = nc[b][a];
+                const double old = O[t & 3][ly + 2][lx + 2];
+                const double nv =
+                    ((1.0 / 6.0) * (nc[b][a + 1] + nc[b][a - 1] + nc[b + 1][a] + nc[b - 1][a] +
+                                    N[mod3(t + 1)][b][a] + N[mod3(t - 1)][b][a] - sbc * h2)) * 1.0 +
+                    (1.0 - 1.0) * old;
+                const bool upd = ((i + j + t) & 1) == 1;
+                const double v = upd ? nv : cur;
+                dsum += upd ? fabs(v - old) : 0.0;
                 eout[sidx(nx, ny, i, j, t)] = v;
             }
         }
