@@ -212,6 +212,8 @@ __global__ __launch_bounds__(ZX *ZY) void k_sg_rbz(
     fetch_src_red(k0 - 1, sr);
     sb = fetch_src_blk(k0 - 2);
     double dsum = 0.0;
+    // N's buffers of planes t-1, t, t+1 (mod 3), rotated along the march
+    int nm = mod3(k0 - 3), n0 = mod3(k0 - 2), np = mod3(k0 - 1);
     for (int t = k0 - 2; t <= k1; ++t) {
         if (t + 2 <= nz + 1) store_plane(t + 2, pf);
         const double sr0 = sr[0], sr1 = sr[1], sbc = sb;
@@ -225,7 +227,7 @@ __global__ __launch_bounds__(ZX *ZY) void k_sg_rbz(
             double(*om)[OX] = O[(kr - 1) & 3];
             double(*oc)[OX] = O[kr & 3];
             double(*op)[OX] = O[(kr + 1) & 3];
-            double(*nn)[NX] = N[mod3(kr)];
+            double(*nn)[NX] = N[np];
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
                 const int q = tid + s * T;
@@ -255,19 +257,25 @@ __global__ __launch_bounds__(ZX *ZY) void k_sg_rbz(
             const int lx = tid % ZX, ly = tid / ZX;
             const int i = x0 + 1 + lx, j = y0 + 1 + ly;
             if (i <= nx && j <= ny) {
-                double(*nc)[NX] = N[mod3(t)];
+                double(*nc)[NX] = N[n0];
                 const int a = lx + 1, b = ly + 1;
                 const double cur = nc[b][a];
                 const double old = O[t & 3][ly + 2][lx + 2];
                 const double nv =
                     ((1.0 / 6.0) * (nc[b][a + 1] + nc[b][a - 1] + nc[b + 1][a] + nc[b - 1][a] +
-                                    N[mod3(t + 1)][b][a] + N[mod3(t - 1)][b][a] - sbc * h2)) * 1.0 +
+                                    N[np][b][a] + N[nm][b][a] - sbc * h2)) * 1.0 +
                     (1.0 - 1.0) * old;
                 const bool upd = ((i + j + t) & 1) == 1;
                 const double v = upd ? nv : cur;
                 dsum += upd ? fabs(v - old) : 0.0;
                 eout[sidx(nx, ny, i, j, t)] = v;
             }
+        }
+        {
+            const int r = nm;
+            nm = n0;
+            n0 = np;
+            np = r;
         }
         // No barrier before the next step: its store of old plane t+3 goes to
         // the buffer of plane t-1, which no thread reads after the first
