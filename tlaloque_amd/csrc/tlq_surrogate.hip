@@ -464,11 +464,15 @@ __global__ __launch_bounds__(64) void k_mlp_eval(const unsigned short *__restric
 // Then the CFL maxima of the cells finished here.
 
 // Exclusive prefix sums of the per-tile queue counts: off[b] = cnt[0] + ... +
-// cnt[b-1], off[n] = the total; also zeroes the fallback counter.  One block.
-__global__ __launch_bounds__(1024) void k_queue_scan(const unsigned *__restrict__ cnt,
-                                                     unsigned *__restrict__ off, int n,
+// cnt[b-1], off[n] = the total.  Block q scans the q-th list of n counts into
+// the q-th list of n+1 offsets (the network's and the integrator's queues, in
+// one launch); block 0 also zeroes the fallback counter.
+__global__ __launch_bounds__(1024) void k_queue_scan(const unsigned *__restrict__ cnts,
+                                                     unsigned *__restrict__ offs, int n,
                                                      unsigned *__restrict__ fcount) {
     __shared__ unsigned part[1024];
+    const unsigned *const cnt = cnts + (long long)blockIdx.x * n;
+    unsigned *const off = offs + (long long)blockIdx.x * (n + 1);
     const int t = threadIdx.x, per = (n + 1023) / 1024;
     const int b0 = min(t * per, n), b1 = min(b0 + per, n);
     unsigned sum = 0;
@@ -487,7 +491,7 @@ __global__ __launch_bounds__(1024) void k_queue_scan(const unsigned *__restrict_
         run += cnt[b];
     }
     if (t == 1023) off[n] = part[1023];
-    if (t == 0) *fcount = 0;
+    if (t == 0 && blockIdx.x == 0) *fcount = 0;
 }
 
 // Segment b of the queue holding position p: off[b] <= p < off[b+1] (off
@@ -547,16 +551,19 @@ __global__ __launch_bounds__(128) void k_cool_dense(const Problem *__restrict__ 
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     TabRow *const trow = reinterpret_cast<TabRow *>(s_tab);
     YRow *const ty = with_y ? reinterpret_cast<YRow *>(s_tab + P.ntable * sizeof(TabRow)) : nullptr;
-    if (sizeof(R) == 4 || MODE == 2) stage_cool_tab(P, trow, ty, threadIdx.x, blockDim.x);  // visible after the barrier below
-    const CoolTab tab = cool_tab(P, trow, ty);
     const unsigned nseg_cells = MODE == 2 ? offn[nseg] : offi[nseg];
     const unsigned nq = MODE == 2 ? nseg_cells : nseg_cells + *fcount;
+    // The grid is sized for the largest queue: a block past this one's end has
+    // no cell (its first wavefront's first position is blockIdx.x * 128).
+    if (blockIdx.x * 128u >= nq) return;
+    if (sizeof(R) == 4 || MODE == 2) stage_cool_tab(P, trow, ty, threadIdx.x, blockDim.x);  // visible after the barrier below
+    const CoolTab tab = cool_tab(P, trow, ty);
     const double dt = dt_from_maxima(P, st->maxima[parity]);
     unsigned nfb = 0;  // this wavefront's buffered fallbacks
     bf16x4 w2a[4][4];
     float b3 = 0.f;
     // T_min / Lambda(T_min): k_B T_min / ((gamma-1) n Lambda) is the cooling time at the gate
-    if (MODE == 2 && nq > 0) {
+    if (MODE == 2) {
         load_small_f(sm, thq, threadIdx.x, blockDim.x);
         b3 = bf2f(thq[OB3]);
         load_w2_fwd(thq, lane, w2a);
@@ -654,7 +661,7 @@ __global__ __launch_bounds__(128) void k_cool_dense(const Problem *__restrict__ 
         }
     }
     if (MODE == 2 && nfb) flush_fallbacks(s_fbuf[wv], nfb, lane, fqueue, fcount);
-    if (nq > 0) block_max3<128>(mx, my, mz, st->maxima[parity ^ 1]);
+    block_max3<128>(mx, my, mz, st->maxima[parity ^ 1]);
 }
 
 // ---------------------------------------------------------------------------
@@ -708,8 +715,7 @@ hipError_t sur_cool_pass(hipStream_t s, const Problem *P, int nseg, long long C,
     const int y3 = mode != 3;
     const unsigned lds2 = cool_tab_bytes(ntable, true);
     const unsigned lds3 = sizeof(R) == 4 ? cool_tab_bytes(ntable, y3) : 0u;
-    hipLaunchKernelGGL(k_queue_scan, dim3(1), dim3(1024), 0, s, seg, offn, nseg, fcount);
-    hipLaunchKernelGGL(k_queue_scan, dim3(1), dim3(1024), 0, s, seg + nseg, offi, nseg, fcount);
+    hipLaunchKernelGGL(k_queue_scan, dim3(2), dim3(1024), 0, s, seg, offn, nseg, fcount);
     if (mode == 2)
         hipLaunchKernelGGL((k_cool_dense<R, 2>), dim3(nb), dim3(128), lds2, s, P, U, thq, st,
                            parity, queue, offn, offi, nseg, C, fqueue, fcount, 1);
