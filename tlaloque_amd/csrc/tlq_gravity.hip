@@ -167,6 +167,23 @@ __global__ __launch_bounds__(ZX *ZY) void k_sg_rbz(
         if constexpr (FROM_U) return src0<R>(U, g, fpg, i, j, k);
         else return rs[sidx(nx, ny, i, j, k)];
     };
+    // Per-lane bounds of this thread's cells, fixed along the march, and per
+    // wavefront: when every lane of a wavefront is in bounds, its loads run
+    // unmasked behind a wavefront-uniform branch.
+    bool inO[2], inR[2], allO[2], allR[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int q = tid + s * T;
+        const int i = x0 + 1 + q % OX - 2, j = y0 + 1 + q / OX - 2;
+        inO[s] = q < OX * OY && i >= 0 && i <= nx + 1 && j >= 0 && j <= ny + 1;
+        const int ir = x0 + q % NX, jr = y0 + q / NX;
+        inR[s] = q < NX * NY && ir >= 1 && ir <= nx && jr >= 1 && jr <= ny;
+        allO[s] = __ballot(inO[s]) == ~0ull;
+        allR[s] = __ballot(inR[s]) == ~0ull;
+    }
+    const int ib = x0 + 1 + tid % ZX, jb = y0 + 1 + tid / ZX;
+    const bool inB = ib <= nx && jb <= ny;
+    const bool allB = __ballot(inB) == ~0ull;
     // old plane k (0 <= k <= nz+1), this thread's two ring-2 cells
     auto fetch_plane = [&](int k, double (&r)[2]) {
 #pragma unroll
@@ -174,8 +191,8 @@ __global__ __launch_bounds__(ZX *ZY) void k_sg_rbz(
             const int q = tid + s * T;
             const int lx = q % OX - 2, ly = q / OX - 2;
             const int i = x0 + 1 + lx, j = y0 + 1 + ly;
-            r[s] = (q < OX * OY && i >= 0 && i <= nx + 1 && j >= 0 && j <= ny + 1)
-                       ? ein[sidx(nx, ny, i, j, k)] : 0.0;
+            if (allO[s]) r[s] = ein[sidx(nx, ny, i, j, k)];
+            else r[s] = inO[s] ? ein[sidx(nx, ny, i, j, k)] : 0.0;
         }
     };
     auto store_plane = [&](int k, const double (&r)[2]) {
@@ -187,19 +204,29 @@ __global__ __launch_bounds__(ZX *ZY) void k_sg_rbz(
         }
     };
     // sources of this thread's two ring-1 cells on plane kr (colour 0, interior)
+    // (the lanes of the other colour load too, behind the uniform branch: the
+    // same cache lines, and the value is dropped)
     auto fetch_src_red = [&](int kr, double (&r)[2]) {
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             const int q = tid + s * T;
             const int i = x0 + q % NX, j = y0 + q / NX;
-            r[s] = (q < NX * NY && i >= 1 && i <= nx && j >= 1 && j <= ny && kr >= 1 && kr <= nz &&
-                    ((i + j + kr) & 1) == 0) ? src(i, j, kr) : 0.0;
+            double v = 0.0;
+            if (kr >= 1 && kr <= nz) {
+                if (allR[s]) v = src(i, j, kr);
+                else if (inR[s]) v = src(i, j, kr);
+            }
+            r[s] = ((i + j + kr) & 1) == 0 ? v : 0.0;
         }
     };
     // source of this thread's tile cell on plane t (colour 1, interior)
     auto fetch_src_blk = [&](int t) -> double {
-        const int i = x0 + 1 + tid % ZX, j = y0 + 1 + tid / ZX;
-        return (i <= nx && j <= ny && t >= 1 && t <= nz && ((i + j + t) & 1) == 1) ? src(i, j, t) : 0.0;
+        double v = 0.0;
+        if (t >= 1 && t <= nz) {
+            if (allB) v = src(ib, jb, t);
+            else if (inB) v = src(ib, jb, t);
+        }
+        return ((ib + jb + t) & 1) == 1 ? v : 0.0;
     };
     double pf[2], sr[2], sb;
     if (k0 - 2 >= 0) {
