@@ -13,7 +13,9 @@ so the comparisons are tight:
     routine (a solve in every Godunov call, as the reference): UPrim after 1
     and 4 steps within 1e-9 of the field's scale, PHIT within 1e-10;
   * one solve against the oracle's on a Plummer sphere, FP64 and FP32 (FP32
-    rounds the density the source reads);
+    rounds the density the source reads), with the finest level on either
+    relaxation kernel (the one-pass iteration or two half sweeps), and the
+    two kernels bitwise equal on a 64^3 solve and three steps after it;
   * multipole boundary values (the MPI build) against the oracle's, and
     their refresh when mod(it, 10) = 0 only;
   * an FP32 run with self-gravity and a point mass against the FP64 oracle,
@@ -131,8 +133,25 @@ def _plummer(n, prec, mode=P.SG_FIXED):
     return p, u0, phi0
 
 
+# The relaxation kernels: by default levels of 4M cells and more take the
+# one-pass red-black iteration and smaller ones two in-place half sweeps, so
+# the meshes here would run the half sweeps only.  TLQ_MG_ONEPASS_MIN_CELLS
+# moves the threshold when a context is created: 32768 puts these 32^3 and
+# 64^3 levels on the one-pass kernel.
+ONEPASS = [None, "32768"]
+
+
+def _onepass(monkeypatch, value):
+    if value is None:
+        monkeypatch.delenv("TLQ_MG_ONEPASS_MIN_CELLS", raising=False)
+    else:
+        monkeypatch.setenv("TLQ_MG_ONEPASS_MIN_CELLS", value)
+
+
+@pytest.mark.parametrize("onepass", ONEPASS)
 @pytest.mark.parametrize("prec,tol", [(P.PREC_FP64, 1e-10), (P.PREC_FP32, 1e-6)])
-def test_solve_matches_oracle(prec, tol):
+def test_solve_matches_oracle(prec, tol, onepass, monkeypatch):
+    _onepass(monkeypatch, onepass)
     p, u0, phi0 = _plummer(32, prec)
     o = oracle.Oracle(p.replace(precision=P.PREC_FP64))
     o.set_phi(phi0)
@@ -149,6 +168,31 @@ def test_solve_matches_oracle(prec, tol):
     assert lv == 5 and 1 <= cyc <= 31
     assert err <= tol
     assert again[0] <= 2        # from a converged potential one or two cycles suffice
+
+
+@pytest.mark.parametrize("prec", [P.PREC_FP64, P.PREC_FP32])
+def test_one_pass_iteration_equals_half_sweeps(prec, monkeypatch):
+    """The one-pass red-black iteration and the two in-place half sweeps form
+    the same updates in the same order: a Plummer solve at 64^3 and three fused
+    steps after it give bitwise equal potentials and states with the 64^3 and
+    32^3 levels on either kernel."""
+    p, u0, phi0 = _plummer(64, prec)
+    out = {}
+    for value in ONEPASS:
+        _onepass(monkeypatch, value)
+        with _hc()(p) as h:
+            h.upload_phi(phi0)
+            h.set_state(u0)
+            phi = h.download_phi()
+            cyc = h.gravity_info()[1]
+            h.advance(3)
+            out[value] = (phi, cyc, h.download("U"), h.download_phi())
+    a, b = out[None], out["32768"]
+    print(prec, "cycles", a[1], b[1])
+    assert a[1] == b[1] and 1 <= a[1] <= 31
+    assert np.array_equal(a[0], b[0])
+    assert np.array_equal(a[2], b[2]) and np.isfinite(a[2]).all()
+    assert np.array_equal(a[3], b[3])
 
 
 def test_multipole_faces_match_oracle_and_refresh_every_ten_steps():

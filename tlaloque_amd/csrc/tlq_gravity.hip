@@ -4,6 +4,7 @@
 #include "tlq_gravity.hpp"
 
 #include <cmath>
+#include <cstdlib>
 
 namespace tlq {
 
@@ -140,7 +141,7 @@ __global__ __launch_bounds__(RB_TX *RB_TY) void k_sg_relax(
 // Block ids are remapped so that each of the 8 XCDs (blocks go to them round
 // robin) takes a contiguous range of tiles: tiles that share ring cells then
 // share one L2.
-constexpr int ZX = 32, ZY = 16, ZZ = 32;
+constexpr int ZX = 32, ZY = 8, ZZ = 64;
 
 __device__ __forceinline__ int mod3(int t) { return ((t % 3) + 3) % 3; }
 
@@ -573,9 +574,27 @@ long long relax_blocks(const SgLevel &L) {
 
 constexpr int MOM_BLOCKS = 1024;
 constexpr int SUM_BLOCKS = 256;
-constexpr long long RBZ_MIN_CELLS = 32768;   // levels from 32^3 up take the one-pass iteration
+// Levels of at least RBZ_MIN_CELLS cells take the one-pass iteration, smaller
+// ones the two half sweeps (the same potential bit for bit).  A one-pass march
+// is latency-bound on a level with few tiles (a 128^3 level has 128), where
+// the half sweeps, with many more blocks, are faster.  The environment
+// variable TLQ_MG_ONEPASS_MIN_CELLS, read when a solver is set up, overrides
+// the threshold.
+constexpr long long RBZ_MIN_CELLS = 4194304;
 
-bool use_rbz(const SgLevel &L) { return (long long)L.nx * L.ny * L.nz >= RBZ_MIN_CELLS; }
+long long onepass_min_cells() {
+    const char *e = std::getenv("TLQ_MG_ONEPASS_MIN_CELLS");
+    if (e && *e) {
+        char *end = nullptr;
+        const long long v = std::strtoll(e, &end, 10);
+        if (end && *end == 0 && v > 0) return v;
+    }
+    return RBZ_MIN_CELLS;
+}
+
+bool use_rbz(const SgState &S, const SgLevel &L) {
+    return (long long)L.nx * L.ny * L.nz >= S.onepass_min;
+}
 
 dim3 rbz_grid(const SgLevel &L) {
     return dim3(nblocks(L.nx, ZX), nblocks(L.ny, ZY), nblocks(L.nz, ZZ));
@@ -600,6 +619,7 @@ long long sg_coarse_cells(int nx, int ny, int nz, int levels) {
 
 hipError_t sg_alloc(SgState &S, int nx, int ny, int nz, int levels, double xspan, hipStream_t stream) {
     S.levels = levels;
+    S.onepass_min = onepass_min_cells();
     for (int l = 0; l <= levels; ++l) {
         SgLevel &L = S.lv[l];
         L.nx = nx >> l;
@@ -611,7 +631,7 @@ hipError_t sg_alloc(SgState &S, int nx, int ny, int nz, int levels, double xspan
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_sg_fill, dim3(std::min<long long>(nblocks(n, NT), 65535)), dim3(NT), 0,
                            stream, L.e, (long long)n, 0.0);
-        if (use_rbz(L)) {
+        if (use_rbz(S, L)) {
             e = hipMalloc(&L.e2, n * sizeof(double));
             if (e != hipSuccess) return e;
             hipLaunchKernelGGL(k_sg_fill, dim3(std::min<long long>(nblocks(n, NT), 65535)), dim3(NT),
@@ -627,7 +647,7 @@ hipError_t sg_alloc(SgState &S, int nx, int ny, int nz, int levels, double xspan
     S.phi0 = S.lv[0].e;
     long long np = (long long)MOM_BLOCKS * SG_NMOM;
     for (int l = 0; l <= levels; ++l)
-        np = std::max(np, use_rbz(S.lv[l]) ? rbz_blocks(S.lv[l]) : 2 * relax_blocks(S.lv[l]));
+        np = std::max(np, use_rbz(S, S.lv[l]) ? rbz_blocks(S.lv[l]) : 2 * relax_blocks(S.lv[l]));
     S.npartial = np;
     hipError_t e = hipMalloc(&S.partial, (size_t)S.npartial * sizeof(double));
     if (e != hipSuccess) return e;
@@ -670,7 +690,7 @@ void sum_partials(hipStream_t s, SgState &S, const double *partial, long long n,
 template <class R>
 void relax(hipStream_t s, const Grid &g, const R *U, double fpg, SgState &S, int l, bool want_dif) {
     SgLevel &L = S.lv[l];
-    if (use_rbz(L)) {
+    if (use_rbz(S, L)) {
         const dim3 gd = rbz_grid(L);
         for (int it = 1; it <= 5; ++it) {
             double *part = (want_dif && it == 5) ? S.partial : nullptr;

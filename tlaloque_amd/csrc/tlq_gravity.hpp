@@ -46,6 +46,9 @@ struct SgState {
     int boundary = 0;       // 1: faces kept as set; 2: multipole every 10 steps
     double gconst = 0.0;
     double tol = 10.0;           // V-cycles while Dif > tol (sources.f90:140 uses 10)
+    // Levels of at least this many cells relax with the one-pass iteration,
+    // smaller ones with two in-place half sweeps (set by sg_alloc).
+    long long onepass_min = 0;
     SgLevel lv[SG_MAX_LEVELS + 1]{};
     double *partial = nullptr;   // per-block partial sums
     long long npartial = 0;
