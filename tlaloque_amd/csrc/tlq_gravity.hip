@@ -133,9 +133,9 @@ __global__ __launch_bounds__(RB_TX *RB_TY) void k_sg_relax(
 // values -- the same updates, in the same order of terms, as the two half
 // sweeps of k_sg_relax, at one read and one write of E per iteration.
 // A block owns a ZX x ZY column tile and marches planes k0..k1; at step t it
-// computes colour 0 of plane t+1 on the tile and a ring of 1 (the ring values
-// are recomputed by the neighbour tiles, identically), then colour 1 of plane
-// t on the tile, and writes plane t.  The old plane t+3 (ring 2 around the
+// computes colour 0 of plane t+1 on the tile and a ring of 1, one colour-0
+// cell per thread (the ring values are recomputed by the neighbour tiles,
+// identically), then colour 1 of plane t on the tile, and writes plane t.  The old plane t+3 (ring 2 around the
 // tile) and the sources of the next step are loaded into registers during
 // step t, and the barriers wait on LDS only, so the loads stay in flight.
 // Block ids are remapped so that each of the 8 XCDs (blocks go to them round
@@ -171,17 +171,31 @@ __global__ __launch_bounds__(ZX *ZY) void k_sg_rbz(
     // Per-lane bounds of this thread's cells, fixed along the march, and per
     // wavefront: when every lane of a wavefront is in bounds, its loads run
     // unmasked behind a wavefront-uniform branch.
-    bool inO[2], inR[2], allO[2], allR[2];
+    bool inO[2], allO[2];
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
         const int q = tid + s * T;
         const int i = x0 + 1 + q % OX - 2, j = y0 + 1 + q / OX - 2;
         inO[s] = q < OX * OY && i >= 0 && i <= nx + 1 && j >= 0 && j <= ny + 1;
-        const int ir = x0 + q % NX, jr = y0 + q / NX;
-        inR[s] = q < NX * NY && ir >= 1 && ir <= nx && jr >= 1 && jr <= ny;
         allO[s] = __ballot(inO[s]) == ~0ull;
-        allR[s] = __ballot(inR[s]) == ~0ull;
     }
+    // Colour 0 of a plane on the tile and its ring of 1: the NC0 colour-0
+    // cells of the NX x NY region, one per thread (tid < NC0).  Thread tid
+    // takes row ly = tid / (NX/2) - 1 and the p-th colour-0 cell of that row,
+    // p = tid % (NX/2): lx = 2p - 1 when ly + kr is odd, 2p when it is even
+    // (x0 and y0 are even, so the colour of (lx, ly) on plane kr is that of
+    // lx + ly + kr).
+    constexpr int HX = NX / 2, NC0 = HX * NY;
+    static_assert(NX % 2 == 0 && ZX % 2 == 0 && ZY % 2 == 0 && NC0 <= T, "one colour-0 cell per thread");
+    const bool hasR = tid < NC0;
+    const int lyr = tid / HX - 1, lxo = 2 * (tid % HX) - 1;  // lx for ly + kr odd; lxo + 1 for even
+    auto red_in = [&](int lx) {
+        const int i = x0 + 1 + lx, j = y0 + 1 + lyr;
+        return hasR && i >= 1 && i <= nx && j >= 1 && j <= ny;
+    };
+    const bool inRo = red_in(lxo), inRe = red_in(lxo + 1);
+    const bool allR = __ballot(inRo && inRe) == ~0ull, anyR = __ballot(hasR) != 0ull;
+    auto red_lx = [&](int kr) { return lxo + ((lyr + kr + 1) & 1); };
     const int ib = x0 + 1 + tid % ZX, jb = y0 + 1 + tid / ZX;
     const bool inB = ib <= nx && jb <= ny;
     const bool allB = __ballot(inB) == ~0ull;
@@ -204,21 +218,16 @@ __global__ __launch_bounds__(ZX *ZY) void k_sg_rbz(
             if (q < OX * OY) o[q / OX][q % OX] = r[s];
         }
     };
-    // sources of this thread's two ring-1 cells on plane kr (colour 0, interior)
-    // (the lanes of the other colour load too, behind the uniform branch: the
-    // same cache lines, and the value is dropped)
-    auto fetch_src_red = [&](int kr, double (&r)[2]) {
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            const int q = tid + s * T;
-            const int i = x0 + q % NX, j = y0 + q / NX;
-            double v = 0.0;
-            if (kr >= 1 && kr <= nz) {
-                if (allR[s]) v = src(i, j, kr);
-                else if (inR[s]) v = src(i, j, kr);
-            }
-            r[s] = ((i + j + kr) & 1) == 0 ? v : 0.0;
+    // source of this thread's colour-0 cell on plane kr (0 outside the interior)
+    auto fetch_src_red = [&](int kr) -> double {
+        const int lx = red_lx(kr);
+        const int i = x0 + 1 + lx, j = y0 + 1 + lyr;
+        double v = 0.0;
+        if (kr >= 1 && kr <= nz) {
+            if (allR) v = src(i, j, kr);
+            else if (anyR && ((lx == lxo) ? inRo : inRe)) v = src(i, j, kr);
         }
+        return v;
     };
     // source of this thread's tile cell on plane t (colour 1, interior)
     auto fetch_src_blk = [&](int t) -> double {
@@ -229,7 +238,7 @@ __global__ __launch_bounds__(ZX *ZY) void k_sg_rbz(
         }
         return ((ib + jb + t) & 1) == 1 ? v : 0.0;
     };
-    double pf[2], sr[2], sb;
+    double pf[2], sr, sb;
     if (k0 - 2 >= 0) {
         fetch_plane(k0 - 2, pf);
         store_plane(k0 - 2, pf);
@@ -237,16 +246,16 @@ __global__ __launch_bounds__(ZX *ZY) void k_sg_rbz(
     fetch_plane(k0 - 1, pf);
     store_plane(k0 - 1, pf);
     fetch_plane(k0, pf);        // plane t+2 of the first step (k0 <= nz)
-    fetch_src_red(k0 - 1, sr);
+    sr = fetch_src_red(k0 - 1);
     sb = fetch_src_blk(k0 - 2);
     double dsum = 0.0;
     // N's buffers of planes t-1, t, t+1 (mod 3), rotated along the march
     int nm = mod3(k0 - 3), n0 = mod3(k0 - 2), np = mod3(k0 - 1);
     for (int t = k0 - 2; t <= k1; ++t) {
         if (t + 2 <= nz + 1) store_plane(t + 2, pf);
-        const double sr0 = sr[0], sr1 = sr[1], sbc = sb;
+        const double src_r = sr, sbc = sb;
         if (t + 3 <= nz + 1) fetch_plane(t + 3, pf);
-        fetch_src_red(t + 2, sr);
+        sr = fetch_src_red(t + 2);
         sb = fetch_src_blk(t + 1);
         lds_barrier();
         // colour 0 of plane kr = t+1 on the tile and its ring of 1
@@ -256,23 +265,17 @@ __global__ __launch_bounds__(ZX *ZY) void k_sg_rbz(
             double(*oc)[OX] = O[kr & 3];
             double(*op)[OX] = O[(kr + 1) & 3];
             double(*nn)[NX] = N[np];
-#pragma unroll
-            for (int s = 0; s < 2; ++s) {
-                const int q = tid + s * T;
-                if (q >= NX * NY) continue;
-                const int lx = q % NX - 1, ly = q / NX - 1;
+            if (hasR) {
+                const int lx = red_lx(kr), ly = lyr;
                 const int i = x0 + 1 + lx, j = y0 + 1 + ly;
                 const int a = lx + 2, b = ly + 2;
-                // every lane forms the update and keeps it on its colour's cells
-                // (a select instead of a branch: fewer exec-mask instructions on
-                // the scalar unit, the kernel's busiest)
                 const double old = oc[b][a];
                 const double nv =
                     ((1.0 / 6.0) * (oc[b][a + 1] + oc[b][a - 1] + oc[b + 1][a] + oc[b - 1][a] +
-                                    op[b][a] + om[b][a] - (s ? sr1 : sr0) * h2)) * 1.0 +
+                                    op[b][a] + om[b][a] - src_r * h2)) * 1.0 +
                     (1.0 - 1.0) * old;
-                const bool upd = i >= 1 && i <= nx && j >= 1 && j <= ny && kr >= 1 && kr <= nz &&
-                                 ((i + j + kr) & 1) == 0;
+                // outside the interior (ghost cells) the old value stands
+                const bool upd = i >= 1 && i <= nx && j >= 1 && j <= ny && kr >= 1 && kr <= nz;
                 const double v = upd ? nv : old;
                 const bool in_tile = lx >= 0 && lx < ZX && ly >= 0 && ly < ZY && kr >= k0 && kr <= k1;
                 dsum += upd && in_tile ? fabs(v - old) : 0.0;
@@ -293,6 +296,8 @@ __global__ __launch_bounds__(ZX *ZY) void k_sg_rbz(
                     ((1.0 / 6.0) * (nc[b][a + 1] + nc[b][a - 1] + nc[b + 1][a] + nc[b - 1][a] +
                                     N[np][b][a] + N[nm][b][a] - sbc * h2)) * 1.0 +
                     (1.0 - 1.0) * old;
+                // every lane forms the update; a colour-0 cell keeps its new
+                // value from N (a select: no exec-mask change on the scalar unit)
                 const bool upd = ((i + j + t) & 1) == 1;
                 const double v = upd ? nv : cur;
                 dsum += upd ? fabs(v - old) : 0.0;
