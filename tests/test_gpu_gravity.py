@@ -170,13 +170,16 @@ def test_solve_matches_oracle(prec, tol, onepass, monkeypatch):
     assert again[0] <= 2        # from a converged potential one or two cycles suffice
 
 
+@pytest.mark.parametrize("n", [64, 48])
 @pytest.mark.parametrize("prec", [P.PREC_FP64, P.PREC_FP32])
-def test_one_pass_iteration_equals_half_sweeps(prec, monkeypatch):
+def test_one_pass_iteration_equals_half_sweeps(prec, n, monkeypatch):
     """The one-pass red-black iteration and the two in-place half sweeps form
-    the same updates in the same order: a Plummer solve at 64^3 and three fused
-    steps after it give bitwise equal potentials and states with the 64^3 and
-    32^3 levels on either kernel."""
-    p, u0, phi0 = _plummer(64, prec)
+    the same updates in the same order: a Plummer solve and three fused steps
+    after it give bitwise equal potentials and states with the levels of at
+    least 32^3 cells on either kernel.  At 64^3 the one-pass tiles (32 x 8)
+    cover the 64^3 and 32^3 levels exactly; at 48^3 the 48^3 level ends in a
+    half tile in x."""
+    p, u0, phi0 = _plummer(n, prec)
     out = {}
     for value in ONEPASS:
         _onepass(monkeypatch, value)
