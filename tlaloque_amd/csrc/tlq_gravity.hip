@@ -151,7 +151,7 @@ __global__ __launch_bounds__(ZX *ZY) void k_sg_rbz(
     const R *__restrict__ U, Grid g, double fpg, int nx, int ny, int nz, double dxl,
     double *__restrict__ partial) {
     constexpr int OX = ZX + 4, OY = ZY + 4, NX = ZX + 2, NY = ZY + 2, T = ZX * ZY;
-    static_assert(OX * OY <= 2 * T && NX * NY <= 2 * T, "two ring cells per thread");
+    static_assert(OX * OY <= 2 * T, "two ring-2 cells per thread");
     __shared__ double O[4][OY][OX];   // old planes, ring 2 (four: see the loop's end)
     __shared__ double N[3][NY][NX];   // new planes, ring 1
     __shared__ double red[T / 64];
