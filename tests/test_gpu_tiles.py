@@ -11,6 +11,9 @@ user_boundaries, boundaries.f90:11-24), tabulated cooling and a softened
 point mass; the GPU is compared with the FP64 oracle (the reference
 restatement, bit for bit on the reference's own runs).
 
+The FP64 fused path runs with each compiled solver and limiter instance
+(HLLC and HLL, van Leer and the run-time limiters).
+
 Bars: FP64 fused and per-routine paths, max relative difference per field
 <= 1e-12 (measured 2.8e-15 and 1.2e-16: the device's exp/log10/pow in the
 cooling may differ from glibc's in the last bits); FP32 path, L1 of rho, p
@@ -114,8 +117,14 @@ def test_tile_problem_exercises_the_sources():
                 assert rho1[k + 1, j + 1, i + 1] > 10 * np.median(rho1)
 
 
-def test_fused_fp64_matches_oracle_across_tiles():
+# the fused kernels are compiled per solver and for van Leer apart from the
+# other limiters (read at run time): every instance crosses the tile faces
+@pytest.mark.parametrize("solver,limiter", [
+    (P.SOLVER_HLLC, P.LIMITER_VANLEER), (P.SOLVER_HLL, P.LIMITER_VANLEER),
+    (P.SOLVER_HLLC, P.LIMITER_MINMOD), (P.SOLVER_HLL, P.LIMITER_SUPERBEE)])
+def test_fused_fp64_matches_oracle_across_tiles(solver, limiter):
     p, u0 = tile_problem(*FP64_MESH, FP64_WIND, P.PREC_FP64)
+    p = p.replace(solver=solver, limiter=limiter)
     o = _oracle(p, u0, STEPS)
     with HydroCore(p, cooling_table=cooling_table()) as h:
         h.set_state(u0)
@@ -124,7 +133,7 @@ def test_fused_fp64_matches_oracle_across_tiles():
         u = h.download("U")
     assert it == STEPS and t == pytest.approx(o.t, rel=1e-12)
     err = max_rel(interior(prims_from_cons(u, p.gamma)), interior(o.prim))
-    print(f"fused FP64 72x20x136: max rel {err:.3e}")
+    print(f"fused FP64 72x20x136 solver {solver} limiter {limiter}: max rel {err:.3e}")
     assert err <= 1e-12
 
 
