@@ -21,6 +21,7 @@ import pytest
 from helpers import (cooling_table, interior, l1_rel, load_ref, max_rel, prims_from_cons,
                      ref_cases)
 from tlaloque_amd import params as P
+from tlaloque_amd import problems
 from tlaloque_amd.hydro import HydroCore
 
 pytestmark = pytest.mark.gpu
@@ -172,6 +173,48 @@ def test_fused_hll_fp64_matches_oracle(case):
         h.advance(3)
         w = prims_from_cons(h.download("U"), p.gamma)
     assert max_rel(interior(w), interior(o.prim)) <= 1e-12, case
+
+
+def _small_problem(shape, bc, precision, seed=11):
+    nx, ny, nz = shape
+    p = P.HydroParams(nx=nx, ny=ny, nz=nz, xl=0.0, xr=float(nx), yl=0.0, yr=float(ny), zl=0.0,
+                      zr=float(nz), gamma=1.4, cfl=0.4, eta=0.05, bc=(bc,) * 6,
+                      precision=precision)
+    rng = np.random.default_rng(seed)
+    rho = 1.0 + 0.5 * rng.uniform(-1, 1, p.shape)
+    vel = 0.3 * rng.uniform(-1, 1, p.shape + (3,))
+    pres = 1.0 + 0.5 * rng.uniform(-1, 1, p.shape)
+    return p, problems.cons_from_prims(p, rho, vel, pres)
+
+
+@pytest.mark.parametrize("shape", [(2, 2, 2), (3, 2, 5), (2, 9, 2)])
+@pytest.mark.parametrize("bc", [P.BC_OUTFLOW, P.BC_REFLECTIVE, P.BC_PERIODIC])
+def test_smallest_meshes_match_oracle(shape, bc):
+    """The smallest meshes tlq_create accepts (two cells on an axis): every
+    interior cell is a boundary cell, and the two ghost layers of the
+    boundary fill (boundaries.f90:432-533) outnumber the interior.  FP64 fused
+    path against the oracle, 3 steps; the FP32 path stays finite and within
+    1e-5 of it, field by field relative to the field's largest value (measured
+    up to 4.8e-7).  The reflective boxes pin the FP32 van Leer slope's
+    symmetry: with a slope that differed by an ulp between lim_avg(a, b) and
+    lim_avg(b, a), the face states beside a wall were no longer mirror images
+    and the pressure there was off by 1.2e-4 in the 2 x 2 x 2 box."""
+    p, u0 = _small_problem(shape, bc, P.PREC_FP64)
+    o = _oracle_steps(p, u0, 3)
+    with _core(p) as h:
+        h.set_state(u0)
+        h.advance(3)
+        w = prims_from_cons(h.download("U"), p.gamma)
+    assert max_rel(interior(w), interior(o.prim)) <= 1e-12, (shape, bc)
+    p32, _ = _small_problem(shape, bc, P.PREC_FP32)
+    with _core(p32) as h:
+        h.set_state(u0)
+        h.advance(3)
+        assert h.check_state() == 0
+        w32 = prims_from_cons(h.download("U"), p.gamma)
+    e32 = max_rel(interior(w32), interior(o.prim))
+    print(shape, bc, "FP32 max rel", e32)
+    assert e32 <= 1e-5, (shape, bc)
 
 
 def test_routines_hll_fp64_match_oracle():

@@ -180,14 +180,22 @@ template <class R> __device__ __forceinline__ R lim_avg(int opt, R a, R b) {
             return a * b * (a + b) / (a * a + b * b);
         } else {
             // a b (a+b)/(a^2+b^2) = (a+b) r/(1+r^2) = (a+b) / (r + 1/r) with
-            // r = a/b: a ratio, so no product underflows for cgs density
-            // differences (~1e-25) and none overflows.  r = max(a/b, 0)
-            // gives the zero slope where a b <= 0 without a select (a NaN
-            // ratio, a = b = 0, goes to 0 as well), and r = 0 or r = inf
-            // (b = 0, or a quotient past the FP32 range) gives 1/(r + 1/r)
-            // = 0.  Three hardware reciprocals, no division.
-            const R r = fmaxf(a * __builtin_amdgcn_rcpf(b), 0.f);
-            return (a + b) * __builtin_amdgcn_rcpf(r + __builtin_amdgcn_rcpf(r));
+            // r the ratio of the smaller to the larger magnitude: a ratio, so
+            // no product underflows for cgs density differences (~1e-25) and
+            // none overflows.  Taking the ratio in magnitude order keeps the
+            // result bitwise symmetric in (a, b), as the product form is: at
+            // a reflective wall the ghost's slope of the normal velocity is
+            // lim_avg(b, a) against the cell's lim_avg(a, b), and only equal
+            // slopes give mirror-image face states (with a/b and b/a the two
+            // differed by an ulp and HLLC left a 1e-5 pressure error beside
+            // the wall).  r = max(lo/hi, 0) gives the zero slope where a b <= 0
+            // without a select (a NaN ratio, a = b = 0, goes to 0 as well),
+            // and r = 0 gives 1/(r + 1/r) = 0.  Three hardware reciprocals, no
+            // division.
+            const bool sw = __builtin_fabsf(a) > __builtin_fabsf(b);
+            const R lo = sw ? b : a, hi = sw ? a : b;
+            const R r = fmaxf(lo * __builtin_amdgcn_rcpf(hi), 0.f);
+            return (lo + hi) * __builtin_amdgcn_rcpf(r + __builtin_amdgcn_rcpf(r));
         }
     case LIM_MINMOD:
         s = __builtin_copysign(R(1), a);
