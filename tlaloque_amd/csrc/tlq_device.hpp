@@ -179,23 +179,24 @@ template <class R> __device__ __forceinline__ R lim_avg(int opt, R a, R b) {
             if (a * b <= 0.0) return 0.0;
             return a * b * (a + b) / (a * a + b * b);
         } else {
-            // a b (a+b)/(a^2+b^2) = (a+b) r/(1+r^2) = (a+b) / (r + 1/r) with
-            // r the ratio of the smaller to the larger magnitude: a ratio, so
+            // No contraction here: fma(a, 1/b, b/a) would round the two terms
+            // of s differently and lose the symmetry below.
+#pragma clang fp contract(off)
+            // a b (a+b)/(a^2+b^2) = (a+b) / s with s = a/b + b/a: ratios, so
             // no product underflows for cgs density differences (~1e-25) and
-            // none overflows.  Taking the ratio in magnitude order keeps the
-            // result bitwise symmetric in (a, b), as the product form is: at
-            // a reflective wall the ghost's slope of the normal velocity is
+            // none overflows.  Both ratios are formed, so swapping a and b
+            // swaps the two terms of s and the result is bitwise symmetric in
+            // (a, b), as the product form is (and odd, as it is).  At a
+            // reflective wall the ghost's slope of the normal velocity is
             // lim_avg(b, a) against the cell's lim_avg(a, b), and only equal
-            // slopes give mirror-image face states (with a/b and b/a the two
-            // differed by an ulp and HLLC left a 1e-5 pressure error beside
-            // the wall).  r = max(lo/hi, 0) gives the zero slope where a b <= 0
-            // without a select (a NaN ratio, a = b = 0, goes to 0 as well),
-            // and r = 0 gives 1/(r + 1/r) = 0.  Three hardware reciprocals, no
-            // division.
-            const bool sw = __builtin_fabsf(a) > __builtin_fabsf(b);
-            const R lo = sw ? b : a, hi = sw ? a : b;
-            const R r = fmaxf(lo * __builtin_amdgcn_rcpf(hi), 0.f);
-            return (lo + hi) * __builtin_amdgcn_rcpf(r + __builtin_amdgcn_rcpf(r));
+            // slopes give mirror-image face states: with r = a/b and 1/r in
+            // place of b/a the two differed by an ulp, and HLLC left a 1e-5
+            // pressure error beside the wall.  Where a b < 0, s < 0 and the
+            // max gives the zero slope; a = 0 or b = 0 makes s infinite (1/s =
+            // 0), and a = b = 0 a NaN that the max takes to 0.  Three hardware
+            // reciprocals, no division, no select.
+            const R s = a * __builtin_amdgcn_rcpf(b) + b * __builtin_amdgcn_rcpf(a);
+            return (a + b) * fmaxf(__builtin_amdgcn_rcpf(s), 0.f);
         }
     case LIM_MINMOD:
         s = __builtin_copysign(R(1), a);
